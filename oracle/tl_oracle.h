@@ -1,0 +1,107 @@
+/* CPU restatement of the reference's extended-source light-curve path
+ * (LightCurveIRS, src/LightCurves/lcirs.cc) in plain C.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker the HIP engine is
+ * compared against: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  The product (triplelens_amd/) never links,
+ * loads or calls it.
+ *
+ * Every entry point returns 0 on success and a negative value on bad input.
+ */
+#ifndef TL_ORACLE_H
+#define TL_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tlo_lc tlo_lc;
+
+/* LightCurveIRS constructor (lcirs.cc:6-37): lens, critical curves (500
+ * steps), cumulative caustic boxes and the image-plane grid. */
+tlo_lc* tlo_lcirs_new(double a, double b, double th, double m2, double m3,
+                      double rho, unsigned lcLength, long ppr);
+void tlo_lcirs_delete(tlo_lc* lc);
+
+/* set_limb_darkening (lcirs.cc:579-598): model 1 = "linear" (normalised
+ * linear with parameter v), anything else = default A=0.6, B=0.4. */
+int tlo_set_limb_darkening(tlo_lc* lc, int model, double v);
+
+/* Grid: out[0..7] = N, L, step, BLx, BLy, TRx, TRy, ampScale, then the 6
+ * cumulative caustic boxes (minx, miny, maxx, maxy) at out[8..31], then
+ * out[32], out[33] = limb-darkening A, B, out[34] = rho^2 (rho*rho),
+ * out[35] = pow(rho, 2.0). */
+int tlo_grid(const tlo_lc* lc, double* out);
+
+/* getLCIRS (lcirs.cc:103-191): lcLength+1 points at t = i/lcLength.
+ * amp[i] = magnification, rays[i] = counted rays, edge[i] = 1 when the fill
+ * of point i evaluated a pixel in column <= 1 or >= N-1 (may be NULL). */
+int tlo_get_lc_irs(tlo_lc* lc, double sx, double sy, double ex, double ey,
+                   double* amp, uint64_t* rays, uint8_t* edge);
+
+/* The seeds getLCIRS fills from, per point, with the same root-reuse chain:
+ * seedXY[2*(i*maxSeeds+k)+{0,1}], nSeeds[i] (true count, may exceed
+ * maxSeeds), nImg[i] = number of point images of the centre position. */
+int tlo_seeds(tlo_lc* lc, double sx, double sy, double ex, double ey,
+              int maxSeeds, double* seedXY, int* nSeeds, int* nImg);
+
+/* Fill one source position from given seeds (image-plane coordinates), as the
+ * seed loop of getLCIRS does (lcirs.cc:168-181).  Returns rays, sum of
+ * brightness (before ampScale) and the edge flag. */
+int tlo_fill_point(const tlo_lc* lc, double px, double py, int nSeeds,
+                   const double* seedXY, uint64_t* rays, double* brightSum,
+                   int* edge);
+
+/* Critical curves / caustics (ccc.cc:42-98,160-203), 6*len complex values,
+ * root-major (copy_cc_ca layout). */
+int tlo_ccc(double a, double b, double th, double m2, double m3, int len,
+            double* cc, double* ca);
+
+/* Degree-10 lens polynomial at source position (zr, zi): 11 complex
+ * coefficients, lowest order first, in the sign convention of the reference
+ * (leading coefficient -prod(conj(zeta) - conj(z_k))). */
+int tlo_coeffs(double a, double b, double th, double m2, double m3,
+               double zr, double zi, double* out);
+
+/* Laguerre::solveRoots (laguerre.cc:103-167).  init == NULL: fresh solve;
+ * else deg+1 values (the last fills the slot the reference reads one past
+ * the end of the initial-root vector). */
+int tlo_laguerre_solve(int deg, const double* coeffs, const double* init,
+                       double* roots);
+/* Laguerre::laguerre (laguerre.cc:18-99) on one starting point; returns 1
+ * when converged.  x is updated in place (2 doubles). */
+int tlo_laguerre_one(int deg, const double* coeffs, double* x);
+
+/* ImgPoint::getImages chain over n positions (imgpoint.cc:208-274):
+ * roots 10 complex per position, isImg 10 flags, nImg per position. */
+int tlo_images(double a, double b, double th, double m2, double m3, int n,
+               const double* pos, double* roots, int* isImg, int* nImg);
+
+/* Squared source-plane distance of one ray, in the operation order of
+ * irsSIMD (lcirs.cc:290-341). */
+double tlo_irs(const tlo_lc* lc, double x, double y, double sx, double sy);
+
+/* Point-source magnification (LightCurveBase::getPointAmp, lcbase.cc:17-38)
+ * along n positions with one root chain. */
+int tlo_point_amp(double a, double b, double th, double m2, double m3, int n,
+                  const double* pos, double* out);
+
+/* The complex-arithmetic helpers, exposed for tests against libm/libgcc. */
+double tlo_hypot(double x, double y);
+void tlo_csqrt(double re, double im, double* out);
+void tlo_cdiv(double a, double b, double c, double d, double* out);
+
+/* Multi-threaded batch of independent light curves (CPU baseline):
+ * curve k uses params[8*k .. 8*k+7] = a, b, th, m2, m3, rho, ppr, v and
+ * traj[4*k .. 4*k+3] = sx, sy, ex, ey, with lens[k] points; results go to
+ * amp/rays at offset off[k] (lens[k]+1 values each).  nthreads workers. */
+int tlo_batch(int ncurves, const double* params, const double* traj,
+              const unsigned* lens, const uint64_t* off, int nthreads,
+              double* amp, uint64_t* rays);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
