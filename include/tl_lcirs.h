@@ -1,0 +1,126 @@
+/* C ABI of the MI355X IRS light-curve engine (liblcirs.so).
+ *
+ * The first group of entry points is the reference's own LightCurveIRS ABI,
+ * with the same names, argument types and meaning, so that the reference's
+ * ctypes class LC_irs (Examples/ctypes_classes.py:128-193) drives this
+ * library unchanged.  Every function is safe to call from C or ctypes: no C++
+ * exception crosses the boundary.  Where the reference has no error channel
+ * (its functions return void) failures are recorded and can be read with
+ * tl_last_error().
+ *
+ * Threading: one object must not be used from two threads at once (as in the
+ * reference); different objects may be used concurrently; the GPU work of all
+ * objects goes through one engine per process and is serialised there.
+ */
+#ifndef TL_LCIRS_H
+#define TL_LCIRS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- reference ABI (src/LightCurves/lcirs.cc:515-609) -------------------- */
+
+/* LightCurveIRS constructor (lcirs.cc:517-536).  Lens a, b, th, m2, m3 in the
+ * reference frame (z1 = 0, z2 = a, z3 = b e^{i th}), source radius, lcLength
+ * and pointsPerRadius.  Builds the critical curves, caustic boxes and
+ * image-plane grid on the host.  Returns NULL on invalid input (the reference
+ * has no such check). */
+void* lcirs_new(double a, double b, double th, double m2, double m3, double sourceSize,
+                unsigned lcLength, long pointsPerRadius);
+
+/* Point-source light curve (lcirs.cc:539-549 -> LightCurveBase::getLC,
+ * lcbase.cc:40-52): lcLength positions at t = i/(lcLength-1). */
+void get_lc(void* lc, double iniX, double iniY, double finX, double finY);
+
+/* Extended-source light curve by inverse ray shooting (lcirs.cc:552-562 ->
+ * LightCurveIRS::getLCIRS, lcirs.cc:103-191): lcLength+1 positions at
+ * t = i/lcLength, computed on the GPU. */
+void get_lc_irs(void* lc, double iniX, double iniY, double finX, double finY);
+
+/* Copies the first lcLength values of the last light curve (lcirs.cc:566-576). */
+void copy_lc(void* lc, double* lcArray);
+
+/* Limb darkening (lcirs.cc:579-598): "linear" selects the normalised linear
+ * profile with parameter v, anything else the default A = 0.6, B = 0.4. */
+void set_limb_darkening(void* lc, const char* model, double v);
+
+/* Amoeba and parameter print-out (lcirs.cc:601-607): after each get_lc_irs,
+ * point i's runs are written to "<amoebaPrefix><i>" as "ny,xleft,xright"
+ * lines (amoeba.cc:90-108) and the grid parameters to parFile as JSON
+ * (lcirs.cc:479-512). */
+void set_amoeba_printout(void* lc, const char* amoebaPrefix, const char* parFile);
+
+/* ---- additions (no counterpart in the reference) ------------------------ */
+
+/* Frees an object (the reference never frees them). */
+void lcirs_delete(void* lc);
+
+/* Per-point results of the last get_lc_irs, lcLength+1 values each:
+ * magnifications (all of them, copy_lc stops at lcLength), ray counts and
+ * status flags (bit values in TL_FLAG_*).  Return 0, or -1 if there is no
+ * light curve yet. */
+int lcirs_copy_lc_full(void* lc, double* out);
+int lcirs_copy_ray_counts(void* lc, uint64_t* out);
+int lcirs_copy_flags(void* lc, uint32_t* out);
+
+/* Grid parameters: out[0..7] = N, L, step, BLx, BLy, TRx, TRy, ampScale, the
+ * 6 cumulative caustic boxes at out[8..31] (minx, miny, maxx, maxy), then
+ * out[32] = A, out[33] = B of the limb-darkening profile, out[34] = rho^2,
+ * out[35] = pow(rho, 2). */
+int lcirs_get_grid(void* lc, double* out);
+
+/* Critical curves and caustics of the object, 6*500 complex values each,
+ * root-major (as the reference's copy_cc_ca lays them out, ccc.cc:280-294). */
+int lcirs_copy_cc_ca(void* lc, double* cc, double* ca);
+
+/* Many light curves in one engine run: objects lcs[k] with trajectories
+ * traj[4k..4k+3] = iniX, iniY, finX, finY.  Afterwards each object holds its
+ * light curve as after get_lc_irs.  Returns 0 or a negative error. */
+int lcirs_batch_irs(int n, void** lcs, const double* traj);
+
+/* Seeds of the last light curve's positions as the GPU seed kernel found
+ * them: for point i, nSeeds[i] pixel pairs at pix[2*(i*64 + k) + {0,1}].
+ * Returns 0 or -1. */
+int lcirs_copy_seeds(void* lc, int* nSeeds, int* pix);
+
+/* Fills given seeds on the GPU (no seed search): n source positions pos[2i..],
+ * seeds as pixel pairs pix[2*(i*64+k)+{0,1}] for k < nSeeds[i].  Writes ray
+ * counts, magnifications and flags per position.  For tests of the fill. */
+int lcirs_fill_seeds(void* lc, int n, const double* pos, const int* nSeeds, const int* pix,
+                     uint64_t* rays, double* amp, uint32_t* flags);
+
+/* Engine counters of the last run: out[0] fill tasks, out[1] evaluated
+ * pixels, out[2] runs inserted, out[3] duplicate runs, out[4] seed-kernel
+ * milliseconds, out[5] fill milliseconds (sum over launches), out[6] fill
+ * launches, out[7] total milliseconds of the last run, out[8] summed busy
+ * time of the fill's wavefronts (ms), out[9] of that, time in run-table
+ * inserts (ms), out[10] idle polls, out[11] wavefronts of the last launch.
+ * out must hold 12 values. */
+int tl_last_stats(double* out);
+
+/* Engine settings: name/value pairs ("fill_workgroups", "probe_limit",
+ * "queue_log2", "scout_rows", "scout_jump", "time_limit_s").  Returns 0 or
+ * -1 for an unknown name. */
+int tl_set_option(const char* name, double value);
+
+/* Description of the last error on this thread ("" if none). */
+const char* tl_last_error(void);
+
+/* Device name and compute-unit count of the engine's GPU; 0 or -1. */
+int tl_device_info(char* name, int len, int* cus);
+
+#define TL_FLAG_EDGE 1u
+#define TL_FLAG_SEED_OVERFLOW 2u
+#define TL_FLAG_HASH_FULL 4u
+#define TL_FLAG_LABEL_FULL 8u
+#define TL_FLAG_QUEUE_FULL 16u
+#define TL_FLAG_TIMEOUT 32u
+#define TL_FLAG_ROW_OVERFLOW 64u
+
+#ifdef __cplusplus
+}
+#endif
+#endif

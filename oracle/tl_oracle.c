@@ -237,7 +237,11 @@ static int lag_iter(const cx* poly, int m, cx* xp) {
     if (cabs_(gp) < cabs_(gm)) gp = gm;
     cx dx;
     if (cabs_(gp) > 0.0) dx = cdiv_rc(md, gp);
-    else dx = cscale(mk(cos(1.0 * i), sin(1.0 * i)), 1.0 + cabs_(x));
+    else {
+      double sn, cs;
+      sincos(1.0 * i, &sn, &cs);
+      dx = cscale(mk(cs, sn), 1.0 + cabs_(x));
+    }
     cx x1 = csub(x, dx);
     if (x.re == x1.re && x.im == x1.im) { *xp = x; return 1; }
     if (i % LAG_MT) x = x1;
@@ -327,7 +331,9 @@ static void lens_init(lens_t* L, double a, double b, double th, double m2, doubl
   L->m2 = m2; L->m3 = m3;
   L->z1 = mk(0.0, 0.0);
   L->z2 = mk(a, 0.0);
-  L->z3 = mk(b * cos(th), b * sin(th));
+  double sn, cs;
+  sincos(th, &sn, &cs); /* g++ merges the reference's cos/sin pair into sincos */
+  L->z3 = mk(b * cs, b * sn);
 }
 
 /* ---- degree-10 polynomial ---------------------------------------------- */
@@ -572,7 +578,9 @@ static void ccc_compute(const lens_t* L, int len, cx* cc, cx* ca) {
   int have = 0;
   for (int j = 0; j < len; j++) {
     double phi = 2.0 * 3.14159265359 * j / len;
-    cx e = mk(cos(phi), sin(phi));
+    double sn, cs;
+    sincos(phi, &sn, &cs);
+    cx e = mk(cs, sn);
     p[6] = cneg(e);
     p[5] = cmul(cscale(e, 2.0), z2pz3);
     /* 1.0 - complex: real 1.0 - re, imag -im (libstdc++ operator-(T, complex)) */
@@ -912,6 +920,7 @@ static int flood(tlo_lc* lc, fillacc_t* acc, long nx0, long ny0) {
   return 0;
 }
 
+static void amoeba_stats(const tlo_lc* lc);
 static int fill_seeds(tlo_lc* lc, double px, double py, int n, const cx* seeds,
                       uint64_t* rays, double* sum, int* edge) {
   if (fill_reserve(lc)) return -1;
@@ -919,12 +928,31 @@ static int fill_seeds(tlo_lc* lc, double px, double py, int n, const cx* seeds,
   fillacc_t acc = {lc, px, py, 0.0, 0, 0};
   for (int k = 0; k < n; k++)
     if (flood(lc, &acc, x_to_nx(lc, seeds[k].re), y_to_ny(lc, seeds[k].im))) return -1;
+  amoeba_stats(lc);
   amoeba_reset(lc);
   *rays = acc.count;
   *sum = acc.sum;
   if (edge) *edge = acc.edge;
   return 0;
 }
+
+/* Amoeba statistics of the last fill_seeds call (test instrumentation):
+ * out[0] = ranges, out[1] = rows with ranges, out[2] = max ranges in a row,
+ * out[3] = min row, out[4] = max row. */
+static long g_stats[5];
+static void amoeba_stats(const tlo_lc* lc) {
+  long mx = 0, mn = -1, mxr = -1;
+  for (long i = 0; i < lc->nTouched; i++) {
+    long row = lc->touched[i], c = 0;
+    for (int32_t r = lc->head[row]; r >= 0; r = lc->rnext[r]) c++;
+    if (c > mx) mx = c;
+    if (mn < 0 || row < mn) mn = row;
+    if (row > mxr) mxr = row;
+  }
+  g_stats[0] = lc->nRanges; g_stats[1] = lc->nTouched; g_stats[2] = mx;
+  g_stats[3] = mn; g_stats[4] = mxr;
+}
+int tlo_fill_stats(long* out) { for (int i = 0; i < 5; i++) out[i] = g_stats[i]; return 0; }
 
 int tlo_fill_point(const tlo_lc* clc, double px, double py, int nSeeds,
                    const double* seedXY, uint64_t* rays, double* brightSum, int* edge) {

@@ -1,0 +1,837 @@
+// Host side of the IRS engine: LightCurveIRS objects (lens, critical curves,
+// caustic boxes, image-plane grid), the GPU engine that runs the seed and fill
+// kernels, and the C ABI declared in include/tl_lcirs.h.
+//
+// Set-up follows the reference constructor (src/LightCurves/lcirs.cc:6-92,
+// src/CCC/ccc.cc:42-203) with the same IEEE operations, so the grid (N,
+// corners, pixel size) is the reference's to the last bit.  Everything per
+// source position runs on the GPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <mutex>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "tl_engine.h"
+#include "../../include/tl_lcirs.h"
+
+using namespace tl;
+
+namespace {
+
+thread_local std::string g_err;
+
+void set_err(const std::string& s) {
+  g_err = s;
+  if (std::getenv("TL_VERBOSE")) std::fprintf(stderr, "liblcirs: %s\n", s.c_str());
+}
+
+#define HIPCHK(x)                                                                  \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess)                                                          \
+      throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_));    \
+  } while (0)
+
+TrigTable host_trig() {
+  TrigTable t;
+  for (int i = 0; i <= kLagMaxIt; i++) host_sincos(1.0 * i, &t.s[i], &t.c[i]);
+  return t;
+}
+const TrigTable& trig() {
+  static const TrigTable t = host_trig();
+  return t;
+}
+
+// ---------------------------------------------------------------------------
+// Light-curve object
+// ---------------------------------------------------------------------------
+struct Curve {
+  double a, b, th, m2, m3, rho;
+  unsigned lcLength;
+  long ppr;
+  Lens L;
+  PolyBase pb;
+  double rho2, rho2pow, ldA, ldB, intensity, ampScale;
+  long long N;
+  double Ldbl, step, blx, bly, trx, try_;
+  cx boxMin[6], boxMax[6];
+  std::vector<cx> cc, ca;      // 6 x kCccLen, root-major
+  RootChain chain;             // root-reuse state carried between calls
+  std::vector<double> lcVec;   // lcLength values (copy_lc)
+  std::vector<double> full;    // lcLength + 1 values of the last IRS curve
+  std::vector<uint64_t> rays;
+  std::vector<uint32_t> flags;
+  std::vector<int> nSeeds, seedPix;
+  bool hasIRS = false;
+  bool amoebaOut = false;
+  std::string amoebaPrefix, parFile;
+};
+
+// utils.h:16-28
+void minmax_(const cx* v, int n, cx* mn, cx* mx) {
+  for (int i = 0; i < n; i++) {
+    if (v[i].re < mn->re) mn->re = v[i].re;
+    if (v[i].im < mn->im) mn->im = v[i].im;
+    if (v[i].re > mx->re) mx->re = v[i].re;
+    if (v[i].im > mx->im) mx->im = v[i].im;
+  }
+}
+
+// CriticalCurveCaustic::getCC + getCa (ccc.cc:42-98,160-203): the degree-6
+// critical-curve polynomial in the reference's operation order, solved at
+// phi = 2*3.14159265359*j/len with root polishing continued from step to step.
+void critical_curves(const Lens& L, std::vector<cx>& cc, std::vector<cx>& ca) {
+  const int len = kCccLen;
+  cc.assign(6 * len, mk(0, 0));
+  ca.assign(6 * len, mk(0, 0));
+  const cx z2 = L.z2, z3 = L.z3;
+  const double m2 = L.m2, m3 = L.m3;
+  cx p[7];
+  const cx z2pz3 = cadd(z2, z3);
+  p[1] = cscale(cmul(cmul(cscale(z2, 2.0), z3), z2pz3), m2 + m3 - 1.0);
+  p[0] = cscale(cmul(cmul(cmul(cneg(z2), z2), z3), z3), m2 + m3 - 1.0);
+  cx roots[6];
+  bool have = false;
+  for (int j = 0; j < len; j++) {
+    const double phi = 2.0 * 3.14159265359 * j / len;
+    double sn, cs;
+    host_sincos(phi, &sn, &cs);
+    const cx e = mk(cs, sn);
+    p[6] = cneg(e);
+    p[5] = cmul(cscale(e, 2.0), z2pz3);
+    const cx q4 = cmul(e, cadd(cadd(cmul(z2, z2), cmul(z3, z3)), cmul(cscale(z2, 4.0), z3)));
+    p[4] = mk(1.0 - q4.re, -q4.im);
+    p[3] = cadd(cadd(cmul(cmul(cmul(cscale(e, 2.0), z2), z3), z2pz3),
+                     cscale(cscale(z2, 2.0), m2 - 1.0)),
+                cscale(cscale(z3, 2.0), m3 - 1.0));
+    const cx t1 = cmul(cmul(cmul(cmul(cneg(e), z2), z2), z3), z3);
+    const cx t2 = cmul(cscale(z2, 4.0 * (1.0 - m3 - m2)), z3);
+    const cx t3 = cscale(cmul(z2, z2), m2 - 1.0);
+    const cx t4 = cscale(cmul(z3, z3), m3 - 1.0);
+    p[2] = csub(csub(cadd(t1, t2), t3), t4);
+    if (!have) {
+      lag_solve<6>(p, nullptr, roots, &trig());
+      have = true;
+    } else {
+      lag_polish<6>(p, roots, &trig());
+      if (!lag_check<6>(p, roots)) lag_solve<6>(p, nullptr, roots, &trig());
+    }
+    for (int k = 0; k < 6; k++) cc[k * len + j] = roots[k];
+  }
+  for (int k = 0; k < 6; k++)
+    for (int j = 0; j < len; j++) {
+      const cx s = cc[k * len + j];
+      cx v = csub(s, cdiv_rc(L.m1, cconj(csub(s, L.z1))));
+      v = csub(v, cdiv_rc(L.m2, cconj(csub(s, L.z2))));
+      v = csub(v, cdiv_rc(L.m3, cconj(csub(s, L.z3))));
+      ca[k * len + j] = v;
+    }
+}
+
+void ld_default(Curve& c) {
+  c.ldA = 0.6;
+  c.ldB = 0.4;
+  c.intensity = 1.0 / (c.ldA + 2.0 / 3.0 * c.ldB);
+}
+
+Curve* make_curve(double a, double b, double th, double m2, double m3, double rho,
+                  unsigned lcLength, long ppr) {
+  if (!(rho > 0.0) || !std::isfinite(rho)) throw std::invalid_argument("sourceSize must be > 0");
+  if (ppr <= 0) throw std::invalid_argument("pointsPerRadius must be > 0");
+  if (lcLength == 0) throw std::invalid_argument("lcLength must be > 0");
+  for (double v : {a, b, th, m2, m3})
+    if (!std::isfinite(v)) throw std::invalid_argument("lens parameters must be finite");
+  if (m3 == 0.0) throw std::invalid_argument("m3 == 0 (binary lens) is not supported");
+  auto* c = new Curve();
+  c->a = a; c->b = b; c->th = th; c->m2 = m2; c->m3 = m3; c->rho = rho;
+  c->lcLength = lcLength;
+  c->ppr = ppr;
+  c->L = make_lens(a, b, th, m2, m3);
+  poly_base(c->L, &c->pb);
+  c->rho2 = rho * rho;
+  c->rho2pow = std::pow(rho, 2.0);
+  ld_default(*c);
+  critical_curves(c->L, c->cc, c->ca);
+  // _getCaBoxes (lcirs.cc:39-48): cumulative, origin included
+  cx mn = mk(0, 0), mx = mk(0, 0);
+  for (int k = 0; k < 6; k++) {
+    minmax_(c->ca.data() + k * kCccLen, kCccLen, &mn, &mx);
+    c->boxMin[k] = mn;
+    c->boxMax[k] = mx;
+  }
+  // _getImgPlanePars (lcirs.cc:51-92)
+  mn = mk(0, 0);
+  mx = mk(0, 0);
+  for (int k = 0; k < 6; k++) minmax_(c->cc.data() + k * kCccLen, kCccLen, &mn, &mx);
+  double halfEdge = mx.re - mn.re;
+  if (mx.im - mn.im > halfEdge) halfEdge = mx.im - mn.im;
+  halfEdge *= 3.0 / 2.0;
+  const double cxr = (mx.re + mn.re) / 2.0, cxi = (mx.im + mn.im) / 2.0;
+  c->trx = cxr + halfEdge;
+  c->try_ = cxi + halfEdge;
+  c->blx = cxr - halfEdge;
+  c->bly = cxi - halfEdge;
+  const double Nd = 2 * halfEdge / rho * (double)ppr;
+  if (!(Nd >= 3.0) || Nd > (double)((1ll << 25) - 2)) {
+    delete c;
+    throw std::invalid_argument("image-plane grid size out of range (3 .. 2^25-2 pixels)");
+  }
+  c->N = (long long)Nd;
+  c->Ldbl = 2 * halfEdge;
+  c->ampScale = c->intensity / M_PI / std::pow((double)ppr, 2.0);
+  c->step = c->Ldbl / (double)((double)c->N - 1.0);
+  for (int k = 0; k < 11; k++) c->chain.tempRoots[k] = mk(0, 0);
+  c->chain.have = 0;
+  c->lcVec.assign(lcLength, 0.0);
+  return c;
+}
+
+// ---------------------------------------------------------------------------
+// Engine
+// ---------------------------------------------------------------------------
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void ensure(size_t k) {
+    if (k <= n) return;
+    if (p) HIPCHK(hipFree(p));
+    p = nullptr;
+    n = 0;
+    size_t want = std::max<size_t>(k, 16);
+    HIPCHK(hipMalloc(&p, want * sizeof(T)));
+    n = want;
+  }
+};
+
+struct Options {
+  int fillWorkgroups = 0;  // 0: 2 per compute unit (8 resident waves)
+  int probeLimit = 4096;
+  int queueLog2 = 21;
+  int scoutRows = 0;
+  int scoutJump = 256;
+  double timeLimitS = 60.0;
+};
+
+struct Stats {
+  double tasks = 0, evals = 0, ins = 0, dup = 0, seedMs = 0, fillMs = 0, fillLaunches = 0,
+         totalMs = 0, busyMs = 0, insertMs = 0, idlePolls = 0, waves = 0;
+};
+
+struct Job {          // one engine run over a set of curves
+  std::vector<Curve*> curves;
+  std::vector<cx> starts, ends;
+  bool runSeeds = true;
+  // when runSeeds is false (lcirs_fill_seeds): explicit positions and seeds
+  std::vector<double2> pos;
+  std::vector<int> nSeeds;
+  std::vector<int2> seeds;
+};
+
+struct Engine {
+  std::mutex mu;
+  bool ready = false;
+  int cus = 0;
+  std::string name;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  DBuf<TrigTable> dTrig;
+  DBuf<CurveDev> dCurves;
+  DBuf<cx> dCaus;
+  DBuf<int2> dSeeds;
+  DBuf<int> dNSeeds, dPtCurve, dBatchPts, dLParent, dLPoint, dInts;
+  DBuf<double2> dPtPos;
+  DBuf<float> dCost;
+  DBuf<uint32_t> dFlags;
+  DBuf<int2> dSeedTasks;
+  DBuf<unsigned long long> dHtab, dLCount, dQSeq, dCounters, dPtCount;
+  DBuf<long long> dLSum, dPending, dPtSum;
+  DBuf<Task> dQBuf;
+  DBuf<RootChain> dChain;
+  DBuf<double> dPointLC;
+  Options opt;
+  Stats stats;
+
+  void init() {
+    if (ready) return;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0)
+      throw std::runtime_error("no HIP device available: the IRS engine needs a gfx950 GPU");
+    HIPCHK(hipSetDevice(0));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, 0));
+    cus = prop.multiProcessorCount;
+    name = prop.gcnArchName;
+    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&ev0));
+    HIPCHK(hipEventCreate(&ev1));
+    dTrig.ensure(1);
+    HIPCHK(hipMemcpy(dTrig.p, &trig(), sizeof(TrigTable), hipMemcpyHostToDevice));
+    ready = true;
+  }
+
+  float elapsed() {
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+    return ms;
+  }
+
+  void run(Job& job);
+  void point_lc(Curve& c, cx s, cx e);
+};
+
+Engine& engine() {
+  static Engine e;
+  return e;
+}
+
+CurveDev to_dev(const Curve& c, cx s, cx e, int pointBase, int causticBase) {
+  CurveDev d;
+  std::memset(&d, 0, sizeof(d));
+  d.L = c.L;
+  d.pb = c.pb;
+  d.rho = c.rho;
+  d.rho2 = c.rho2;
+  d.rho2pow = c.rho2pow;
+  d.ldA = c.ldA;
+  d.ldB = c.ldB;
+  d.blx = c.blx;
+  d.bly = c.bly;
+  d.step = c.step;
+  d.Ldbl = c.Ldbl;
+  d.Nd = (double)c.N;
+  d.N = c.N;
+  d.sx = s.re; d.sy = s.im; d.ex = e.re; d.ey = e.im;
+  d.ppr = (double)c.ppr;
+  d.lcLength = c.lcLength;
+  d.pointBase = pointBase;
+  d.causticBase = causticBase;
+  d.chainHave = c.chain.have;
+  for (int k = 0; k < 11; k++) d.chain[k] = c.chain.tempRoots[k];
+  for (int k = 0; k < 6; k++) {
+    d.boxMin[k] = c.boxMin[k];
+    d.boxMax[k] = c.boxMax[k];
+  }
+  return d;
+}
+
+void Engine::run(Job& job) {
+  init();
+  auto tStart = std::chrono::steady_clock::now();
+  stats = Stats();
+  const int nc = (int)job.curves.size();
+  std::vector<CurveDev> hc(nc);
+  std::vector<int> pointBase(nc + 1, 0);
+  for (int k = 0; k < nc; k++) {
+    const int npts = job.runSeeds ? (int)job.curves[k]->lcLength + 1 : (int)job.pos.size();
+    pointBase[k + 1] = pointBase[k] + npts;
+  }
+  const int nPts = pointBase[nc];
+  if (nPts <= 0) return;
+  std::vector<int> ptCurve(nPts);
+  for (int k = 0; k < nc; k++)
+    for (int p = pointBase[k]; p < pointBase[k + 1]; p++) ptCurve[p] = k;
+  std::vector<cx> caus((size_t)nc * 6 * kCccLen);
+  for (int k = 0; k < nc; k++) {
+    hc[k] = to_dev(*job.curves[k], job.starts[k], job.ends[k], pointBase[k], k * 6 * kCccLen);
+    std::copy(job.curves[k]->ca.begin(), job.curves[k]->ca.end(), caus.begin() + (size_t)k * 6 * kCccLen);
+  }
+  dCurves.ensure(nc);
+  dCaus.ensure(caus.size());
+  dSeeds.ensure((size_t)nPts * kMaxSeeds);
+  dNSeeds.ensure(nPts);
+  dPtPos.ensure(nPts);
+  dCost.ensure(nPts);
+  dFlags.ensure(nPts);
+  dPtCurve.ensure(nPts);
+  HIPCHK(hipMemcpyAsync(dCurves.p, hc.data(), nc * sizeof(CurveDev), hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync(dCaus.p, caus.data(), caus.size() * sizeof(cx), hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync(dPtCurve.p, ptCurve.data(), nPts * sizeof(int), hipMemcpyHostToDevice, stream));
+
+  std::vector<int> nSeeds(nPts);
+  std::vector<float> cost(nPts);
+  std::vector<uint32_t> flags(nPts, 0);
+  if (job.runSeeds) {
+    SeedArgs sa;
+    sa.curves = dCurves.p;
+    sa.nCurves = nc;
+    sa.caustics = dCaus.p;
+    sa.trig = dTrig.p;
+    sa.seeds = dSeeds.p;
+    sa.nSeeds = dNSeeds.p;
+    sa.ptPos = dPtPos.p;
+    sa.cost = dCost.p;
+    sa.flags = dFlags.p;
+    dChain.ensure(nc);
+    sa.chainOut = dChain.p;
+    HIPCHK(hipEventRecord(ev0, stream));
+    HIPCHK((hipError_t)launch_seed(sa, stream));
+    HIPCHK(hipEventRecord(ev1, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    stats.seedMs = elapsed();
+    HIPCHK(hipMemcpy(nSeeds.data(), dNSeeds.p, nPts * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(cost.data(), dCost.p, nPts * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(flags.data(), dFlags.p, nPts * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    std::vector<RootChain> chains(nc);
+    HIPCHK(hipMemcpy(chains.data(), dChain.p, nc * sizeof(RootChain), hipMemcpyDeviceToHost));
+    for (int k = 0; k < nc; k++) job.curves[k]->chain = chains[k];
+  } else {
+    for (int p = 0; p < nPts; p++) {
+      nSeeds[p] = std::min(job.nSeeds[p], kMaxSeeds);
+      cost[p] = (float)nSeeds[p];
+    }
+    HIPCHK(hipMemcpy(dPtPos.p, job.pos.data(), nPts * sizeof(double2), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dSeeds.p, job.seeds.data(), (size_t)nPts * kMaxSeeds * sizeof(int2),
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dFlags.p, flags.data(), nPts * sizeof(uint32_t), hipMemcpyHostToDevice));
+  }
+
+  // Largest predicted cost first, in batches of at most kMaxBatch positions.
+  std::vector<int> order;
+  order.reserve(nPts);
+  for (int p = 0; p < nPts; p++)
+    if (nSeeds[p] > 0) order.push_back(p);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+
+  std::vector<uint64_t> ptRays(nPts, 0);
+  std::vector<long long> ptSumAll(nPts, 0);
+  const unsigned long long qcap = 1ull << opt.queueLog2;
+  dQBuf.ensure(qcap);
+  dQSeq.ensure(qcap);
+  dCounters.ensure(12);
+  dPending.ensure(1);
+  dInts.ensure(4);
+  const int nWg = opt.fillWorkgroups > 0 ? opt.fillWorkgroups : cus * 2;
+
+  for (size_t b0 = 0; b0 < order.size(); b0 += kMaxBatch) {
+    const int nb = (int)std::min<size_t>(kMaxBatch, order.size() - b0);
+    std::vector<int> batchPts(order.begin() + b0, order.begin() + b0 + nb);
+    std::vector<int2> seedTasks;
+    double predicted = 0;
+    for (int b = 0; b < nb; b++) {
+      const int gp = batchPts[b];
+      for (int k = 0; k < nSeeds[gp]; k++) seedTasks.push_back(make_int2(b, k));
+      predicted += std::max(256.0, (double)cost[gp] / 64.0);
+    }
+    dBatchPts.ensure(nb);
+    dSeedTasks.ensure(seedTasks.size());
+    dPtCount.ensure(nb);
+    dPtSum.ensure(nb);
+    HIPCHK(hipMemcpyAsync(dBatchPts.p, batchPts.data(), nb * sizeof(int), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(dSeedTasks.p, seedTasks.data(), seedTasks.size() * sizeof(int2),
+                          hipMemcpyHostToDevice, stream));
+    unsigned long long hcap = 1ull << 16;
+    while ((double)hcap < 4.0 * predicted && hcap < (1ull << 28)) hcap <<= 1;
+    int labelCap = nb + std::max(1 << 16, 16 * nb);
+    for (int attempt = 0; attempt < 4; attempt++) {
+      dHtab.ensure(2 * hcap);
+      dLParent.ensure(labelCap);
+      dLPoint.ensure(labelCap);
+      dLCount.ensure(labelCap);
+      dLSum.ensure(labelCap);
+      FillArgs fa;
+      std::memset(&fa, 0, sizeof(fa));
+      fa.curves = dCurves.p;
+      fa.ptCurve = dPtCurve.p;
+      fa.ptPos = dPtPos.p;
+      fa.batchPts = dBatchPts.p;
+      fa.nBatch = nb;
+      fa.seedTasks = dSeedTasks.p;
+      fa.nSeedTasks = (int)seedTasks.size();
+      fa.seeds = dSeeds.p;
+      fa.htab = dHtab.p;
+      fa.hmask = hcap - 1;
+      fa.probeLimit = opt.probeLimit;
+      fa.lparent = dLParent.p;
+      fa.lpoint = dLPoint.p;
+      fa.lcount = dLCount.p;
+      fa.lsum = dLSum.p;
+      fa.labelNext = dInts.p + 1;
+      fa.labelCap = labelCap;
+      fa.qbuf = dQBuf.p;
+      fa.qseq = dQSeq.p;
+      fa.qhead = dCounters.p + 0;
+      fa.qtail = dCounters.p + 1;
+      fa.qcap = qcap;
+      fa.pending = dPending.p;
+      fa.seedNext = dInts.p + 0;
+      fa.flags = dFlags.p;
+      fa.abortFlag = dInts.p + 2;
+      fa.stats = dCounters.p + 2;
+      fa.timeLimitTicks = (long long)(opt.timeLimitS * 1.0e8);
+      fa.scoutRows = opt.scoutRows;
+      fa.scoutJump = opt.scoutJump;
+      HIPCHK((hipError_t)launch_fill_reset(fa, stream));
+      HIPCHK((hipError_t)launch_clear_u64(dPtCount.p, nb, 0, stream));
+      HIPCHK((hipError_t)launch_clear_u64((unsigned long long*)dPtSum.p, nb, 0, stream));
+      HIPCHK(hipEventRecord(ev0, stream));
+      HIPCHK((hipError_t)launch_fill(fa, nWg, stream));
+      HIPCHK(hipEventRecord(ev1, stream));
+      HIPCHK((hipError_t)launch_resolve(fa, labelCap, dPtCount.p, dPtSum.p, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      stats.fillMs += elapsed();
+      stats.fillLaunches += 1;
+      unsigned long long st[10];
+      int ints[4];
+      HIPCHK(hipMemcpy(st, dCounters.p, 10 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(ints, dInts.p, 3 * sizeof(int), hipMemcpyDeviceToHost));
+      std::vector<uint32_t> bflags(nPts);
+      HIPCHK(hipMemcpy(bflags.data(), dFlags.p, nPts * sizeof(uint32_t), hipMemcpyDeviceToHost));
+      bool hashFull = false, labelFull = false;
+      for (int b = 0; b < nb; b++) {
+        hashFull |= (bflags[batchPts[b]] & kFlagHashFull) != 0;
+        labelFull |= (bflags[batchPts[b]] & kFlagLabelFull) != 0;
+      }
+      stats.tasks += (double)st[2];
+      stats.evals += (double)st[3];
+      stats.ins += (double)st[4];
+      stats.dup += (double)st[5];
+      stats.busyMs += (double)st[6] * 1e-5;
+      stats.insertMs += (double)st[7] * 1e-5;
+      stats.idlePolls += (double)st[8];
+      stats.waves = (double)st[9];
+      if (ints[2]) {
+        for (int b = 0; b < nb; b++) flags[batchPts[b]] |= kFlagTimeout;
+        throw std::runtime_error("fill kernel hit its time limit");
+      }
+      if ((hashFull || labelFull) && attempt < 3) {
+        // clear the overflow bits and run the batch again with bigger tables
+        for (int b = 0; b < nb; b++) bflags[batchPts[b]] &= ~(kFlagHashFull | kFlagLabelFull);
+        HIPCHK(hipMemcpy(dFlags.p, bflags.data(), nPts * sizeof(uint32_t), hipMemcpyHostToDevice));
+        if (hashFull) hcap = std::min<unsigned long long>(hcap << 2, 1ull << 29);
+        if (labelFull) labelCap *= 4;
+        continue;
+      }
+      std::vector<unsigned long long> cnt(nb);
+      std::vector<long long> sum(nb);
+      HIPCHK(hipMemcpy(cnt.data(), dPtCount.p, nb * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(sum.data(), dPtSum.p, nb * sizeof(long long), hipMemcpyDeviceToHost));
+      for (int b = 0; b < nb; b++) {
+        ptRays[batchPts[b]] = cnt[b];
+        ptSumAll[batchPts[b]] = sum[b];
+      }
+      break;
+    }
+  }
+  HIPCHK(hipMemcpy(flags.data(), dFlags.p, nPts * sizeof(uint32_t), hipMemcpyDeviceToHost));
+
+  // results back into the objects
+  std::vector<int2> seedsHost;
+  if (job.runSeeds) {
+    seedsHost.resize((size_t)nPts * kMaxSeeds);
+    HIPCHK(hipMemcpy(seedsHost.data(), dSeeds.p, seedsHost.size() * sizeof(int2), hipMemcpyDeviceToHost));
+  }
+  for (int k = 0; k < nc; k++) {
+    Curve& c = *job.curves[k];
+    const int n = pointBase[k + 1] - pointBase[k];
+    c.full.assign(n, 0.0);
+    c.rays.assign(n, 0);
+    c.flags.assign(n, 0);
+    for (int i = 0; i < n; i++) {
+      const int p = pointBase[k] + i;
+      c.rays[i] = ptRays[p];
+      c.full[i] = ((double)ptSumAll[p] / kSumScale) * c.ampScale;
+      c.flags[i] = flags[p];
+    }
+    if (job.runSeeds) {
+      c.nSeeds.assign(nSeeds.begin() + pointBase[k], nSeeds.begin() + pointBase[k + 1]);
+      c.seedPix.assign((size_t)n * kMaxSeeds * 2, 0);
+      std::memcpy(c.seedPix.data(), seedsHost.data() + (size_t)pointBase[k] * kMaxSeeds,
+                  (size_t)n * kMaxSeeds * sizeof(int2));
+      for (unsigned i = 0; i < c.lcLength && i < (unsigned)n; i++) c.lcVec[i] = c.full[i];
+      c.hasIRS = true;
+    }
+  }
+  stats.totalMs =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tStart).count();
+}
+
+void Engine::point_lc(Curve& c, cx s, cx e) {
+  init();
+  if (c.lcLength < 1) return;
+  CurveDev d = to_dev(c, s, e, 0, 0);
+  dCurves.ensure(1);
+  dChain.ensure(1);
+  dPointLC.ensure(c.lcLength);
+  HIPCHK(hipMemcpyAsync(dCurves.p, &d, sizeof(CurveDev), hipMemcpyHostToDevice, stream));
+  HIPCHK((hipError_t)launch_point_lc(dCurves.p, 1, dTrig.p, dPointLC.p, dChain.p, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  HIPCHK(hipMemcpy(c.lcVec.data(), dPointLC.p, c.lcLength * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(&c.chain, dChain.p, sizeof(RootChain), hipMemcpyDeviceToHost));
+}
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    g_err.clear();
+    return f();
+  } catch (const std::exception& e) {
+    set_err(e.what());
+    return -1;
+  } catch (...) {
+    set_err("unknown error");
+    return -1;
+  }
+}
+
+void write_par_file(const Curve& c) {
+  // printOutLCParameters (lcirs.cc:479-512), including its "m3" entry that
+  // prints m2
+  std::ofstream f(c.parFile);
+  const char* in = "\t";
+  f << "{" << '\n';
+  f << in << "\"sourceRadius\": " << c.rho << ",\n";
+  f << in << "\"pointsPerRadius\": " << c.ppr << ",\n";
+  f << in << "\"imgPlaneSize\": " << c.N << ",\n";
+  f << in << "\"imgPlaneSizeDouble\": " << c.Ldbl << ",\n";
+  f << in << "\"bottomLeftCornerImgX\": " << c.blx << ",\n";
+  f << in << "\"bottomLeftCornerImgY\": " << c.bly << ",\n";
+  f << in << "\"topRightCornerImgX\": " << c.trx << ",\n";
+  f << in << "\"topRightCornerImgY\": " << c.try_ << ",\n";
+  f << in << "\"a\": " << c.a << ",\n";
+  f << in << "\"b\": " << c.b << ",\n";
+  f << in << "\"th\": " << c.th << ",\n";
+  f << in << "\"m1\": " << c.L.m1 << ",\n";
+  f << in << "\"m2\": " << c.m2 << ",\n";
+  f << in << "\"m3\": " << c.m2 << ",\n";
+  f << in << "\"z1x\": " << c.L.z1.re << ",\n";
+  f << in << "\"z1y\": " << c.L.z1.im << ",\n";
+  f << in << "\"z2x\": " << c.L.z2.re << ",\n";
+  f << in << "\"z2y\": " << c.L.z2.im << ",\n";
+  f << in << "\"z3x\": " << c.L.z3.re << ",\n";
+  f << in << "\"z3y\": " << c.L.z3.im << "\n";
+  f << "}";
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+void* lcirs_new(double a, double b, double th, double m2, double m3, double sourceSize,
+                unsigned lcLength, long pointsPerRadius) {
+  Curve* c = nullptr;
+  guarded([&] {
+    c = make_curve(a, b, th, m2, m3, sourceSize, lcLength, pointsPerRadius);
+    return 0;
+  });
+  return c;
+}
+
+void lcirs_delete(void* lc) { delete static_cast<Curve*>(lc); }
+
+void get_lc_irs(void* lc, double iniX, double iniY, double finX, double finY) {
+  int rc = guarded([&] {
+    if (!lc) throw std::invalid_argument("null light-curve object");
+    Curve* c = static_cast<Curve*>(lc);
+    Engine& E = engine();
+    std::lock_guard<std::mutex> lock(E.mu);
+    Job job;
+    job.curves = {c};
+    job.starts = {mk(iniX, iniY)};
+    job.ends = {mk(finX, finY)};
+    E.run(job);
+    if (c->amoebaOut) write_par_file(*c);
+    return 0;
+  });
+  if (rc) std::fprintf(stderr, "get_lc_irs failed: %s\n", g_err.c_str());
+}
+
+void get_lc(void* lc, double iniX, double iniY, double finX, double finY) {
+  int rc = guarded([&] {
+    if (!lc) throw std::invalid_argument("null light-curve object");
+    Curve* c = static_cast<Curve*>(lc);
+    Engine& E = engine();
+    std::lock_guard<std::mutex> lock(E.mu);
+    E.point_lc(*c, mk(iniX, iniY), mk(finX, finY));
+    return 0;
+  });
+  if (rc) std::fprintf(stderr, "get_lc failed: %s\n", g_err.c_str());
+}
+
+void copy_lc(void* lc, double* out) {
+  if (!lc || !out) return;
+  const Curve* c = static_cast<Curve*>(lc);
+  std::copy(c->lcVec.begin(), c->lcVec.end(), out);
+}
+
+void set_limb_darkening(void* lc, const char* model, double v) {
+  guarded([&] {
+    if (!lc) throw std::invalid_argument("null light-curve object");
+    Curve* c = static_cast<Curve*>(lc);
+    if (model && std::strcmp(model, "linear") == 0) {
+      const double i0 = 1.0 / ((1.0 - v / 3.0));
+      c->ldA = i0 * (1.0 - v);
+      c->ldB = i0 * v;
+      c->intensity = 1.0;
+    } else {
+      ld_default(*c);
+    }
+    c->ampScale = c->intensity / M_PI / std::pow((double)c->ppr, 2.0);
+    return 0;
+  });
+}
+
+void set_amoeba_printout(void* lc, const char* amoebaPrefix, const char* parFile) {
+  guarded([&] {
+    if (!lc) throw std::invalid_argument("null light-curve object");
+    Curve* c = static_cast<Curve*>(lc);
+    c->amoebaOut = true;
+    c->amoebaPrefix = amoebaPrefix ? amoebaPrefix : "";
+    c->parFile = parFile ? parFile : "";
+    return 0;
+  });
+}
+
+int lcirs_copy_lc_full(void* lc, double* out) {
+  const Curve* c = static_cast<Curve*>(lc);
+  if (!c || !c->hasIRS) return -1;
+  std::copy(c->full.begin(), c->full.end(), out);
+  return 0;
+}
+
+int lcirs_copy_ray_counts(void* lc, uint64_t* out) {
+  const Curve* c = static_cast<Curve*>(lc);
+  if (!c || !c->hasIRS) return -1;
+  std::copy(c->rays.begin(), c->rays.end(), out);
+  return 0;
+}
+
+int lcirs_copy_flags(void* lc, uint32_t* out) {
+  const Curve* c = static_cast<Curve*>(lc);
+  if (!c || !c->hasIRS) return -1;
+  std::copy(c->flags.begin(), c->flags.end(), out);
+  return 0;
+}
+
+int lcirs_copy_seeds(void* lc, int* nSeeds, int* pix) {
+  const Curve* c = static_cast<Curve*>(lc);
+  if (!c || !c->hasIRS) return -1;
+  std::copy(c->nSeeds.begin(), c->nSeeds.end(), nSeeds);
+  std::copy(c->seedPix.begin(), c->seedPix.end(), pix);
+  return 0;
+}
+
+int lcirs_get_grid(void* lc, double* out) {
+  const Curve* c = static_cast<Curve*>(lc);
+  if (!c) return -1;
+  out[0] = (double)c->N; out[1] = c->Ldbl; out[2] = c->step;
+  out[3] = c->blx; out[4] = c->bly; out[5] = c->trx; out[6] = c->try_;
+  out[7] = c->ampScale;
+  for (int k = 0; k < 6; k++) {
+    out[8 + 4 * k] = c->boxMin[k].re; out[9 + 4 * k] = c->boxMin[k].im;
+    out[10 + 4 * k] = c->boxMax[k].re; out[11 + 4 * k] = c->boxMax[k].im;
+  }
+  out[32] = c->ldA; out[33] = c->ldB; out[34] = c->rho2; out[35] = c->rho2pow;
+  return 0;
+}
+
+int lcirs_copy_cc_ca(void* lc, double* cc, double* ca) {
+  const Curve* c = static_cast<Curve*>(lc);
+  if (!c) return -1;
+  std::memcpy(cc, c->cc.data(), c->cc.size() * sizeof(cx));
+  std::memcpy(ca, c->ca.data(), c->ca.size() * sizeof(cx));
+  return 0;
+}
+
+int lcirs_batch_irs(int n, void** lcs, const double* traj) {
+  return guarded([&] {
+    if (n < 0 || (n > 0 && (!lcs || !traj))) throw std::invalid_argument("bad batch arguments");
+    Engine& E = engine();
+    std::lock_guard<std::mutex> lock(E.mu);
+    Job job;
+    for (int k = 0; k < n; k++) {
+      if (!lcs[k]) throw std::invalid_argument("null light-curve object in batch");
+      job.curves.push_back(static_cast<Curve*>(lcs[k]));
+      job.starts.push_back(mk(traj[4 * k], traj[4 * k + 1]));
+      job.ends.push_back(mk(traj[4 * k + 2], traj[4 * k + 3]));
+    }
+    E.run(job);
+    for (auto* c : job.curves)
+      if (c->amoebaOut) write_par_file(*c);
+    return 0;
+  });
+}
+
+int lcirs_fill_seeds(void* lc, int n, const double* pos, const int* nSeeds, const int* pix,
+                     uint64_t* rays, double* amp, uint32_t* flags) {
+  return guarded([&] {
+    if (!lc || n <= 0 || !pos || !nSeeds || !pix) throw std::invalid_argument("bad arguments");
+    Curve* c = static_cast<Curve*>(lc);
+    Engine& E = engine();
+    std::lock_guard<std::mutex> lock(E.mu);
+    Job job;
+    job.curves = {c};
+    job.starts = {mk(0, 0)};
+    job.ends = {mk(0, 0)};
+    job.runSeeds = false;
+    job.pos.resize(n);
+    job.nSeeds.assign(nSeeds, nSeeds + n);
+    job.seeds.resize((size_t)n * kMaxSeeds);
+    for (int i = 0; i < n; i++) {
+      job.pos[i] = make_double2(pos[2 * i], pos[2 * i + 1]);
+      for (int k = 0; k < kMaxSeeds; k++)
+        job.seeds[(size_t)i * kMaxSeeds + k] =
+            make_int2(pix[2 * ((size_t)i * kMaxSeeds + k)], pix[2 * ((size_t)i * kMaxSeeds + k) + 1]);
+    }
+    E.run(job);
+    for (int i = 0; i < n; i++) {
+      if (rays) rays[i] = c->rays[i];
+      if (amp) amp[i] = c->full[i];
+      if (flags) flags[i] = c->flags[i];
+    }
+    return 0;
+  });
+}
+
+int tl_last_stats(double* out) {
+  const Stats& s = engine().stats;
+  out[0] = s.tasks; out[1] = s.evals; out[2] = s.ins; out[3] = s.dup;
+  out[4] = s.seedMs; out[5] = s.fillMs; out[6] = s.fillLaunches; out[7] = s.totalMs;
+  out[8] = s.busyMs; out[9] = s.insertMs; out[10] = s.idlePolls; out[11] = s.waves;
+  return 0;
+}
+
+int tl_set_option(const char* name, double value) {
+  if (!name) return -1;
+  Options& o = engine().opt;
+  const std::string n(name);
+  if (n == "fill_workgroups") o.fillWorkgroups = (int)value;
+  else if (n == "probe_limit") o.probeLimit = std::max(64, (int)value);
+  else if (n == "queue_log2") o.queueLog2 = std::min(28, std::max(10, (int)value));
+  else if (n == "scout_rows") o.scoutRows = std::max(0, (int)value);
+  else if (n == "scout_jump") o.scoutJump = std::max(1, (int)value);
+  else if (n == "time_limit_s") o.timeLimitS = std::max(0.1, value);
+  else return -1;
+  return 0;
+}
+
+const char* tl_last_error(void) { return g_err.c_str(); }
+
+int tl_device_info(char* name, int len, int* cus) {
+  return guarded([&] {
+    Engine& E = engine();
+    std::lock_guard<std::mutex> lock(E.mu);
+    E.init();
+    if (name && len > 0) {
+      std::strncpy(name, E.name.c_str(), len - 1);
+      name[len - 1] = 0;
+    }
+    if (cus) *cus = E.cus;
+    return 0;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,124 @@
+// Device-side data layout of the IRS engine and the host entry points that
+// launch its kernels.  Shared by tl_kernels.hip and tl_api.cpp.
+#pragma once
+
+#include <stdint.h>
+#include "tl_math.h"
+
+namespace tl {
+
+constexpr int kCccLen = 500;       // critical-curve samples per branch (lcirs.cc:16)
+constexpr int kMaxSeeds = 64;      // seeds kept per source position
+constexpr int kMaxBatch = 16384;   // source positions per fill launch (14-bit key field)
+constexpr int kMaxCover = 16;      // known runs read per row lookup
+
+// Per-point status bits (OR-ed into flags[p]).
+enum : uint32_t {
+  kFlagEdge = 1u,          // a counted pixel lies in column <= 1 or >= N-2, or a seed
+                           // fell outside the columns [0, N-1]: the reference's
+                           // scan-bound behaviour there is not reproduced
+  kFlagSeedOverflow = 2u,  // more than kMaxSeeds seeds; the extra ones were dropped
+  kFlagHashFull = 4u,      // run table probe limit reached
+  kFlagLabelFull = 8u,     // label table full
+  kFlagQueueFull = 16u,    // task queue full
+  kFlagTimeout = 32u,      // the fill launch hit its time limit
+  kFlagRowOverflow = 64u,  // more than kMaxCover runs in one row (work repeated, result exact)
+};
+
+// One light curve (a LightCurveIRS object) as the kernels see it.
+struct CurveDev {
+  Lens L;
+  PolyBase pb;
+  double rho, rho2, rho2pow;   // rho, rho*rho (ray test), pow(rho, 2.0) (intersection test)
+  double ldA, ldB;             // limb-darkening profile A + B sqrt(1 - r^2)
+  double blx, bly, step, Ldbl, Nd;
+  long long N;
+  double sx, sy, ex, ey;       // trajectory
+  double ppr;
+  unsigned lcLength;           // positions t = i/lcLength, i = 0..lcLength
+  int pointBase;               // first global point index
+  int causticBase;             // first caustic sample (6*kCccLen per curve)
+  int chainHave;               // root-reuse state at the start of the run
+  cx chain[11];
+  cx boxMin[6], boxMax[6];
+};
+
+// A row task of batch point `b`: find the maximal hit runs of row `row` that
+// meet [xa, xb].  kind: 0 = sweep step (arriving in direction dir from the
+// parent run [xa, xb] of row row-dir; f1, f2 = sweep anchor column and rows
+// since the anchor), 1 = seed pixel, 2 = scout (speculative interval, new
+// label per run), 3 = look-back (row row, [xa, xb] = the child run's extent,
+// pixels [f1, f2] known: the parent run and its two bounding misses).
+struct alignas(16) Task {
+  int b, row, xa, xb, label, kd, f1, f2;  // kd = kind | (dir < 0 ? 4 : 0)
+};
+
+struct SeedArgs {
+  const CurveDev* curves;
+  int nCurves;
+  const cx* caustics;
+  const TrigTable* trig;
+  int2* seeds;        // [point][kMaxSeeds] pixel (nx, ny)
+  int* nSeeds;
+  double2* ptPos;     // source position of every point
+  float* cost;        // predicted rays (point-source magnification x pi ppr^2)
+  uint32_t* flags;
+  RootChain* chainOut; // per curve: root-reuse state after the last position
+};
+
+struct FillArgs {
+  const CurveDev* curves;
+  const int* ptCurve;         // global point -> curve
+  const double2* ptPos;       // global point -> source position
+  const int* batchPts;        // batch point -> global point
+  int nBatch;
+  const int2* seedTasks;      // (batch point, seed slot) in processing order
+  int nSeedTasks;
+  const int2* seeds;
+  // run table
+  unsigned long long* htab;   // 2 words per entry
+  unsigned long long hmask;   // capacity - 1
+  int probeLimit;
+  // labels
+  int* lparent;
+  int* lpoint;
+  unsigned long long* lcount;
+  long long* lsum;
+  int* labelNext;
+  int labelCap;
+  // queue
+  Task* qbuf;
+  unsigned long long* qseq;
+  unsigned long long* qhead;
+  unsigned long long* qtail;
+  unsigned long long qcap;
+  long long* pending;
+  int* seedNext;
+  // results and status
+  uint32_t* flags;            // per global point
+  int* abortFlag;
+  unsigned long long* stats;  // [0] tasks, [1] evaluated pixels, [2] runs inserted, [3] duplicates,
+                              // [4] busy ticks, [5] insert ticks, [6] idle polls, [7] waves
+  long long timeLimitTicks;   // s_memrealtime ticks (100 MHz)
+  int scoutRows;              // rows swept before a scout is sent (0 = no scouts)
+  int scoutJump;              // rows a scout jumps ahead
+};
+
+// Host-side launchers (tl_kernels.hip).  All return hipError_t as int.
+int launch_seed(const SeedArgs& a, hipStream_t s);
+int launch_fill(const FillArgs& a, int nWorkgroups, hipStream_t s);
+int launch_clear_u64(unsigned long long* p, unsigned long long n, unsigned long long v, hipStream_t s);
+int launch_fill_reset(const FillArgs& a, hipStream_t s);
+int launch_resolve(const FillArgs& a, int nLabels, unsigned long long* ptCount, long long* ptSum,
+                   hipStream_t s);
+// Point-source light curves (LightCurveBase::getLC): curve k writes
+// lcLength values at out[curves[k].pointBase ...].
+int launch_point_lc(const CurveDev* curves, int nCurves, const TrigTable* trig, double* out,
+                    RootChain* chainOut, hipStream_t s);
+
+// Fixed-point scale of the brightness sum: each pixel's A + B sqrt(1 - r^2)
+// is rounded to a multiple of 2^-34 and summed as an integer, which makes
+// the sum independent of the order in which pixels are visited.
+constexpr double kSumScale = 17179869184.0;  // 2^34
+
+}  // namespace tl

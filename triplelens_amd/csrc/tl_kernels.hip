@@ -1,0 +1,1090 @@
+// HIP kernels of the IRS engine (gfx950).
+//
+//  seed_kernel    per light curve, the reference's seed sequence: point images
+//                 with the root-reuse chain, then the images of caustic trial
+//                 points (src/LightCurves/lcirs.cc:116-165), as pixel seeds.
+//  fill_kernel    one persistent, work-stealing launch that grows the image
+//                 "amoebae" of a batch of source positions: wavefronts pop
+//                 row-check tasks, evaluate 64 pixels per step with the FP64
+//                 ray map, record maximal hit runs in a shared run table and
+//                 sweep on to the next row.  Counted set and brightness sum are
+//                 those of lineFloodFill (lcirs.cc:381-466): every 4-connected
+//                 component of hit pixels that holds a hit seed.
+//  resolve_kernel folds the per-label partial sums into per-point results.
+//
+// Compiled with -ffp-contract=off: the hit test rsq < rho^2 is decided on the
+// same IEEE values as the reference's irsSIMD.
+#include <hip/hip_runtime.h>
+#include "tl_engine.h"
+
+namespace tl {
+
+#define AGENT __HIP_MEMORY_SCOPE_AGENT
+
+// Shared state of the fill (run table, labels, queue, counters) is accessed
+// only through relaxed agent-scope atomics, which are performed at the
+// device's coherence point.  Where one such access must be visible before
+// another, mem_order() waits until all of the wave's outstanding memory
+// operations have completed; acquire/release orderings are not used because
+// on this GPU they write back or invalidate the whole L2 cache.
+__device__ __forceinline__ unsigned long long ald(unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, AGENT);
+}
+__device__ __forceinline__ void ast(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, AGENT);
+}
+__device__ __forceinline__ bool acas(unsigned long long* p, unsigned long long* expected,
+                                     unsigned long long desired) {
+  return __hip_atomic_compare_exchange_strong(p, expected, desired, __ATOMIC_RELAXED,
+                                              __ATOMIC_RELAXED, AGENT);
+}
+__device__ __forceinline__ int aldi(int* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, AGENT); }
+__device__ __forceinline__ bool acasi(int* p, int* expected, int desired) {
+  return __hip_atomic_compare_exchange_strong(p, expected, desired, __ATOMIC_RELAXED,
+                                              __ATOMIC_RELAXED, AGENT);
+}
+__device__ __forceinline__ long long afaddl(long long* p, long long v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, AGENT);
+}
+__device__ __forceinline__ void mem_order() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+__device__ __forceinline__ long long wave_sum(long long v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T bcast0(T v) { return __shfl(v, 0, 64); }
+
+// ---------------------------------------------------------------------------
+// Pixel <-> plane mapping (lcirs.cc:355-378)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double nx_to_x(const CurveDev& C, long long nx) {
+  return C.blx + C.step * (double)nx;
+}
+__device__ __forceinline__ double ny_to_y(const CurveDev& C, long long ny) {
+  return C.bly + C.step * (double)ny;
+}
+// (long)((x - BL) * N / L + 0.5), with the operand clamped so that the
+// conversion is defined for any root.
+__device__ __forceinline__ int to_pixel(double v, double bl, double Nd, double L) {
+  double t = (v - bl) * Nd / L + 0.5;
+  if (!(t > -1.0e9)) t = -1.0e9;
+  if (t > 1.0e9) t = 1.0e9;
+  return (int)(long long)t;
+}
+
+// ---------------------------------------------------------------------------
+// Seeds
+// ---------------------------------------------------------------------------
+__device__ double point_amp_term(const Lens& L, cx z) {
+  cx d1 = csub(z, L.z1), d2 = csub(z, L.z2), d3 = csub(z, L.z3);
+  cx s = cadd(cadd(cdiv_rc(L.m1, cmul(d1, d1)), cdiv_rc(L.m2, cmul(d2, d2))),
+              cdiv_rc(L.m3, cmul(d3, d3)));
+  double t = 1.0 / fabs(1.0 - cnorm(s));
+  return t == t ? t : 0.0;
+}
+
+// ---- wave-cooperative seed search -----------------------------------------
+// One wavefront per light curve.  The root-reuse chain is inherently serial
+// and every lane runs it with the same values; the independent parts (the 10
+// root polishes, the 10 image tests and the caustic-segment tests) are spread
+// over the lanes and their results broadcast, so values and their order are
+// those of the serial code.
+
+__device__ __forceinline__ cx shfl_cx(cx v, int src) {
+  return mk(__shfl(v.re, src, 64), __shfl(v.im, src, 64));
+}
+
+__device__ __forceinline__ dcx shfl_dcx(dcx v, int src) {
+  dcx r;
+  r.re.hi = __shfl(v.re.hi, src, 64);
+  r.re.lo = __shfl(v.re.lo, src, 64);
+  r.im.hi = __shfl(v.im.hi, src, 64);
+  r.im.lo = __shfl(v.im.lo, src, 64);
+  return r;
+}
+
+// Coefficient k of the product of p (np terms) and q (nq terms), accumulated
+// from zero in increasing order of p's index, as dpmul does.
+__device__ __forceinline__ dcx poly_coef(const dcx* p, int np, const dcx* q, int nq, int k) {
+  dcx out = dc_zero();
+  for (int i = 0; i < np; i++) {
+    const int j = k - i;
+    if (j >= 0 && j < nq) out = dc_add(out, dc_mul(p[i], q[j]));
+  }
+  return out;
+}
+
+// lens_poly on a wavefront: each lane computes whole coefficients of one
+// stage, with the same operation sequence as the serial code, and the stage's
+// results are broadcast to all lanes.
+__device__ void wave_lens_poly(const Lens& L, const PolyBase& pb, cx zeta, cx* c) {
+  const int ln = __lane_id();
+  const dd m1 = dd_of(L.m1), m2 = dd_of(L.m2), m3 = dd_of(L.m3);
+  const dcx zc = dc_of(cconj(zeta));
+  const dcx w[3] = {dc_sub(zc, dc_of(cconj(L.z1))), dc_sub(zc, dc_of(cconj(L.z2))),
+                    dc_sub(zc, dc_of(cconj(L.z3)))};
+  // stage 1: Q_k[i] = w_k D[i] + S[i], lanes 0..11
+  dcx Q[3][4];
+  {
+    dcx v = dc_zero();
+    if (ln < 12) {
+      const int k = ln >> 2, i = ln & 3;
+      const dcx sv = i < 3 ? pb.S[i] : dc_zero();
+      v = dc_add(dc_mul(w[k], pb.D[i]), sv);
+    }
+    for (int l = 0; l < 12; l++) Q[l >> 2][l & 3] = shfl_dcx(v, l);
+  }
+  // stage 2: q12, q13, q23 (7 coefficients each), lanes 0..20
+  dcx q[3][7];
+  {
+    dcx v = dc_zero();
+    if (ln < 21) {
+      const int pr = ln / 7, k = ln % 7;
+      const dcx* P = pr == 2 ? Q[1] : Q[0];
+      const dcx* R = pr == 0 ? Q[1] : Q[2];
+      v = poly_coef(P, 4, R, 4, k);
+    }
+    for (int l = 0; l < 21; l++) q[l / 7][l % 7] = shfl_dcx(v, l);
+  }
+  // stage 3: q123 = q12 Q3 (lanes 0..9), mix = m1 q23 + m2 q13 + m3 q12 (lanes 10..16)
+  dcx q123[10], mix[7];
+  {
+    dcx v = dc_zero();
+    if (ln < 10) {
+      v = poly_coef(q[0], 7, Q[2], 4, ln);
+    } else if (ln < 17) {
+      const int i = ln - 10;
+      v = dc_add(dc_add(dc_scale(q[2][i], m1), dc_scale(q[1][i], m2)), dc_scale(q[0][i], m3));
+    }
+    for (int l = 0; l < 10; l++) q123[l] = shfl_dcx(v, l);
+    for (int l = 0; l < 7; l++) mix[l] = shfl_dcx(v, 10 + l);
+  }
+  // stage 4: c_i = D mix - (z - zeta) q123, lanes 0..10
+  {
+    cx v = mk(0.0, 0.0);
+    if (ln < 11) {
+      const dcx lin[2] = {dc_of(cneg(zeta)), dc_of(mk(1.0, 0.0))};
+      const dcx left = poly_coef(lin, 2, q123, 10, ln);
+      const dcx right = ln < 10 ? poly_coef(pb.D, 4, mix, 7, ln) : dc_zero();
+      v = dc_round(dc_sub(right, left));
+    }
+    for (int l = 0; l < 11; l++) c[l] = shfl_cx(v, l);
+  }
+}
+
+// Laguerre::laguerre (laguerre.cc:18-99) with every lane holding the same x:
+// the Horner recurrence runs on all lanes, lane j keeps b_j, the moduli
+// |b_j| are taken in parallel and the error bound is accumulated in the
+// reference's order from the broadcast values.
+__device__ bool wave_lag_iter(const cx* poly, int m, cx* xp, const TrigTable* trig) {
+  const double frac[8] = {0.5, 0.23, 0.73, 0.13, 0.37, 0.61, 0.89, 1.0};
+  const int ln = __lane_id();
+  cx x = *xp;
+  const double md = (double)m;
+  if (m == 1) return lag_iter(poly, m, xp, trig);
+  for (int i = 1; i <= kLagMaxIt; i++) {
+    cx b = poly[m];
+    const double err0 = cabs_(b);
+    cx s = mk(0.0, 0.0), f = mk(0.0, 0.0), mine = mk(0.0, 0.0);
+    for (int j = m - 1; j >= 0; j--) {
+      s = cadd(cmul(x, s), f);
+      f = cadd(cmul(x, f), b);
+      b = cadd(cmul(x, b), poly[j]);
+      if (ln == j) mine = b;
+    }
+    const double mabs = cabs_(mine);
+    const double ax = cabs_(x);
+    double err = err0;
+    for (int j = m - 1; j >= 0; j--) err = __shfl(mabs, j, 64) + ax * err;
+    err *= kLagEps;
+    const double babs = __shfl(mabs, 0, 64);  // |b_0|, the polynomial value
+    if (babs <= err) { *xp = x; return true; }
+    cx g = cdiv(f, b);
+    cx h = csub(cmul(g, g), cdiv(cscale(s, 2.0), b));
+    cx rad = cscale(csub(cscale(h, md), cmul(g, g)), md - 1.0);
+    cx sq = csqrt_(rad);
+    cx gp = cadd(g, sq);
+    cx gm = csub(g, sq);
+    if (cabs_(gp) < cabs_(gm)) gp = gm;
+    cx dx;
+    if (cabs_(gp) > 0.0) dx = cdiv_rc(md, gp);
+    else dx = cscale(mk(trig->c[i], trig->s[i]), 1.0 + cabs_(x));
+    cx x1 = csub(x, dx);
+    if (x.re == x1.re && x.im == x1.im) { *xp = x; return true; }
+    if (i % kLagMT) x = x1;
+    else x = csub(x, cmul(mk(frac[(i / kLagMT) & 7], 0.0), dx));
+  }
+  *xp = x;
+  return false;
+}
+
+// Laguerre::solveRoots for the degree-10 image polynomial: deflation on all
+// lanes, then lane k polishes root k.
+__device__ void wave_solve10(const cx* c, const cx* init, cx* roots, const TrigTable* trig) {
+  cx tmp[11];
+  for (int i = 0; i <= 10; i++) tmp[i] = c[i];
+  int nr = 0;
+  for (int j = 10; j >= 1; j--) {
+    cx x = init ? init[j] : mk(kLagEps, kLagEps);
+    if (!wave_lag_iter(tmp, j, &x, trig) && init) x = mk(kLagEps, kLagEps);
+    roots[nr++] = x;
+    cx b = tmp[j];
+    for (int jj = j - 1; jj >= 0; jj--) {
+      cx cc = tmp[jj];
+      tmp[jj] = b;
+      b = cadd(cmul(x, b), cc);
+    }
+  }
+  const int ln = __lane_id();
+  cx mine = roots[ln < 10 ? ln : 0];
+  if (ln < 10) lag_iter(c, 10, &mine, trig);
+  for (int k = 0; k < 10; k++) roots[k] = shfl_cx(mine, k);
+}
+
+// getRootsPrecalculated + getImages (imgpoint.cc:160-274) on a wavefront.
+__device__ int wave_get_images(const Lens& L, const PolyBase& pb, RootChain* rc, cx zeta,
+                               cx* imgs, const TrigTable* trig) {
+  cx c[11];
+  wave_lens_poly(L, pb, zeta, c);
+  const int ln = __lane_id();
+  for (int attempt = 0; attempt < 2; attempt++) {
+    if (!rc->have || attempt == 1) {
+      wave_solve10(c, nullptr, rc->tempRoots, trig);
+      rc->have = 1;
+    } else {
+      cx out[10];
+      wave_solve10(c, rc->tempRoots, out, trig);
+      if (!lag_check<10>(c, out)) wave_solve10(c, nullptr, out, trig);
+      for (int k = 0; k < 10; k++) rc->tempRoots[k] = out[k];
+    }
+    rc->tempRoots[10] = mk(0.0, 0.0);
+    const bool ok = ln < 10 && img_check(L, rc->tempRoots[ln < 10 ? ln : 0], zeta, 1.0e-6);
+    unsigned long long m = __ballot(ok);
+    int n = 0;
+    while (m) {
+      const int k = __builtin_ctzll(m);
+      m &= m - 1;
+      imgs[n++] = rc->tempRoots[k];
+    }
+    if (!(n % 2 != 0 || n < 4)) return n;
+    if (attempt == 1) return n;
+  }
+  return 0;
+}
+
+__global__ void __launch_bounds__(64) seed_kernel(SeedArgs a) {
+  const int c = blockIdx.x;
+  if (c >= a.nCurves) return;
+  const int ln = __lane_id();
+  const CurveDev& C = a.curves[c];
+  const Lens L = C.L;
+  RootChain rc;
+  for (int k = 0; k < 11; k++) rc.tempRoots[k] = C.chain[k];
+  rc.have = C.chainHave;
+  cx imgs[10];
+  const cx s = mk(C.sx, C.sy), e = mk(C.ex, C.ey);
+  const cx* caus = a.caustics + C.causticBase;
+  for (unsigned i = 0; i <= C.lcLength; i++) {
+    const int p = C.pointBase + (int)i;
+    const cx pos = traj_pos(s, e, i, C.lcLength);
+    if (ln == 0) a.ptPos[p] = make_double2(pos.re, pos.im);
+    int cnt = 0;
+    uint32_t fl = 0;
+    int2* out = a.seeds + (size_t)p * kMaxSeeds;
+    const int n = wave_get_images(L, C.pb, &rc, pos, imgs, a.trig);
+    double amp = 0.0;
+    for (int k = 0; k < n; k++) {
+      amp += point_amp_term(L, imgs[k]);
+      if (cnt < kMaxSeeds) {
+        if (ln == 0)
+          out[cnt] = make_int2(to_pixel(imgs[k].re, C.blx, C.Nd, C.Ldbl),
+                               to_pixel(imgs[k].im, C.bly, C.Nd, C.Ldbl));
+        cnt++;
+      } else {
+        fl |= kFlagSeedOverflow;
+      }
+    }
+    for (int r = 0; r < 6; r++) {
+      if (!(pos.re + C.rho >= C.boxMin[r].re && pos.re - C.rho <= C.boxMax[r].re &&
+            pos.im + C.rho >= C.boxMin[r].im && pos.im - C.rho <= C.boxMax[r].im))
+        continue;
+      // segments k..k+1 of this caustic branch, 64 at a time; only the first
+      // segment of every run of consecutive hits solves for images
+      const cx* cr = caus + r * kCccLen;
+      bool prevHit = false;  // hit of the segment before this chunk
+      for (int k0 = 0; k0 < kCccLen - 1; k0 += 64) {
+        const int k = k0 + ln;
+        cx trial = mk(0.0, 0.0);
+        const bool hit = k < kCccLen - 1 && intersection_check(pos, cr[k], cr[k + 1], C.rho,
+                                                               C.rho2pow, &trial);
+        const unsigned long long m = __ballot(hit);
+        const unsigned long long firsts = m & ~((m << 1) | (prevHit ? 1ull : 0ull));
+        unsigned long long f = firsts;
+        while (f) {
+          const int j = __builtin_ctzll(f);
+          f &= f - 1;
+          const cx tp = shfl_cx(trial, j);
+          const int n2 = wave_get_images(L, C.pb, &rc, tp, imgs, a.trig);
+          for (int q = 0; q < n2; q++) {
+            if (cnt < kMaxSeeds) {
+              if (ln == 0)
+                out[cnt] = make_int2(to_pixel(imgs[q].re, C.blx, C.Nd, C.Ldbl),
+                                     to_pixel(imgs[q].im, C.bly, C.Nd, C.Ldbl));
+              cnt++;
+            } else {
+              fl |= kFlagSeedOverflow;
+            }
+          }
+        }
+        prevHit = (m >> 63) & 1ull;
+      }
+    }
+    if (ln == 0) {
+      a.nSeeds[p] = cnt;
+      a.flags[p] = fl;
+      a.cost[p] = (float)(amp * 3.141592653589793 * C.ppr * C.ppr);
+    }
+  }
+  if (ln == 0) a.chainOut[c] = rc;
+}
+
+// LightCurveBase::getLC / getPointAmp (lcbase.cc:17-52): positions at
+// t = i/(lcLength-1), magnification sum over images of 1/|1 - |sum m/(z-z_k)^2|^2|
+// (the squares as products; the reference uses std::pow(z, 2.0)).
+__global__ void __launch_bounds__(64) point_lc_kernel(const CurveDev* curves, int nCurves,
+                                                      const TrigTable* trig, double* out,
+                                                      RootChain* chainOut) {
+  const int c = blockIdx.x;
+  if (c >= nCurves || threadIdx.x != 0) return;
+  const CurveDev& C = curves[c];
+  RootChain rc;
+  for (int k = 0; k < 11; k++) rc.tempRoots[k] = C.chain[k];
+  rc.have = C.chainHave;
+  cx imgs[10];
+  const cx s = mk(C.sx, C.sy), e = mk(C.ex, C.ey);
+  for (unsigned i = 0; i < C.lcLength; i++) {
+    const cx pos = cadd(s, cscale(csub(e, s), i / (C.lcLength - 1.0)));
+    const int n = get_images(C.L, C.pb, &rc, pos, imgs, trig);
+    double amp = 0.0;
+    for (int k = 0; k < n; k++) amp += point_amp_term(C.L, imgs[k]);
+    out[C.pointBase + (int)i] = amp;
+  }
+  chainOut[c] = rc;
+}
+
+int launch_point_lc(const CurveDev* curves, int nCurves, const TrigTable* trig, double* out,
+                    RootChain* chainOut, hipStream_t s) {
+  if (nCurves <= 0) return 0;
+  hipLaunchKernelGGL(point_lc_kernel, dim3(nCurves), dim3(64), 0, s, curves, nCurves, trig, out,
+                     chainOut);
+  return (int)hipGetLastError();
+}
+
+int launch_seed(const SeedArgs& a, hipStream_t s) {
+  if (a.nCurves <= 0) return 0;
+  hipLaunchKernelGGL(seed_kernel, dim3(a.nCurves), dim3(64), 0, s, a);
+  return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Run table: open addressing over 16-byte entries.
+//   word0 = (row+1) << 39 | xL << 14 | b     (0 = empty)
+//   word1 = label << 32 | xR                 (~0 = not yet written)
+// A maximal run is identified by (b, row, xL): two floods that reach the same
+// run compute the same (xL, xR), and the compare-and-swap on word0 lets
+// exactly one of them record it.
+// ---------------------------------------------------------------------------
+constexpr unsigned long long kEmpty1 = ~0ull;
+
+// word0 = (row+1) << 39 | xL << 14 | b, with rows and columns of the grid
+// (0 <= row, xL < 2^25 - 1; scans never leave the columns [0, N-1]) and
+// b < 2^14.  word1 = label << 32 | xR.
+__device__ __forceinline__ unsigned long long run_key(int b, int row, long long xl) {
+  return ((unsigned long long)(row + 1) << 39) | ((unsigned long long)xl << 14) |
+         (unsigned long long)b;
+}
+__device__ __forceinline__ unsigned long long hash_run(unsigned long long key) {
+  unsigned long long h = key;
+  h ^= h >> 33;
+  h *= 0xff51afd7ed558ccdull;
+  h ^= h >> 33;
+  h *= 0xc4ceb9fe1a85ec53ull;
+  h ^= h >> 33;
+  return h;
+}
+
+enum { kInsNew = 1, kInsDup = 0, kInsFull = -1 };
+
+// Records run (b, row, xl, xr) with `label`; lane 0 does the work and the
+// result is broadcast.  On a duplicate *dupLabel receives the stored label.
+__device__ __forceinline__ int insert_run(const FillArgs& A, int b, int row, long long xl, long long xr, int label,
+                          int* dupLabel) {
+  int res = kInsFull, dl = -1;
+  if (__lane_id() == 0) {
+    const unsigned long long key = run_key(b, row, xl);
+    const unsigned long long w1 =
+        ((unsigned long long)(unsigned)label << 32) | (unsigned long long)(unsigned)xr;
+    const unsigned long long h = hash_run(key);
+    for (int i = 0; i < A.probeLimit; i++) {
+      const unsigned long long idx = (h + (unsigned long long)i) & A.hmask;
+      unsigned long long expected = 0;
+      if (acas(&A.htab[2 * idx], &expected, key)) {
+        ast(&A.htab[2 * idx + 1], w1);
+        res = kInsNew;
+        break;
+      }
+      if (expected == key) {
+        // the winner writes word1 right after its compare-and-swap
+        unsigned long long v = ald(&A.htab[2 * idx + 1]);
+        for (int spin = 0; spin < (1 << 20) && v == kEmpty1; spin++) {
+          __builtin_amdgcn_s_sleep(2);
+          v = ald(&A.htab[2 * idx + 1]);
+        }
+        dl = v == kEmpty1 ? -1 : (int)(v >> 32);
+        res = kInsDup;
+        break;
+      }
+    }
+  }
+  res = bcast0(res);
+  *dupLabel = bcast0(dl);
+  return res;
+}
+
+// ---------------------------------------------------------------------------
+// Labels: union-find over flood labels; label b (< nBatch) is the class of
+// the reference seeds of batch point b.  Roots link larger index to smaller.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int find_root(int* parent, int x) {
+  for (int it = 0; it < (1 << 20); it++) {
+    const int p = aldi(&parent[x]);
+    if (p == x) return x;
+    x = p;
+  }
+  return x;
+}
+
+__device__ __forceinline__ void unite(const FillArgs& A, int a, int b) {
+  if (__lane_id() == 0 && a >= 0 && b >= 0 && a != b) {
+    for (int it = 0; it < (1 << 16); it++) {
+      const int ra = find_root(A.lparent, a), rb = find_root(A.lparent, b);
+      if (ra == rb) break;
+      const int hi = ra > rb ? ra : rb, lo = ra > rb ? rb : ra;
+      int expected = hi;
+      if (acasi(&A.lparent[hi], &expected, lo)) break;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Task queue: bounded multi-producer multi-consumer ring with per-slot
+// sequence numbers.  The payload is complete before the sequence number that
+// publishes it is written (mem_order between them), and a slot is handed back
+// only after its payload has been read.  Lane 0 operates.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool q_push(const FillArgs& A, const Task& t) {
+  int ok = 0;
+  if (__lane_id() == 0) {
+    unsigned long long pos = ald(A.qtail);
+    for (int it = 0; it < (1 << 20); it++) {
+      const unsigned long long idx = pos & (A.qcap - 1);
+      const unsigned long long seq = ald(&A.qseq[idx]);
+      const long long dif = (long long)(seq - pos);
+      if (dif == 0) {
+        if (acas(A.qtail, &pos, pos + 1)) {
+          unsigned long long w[4];
+          __builtin_memcpy(w, &t, sizeof(Task));
+          unsigned long long* dst = reinterpret_cast<unsigned long long*>(&A.qbuf[idx]);
+          for (int k = 0; k < 4; k++) ast(&dst[k], w[k]);
+          mem_order();
+          ast(&A.qseq[idx], pos + 1);
+          ok = 1;
+          break;
+        }
+      } else if (dif < 0) {
+        break;  // full
+      } else {
+        pos = ald(A.qtail);
+      }
+    }
+  }
+  return bcast0(ok) != 0;
+}
+
+__device__ __forceinline__ bool q_pop(const FillArgs& A, Task& t) {
+  int ok = 0;
+  unsigned long long w[4] = {0, 0, 0, 0};
+  if (__lane_id() == 0) {
+    unsigned long long pos = ald(A.qhead);
+    if (pos < ald(A.qtail)) {
+      for (int it = 0; it < 64; it++) {
+        const unsigned long long idx = pos & (A.qcap - 1);
+        const unsigned long long seq = ald(&A.qseq[idx]);
+        const long long dif = (long long)(seq - (pos + 1));
+        if (dif == 0) {
+          if (acas(A.qhead, &pos, pos + 1)) {
+            unsigned long long* src = reinterpret_cast<unsigned long long*>(&A.qbuf[idx]);
+            for (int k = 0; k < 4; k++) w[k] = ald(&src[k]);
+            mem_order();
+            ast(&A.qseq[idx], pos + A.qcap);
+            ok = 1;
+            break;
+          }
+        } else if (dif < 0) {
+          break;  // empty, or the slot is still being written
+        } else {
+          pos = ald(A.qhead);
+        }
+      }
+    }
+  }
+  ok = bcast0(ok);
+  for (int k = 0; k < 4; k++) w[k] = bcast0(w[k]);
+  __builtin_memcpy(&t, w, sizeof(Task));
+  return ok != 0;
+}
+
+// ---------------------------------------------------------------------------
+// Row scanning
+// ---------------------------------------------------------------------------
+struct RowCtx {
+  const CurveDev* C;
+  double y, px, py;
+  long long N;
+};
+
+// Ray test and fixed-point brightness of pixel x of the row
+// (lcirs.cc:400-407, limbdarkeningmodel.cc:45-49).
+__device__ __forceinline__ bool eval_px(const RowCtx& R, long long x, long long& val) {
+  const CurveDev& C = *R.C;
+  const double rsq = irs_ray(C.L, nx_to_x(C, x), R.y, R.px, R.py);
+  if (!(rsq < C.rho2)) return false;
+  const double r = rsq / C.rho2;
+  const double v = C.ldA + C.ldB * sqrt(1.0 - r);
+  val = (long long)(v * kSumScale + 0.5);
+  return true;
+}
+
+// A look-back recorded while a row is scanned, processed after the scan:
+// row `row` over [xa, xb] (the new run's extent) without the known pixels
+// [k1, k2] (the run it was reached from and its two bounding misses).
+struct LookBack {
+  int row, xa, xb, k1, k2, dir, label;
+};
+constexpr int kMaxLookBacks = 2;  // kept in two named slots (no indexed array)
+
+// State of the task being processed (uniform across the wave).
+struct TaskState {
+  const FillArgs* A;
+  Task t;
+  int kind, dir, gp;
+  long long N, xa, xb;      // task interval (kind 3: the child run's extent)
+  Task cont;                // follow-up kept by this wave (the next sweep step)
+  bool haveCont;
+  int nBack;
+  LookBack back0, back1;
+  int accLabel;             // totals of new runs, credited to accLabel when
+  long long accSum;         // the label changes or the sweep ends
+  unsigned long long accCnt;
+  unsigned long long evals, nIns, nDup;
+  unsigned long long insTicks;  // time spent in run-table inserts
+};
+
+__device__ __forceinline__ void flag_point(const FillArgs& A, int gp, uint32_t f) {
+  if (__lane_id() == 0) atomicOr(&A.flags[gp], f);
+}
+
+__device__ __forceinline__ void flush_acc(TaskState& S) {
+  if (S.accCnt && __lane_id() == 0) {
+    atomicAdd(&S.A->lcount[S.accLabel], S.accCnt);
+    atomicAdd((unsigned long long*)&S.A->lsum[S.accLabel], (unsigned long long)S.accSum);
+  }
+  S.accCnt = 0;
+  S.accSum = 0;
+}
+
+__device__ __forceinline__ void credit(TaskState& S, int label, unsigned long long cnt,
+                                       long long sum) {
+  if (label != S.accLabel) {
+    flush_acc(S);
+    S.accLabel = label;
+  }
+  S.accCnt += cnt;
+  S.accSum += sum;
+}
+
+__device__ __forceinline__ Task make_task(int b, long long row, long long xa, long long xb,
+                                          int label, int kind, int dir, long long f1,
+                                          long long f2) {
+  Task f;
+  f.b = b; f.row = (int)row; f.xa = (int)xa; f.xb = (int)xb; f.label = label;
+  f.kd = kind | (dir < 0 ? 4 : 0);
+  f.f1 = (int)f1; f.f2 = (int)f2;
+  return f;
+}
+
+// Queues a task.  The pending count is raised, and that increment
+// completed, before the task becomes visible to other waves.
+__device__ __forceinline__ void push_task(TaskState& S, const Task& f) {
+  const FillArgs& A = *S.A;
+  if (__lane_id() == 0) {
+    afaddl(A.pending, 1);
+    mem_order();
+  }
+  if (!q_push(A, f)) {
+    flag_point(A, S.gp, kFlagQueueFull);
+    if (__lane_id() == 0) afaddl(A.pending, -1);
+  }
+}
+
+// Keeps a follow-up as this wave's next step, or queues it.
+__device__ __forceinline__ void add_followup(TaskState& S, const Task& f) {
+  if (!S.haveCont) {
+    S.cont = f;
+    S.haveCont = true;
+    return;
+  }
+  push_task(S, f);
+}
+
+// A maximal hit run [xl, xr] of the task's row was found: record it, credit
+// its pixels to its label and schedule the neighbouring rows.
+__device__ __forceinline__ void handle_run(TaskState& S, long long xl, long long xr, long long sum,
+                           unsigned long long cnt) {
+  const FillArgs& A = *S.A;
+  const Task& t = S.t;
+  int label = t.label;
+  if (S.kind == 2) {
+    // a scout run starts a class of its own
+    int nl = 0;
+    if (__lane_id() == 0) nl = atomicAdd(A.labelNext, 1);
+    nl = bcast0(nl);
+    if (nl >= A.labelCap) {
+      flag_point(A, S.gp, kFlagLabelFull);
+      return;
+    }
+    if (__lane_id() == 0) A.lpoint[nl] = t.b;
+    label = nl;
+  }
+  int dupLabel = -1;
+  const unsigned long long ti0 = __builtin_amdgcn_s_memrealtime();
+  const int ins = insert_run(A, t.b, t.row, xl, xr, label, &dupLabel);
+  S.insTicks += __builtin_amdgcn_s_memrealtime() - ti0;
+  if (ins == kInsFull) {
+    flag_point(A, S.gp, kFlagHashFull);
+    return;
+  }
+  if (ins == kInsDup) {
+    S.nDup++;
+    if (S.kind != 2) unite(A, label, dupLabel);
+    return;
+  }
+  S.nIns++;
+  if (xl <= 1 || xr >= S.N - 2) flag_point(A, S.gp, kFlagEdge);
+  credit(S, label, cnt, sum);
+  const long long row = t.row, N = S.N, center = (xl + xr) / 2;
+  if (S.kind == 1 || S.kind == 2) {
+    for (int d = 1; d >= -1; d -= 2)
+      if (row + d >= 0 && row + d < N)
+        add_followup(S, make_task(t.b, row + d, xl, xr, label, 0, d, center, 0));
+    return;
+  }
+  const int d = S.dir;  // direction of travel
+  if (S.kind == 3) {
+    // a run found looking back: sweep on away from the run it was reached
+    // from, and look back towards that run's row where this one reaches past it
+    if (row + d >= 0 && row + d < N)
+      add_followup(S, make_task(t.b, row + d, xl, xr, label, 0, d, center, 0));
+    if (row - d >= 0 && row - d < N && (xl < S.xa - 1 || xr > S.xb + 1))
+      push_task(S, make_task(t.b, row - d, xl, xr, label, 3, -d, S.xa - 1, S.xb + 1));
+    return;
+  }
+  // sweep step: 1. the next row in the direction of travel
+  if (row + d >= 0 && row + d < N) {
+    // f2 >= 0: rows since the sweep's anchor, no scout sent yet;
+    // f2 < 0: -(rows since the last scout) - 1
+    long long a1 = t.f1, a2 = t.f2 >= 0 ? t.f2 + 1 : t.f2 - 1;
+    const long long rows = a2 >= 0 ? a2 : -a2 - 1;
+    const bool due = A.scoutRows > 0 && ((a2 >= 0 && rows >= A.scoutRows) ||
+                                         (a2 < 0 && rows >= A.scoutJump));
+    if (due) {
+      // send a scout ahead along the run's drift, then re-anchor
+      const long long rt = row + (long long)d * A.scoutJump;
+      if (rt >= 0 && rt < N) {
+        const long long drift = rows > 0 ? (center - a1) / rows : 0;
+        const long long pc = center + drift * A.scoutJump;
+        const long long adr = drift < 0 ? -drift : drift;
+        const long long hw = (xr - xl) / 2 + adr * A.scoutJump / 4 + 32;
+        long long sa = pc - hw, sb = pc + hw;
+        if (sa < 0) sa = 0;
+        if (sb > N - 1) sb = N - 1;
+        if (sa <= sb) push_task(S, make_task(t.b, rt, sa, sb, -1, 2, 0, 0, 0));
+      }
+      a1 = center;
+      a2 = -1;
+    }
+    add_followup(S, make_task(t.b, row + d, xl, xr, label, 0, d, a1, a2));
+  }
+  // 2. the row the sweep came from, where the new run reaches past the parent
+  //    run [xa, xb] and its two bounding misses: looked at after this scan
+  if (row - d >= 0 && row - d < N && (xl < S.xa - 1 || xr > S.xb + 1)) {
+    if (S.nBack < kMaxLookBacks) {
+      LookBack lb;
+      lb.row = (int)(row - d); lb.xa = (int)xl; lb.xb = (int)xr;
+      lb.k1 = (int)(S.xa - 1); lb.k2 = (int)(S.xb + 1); lb.dir = -d; lb.label = label;
+      if (S.nBack == 0) S.back0 = lb;
+      else S.back1 = lb;
+      S.nBack++;
+    } else {
+      push_task(S, make_task(t.b, row - d, xl, xr, label, 3, -d, S.xa - 1, S.xb + 1));
+    }
+  }
+}
+
+// Extends a run leftwards from column `from`, adding the pixels it takes in
+// to acc/cnt.  Returns the run's left end.
+__device__ __forceinline__ long long extend_left(const RowCtx& R, long long from, long long& acc,
+                                 unsigned long long& cnt, unsigned long long& evals) {
+  const int ln = __lane_id();
+  long long xcur = from;
+  while (true) {
+    if (xcur < 0) return 0;
+    const long long x0 = xcur - 63;
+    const long long x = x0 + ln;
+    const bool valid = x >= 0;
+    long long val = 0;
+    const bool hit = valid && eval_px(R, x, val);
+    evals += (unsigned long long)(xcur - (x0 < 0 ? 0 : x0) + 1);
+    const unsigned long long m = __ballot(hit);
+    const unsigned long long inv = ~m;
+    const int z = inv == 0 ? 64 : __builtin_clzll(inv);  // hits from bit 63 down
+    if (ln >= 64 - z) acc += val;
+    cnt += (unsigned long long)z;
+    if (z < 64) return xcur - z + 1;
+    xcur -= 64;
+  }
+}
+
+// Extends a run rightwards from column `from`.  Returns the run's right end.
+__device__ __forceinline__ long long extend_right(const RowCtx& R, long long from, long long& acc,
+                                  unsigned long long& cnt, unsigned long long& evals) {
+  const int ln = __lane_id();
+  long long xcur = from;
+  while (true) {
+    if (xcur > R.N - 1) return R.N - 1;
+    const long long x = xcur + ln;
+    const bool valid = x <= R.N - 1;
+    long long val = 0;
+    const bool hit = valid && eval_px(R, x, val);
+    const long long hiEnd = xcur + 63 < R.N - 1 ? xcur + 63 : R.N - 1;
+    evals += (unsigned long long)(hiEnd - xcur + 1);
+    const unsigned long long m = __ballot(hit);
+    const unsigned long long inv = ~m;
+    const int z = inv == 0 ? 64 : __builtin_ctzll(inv);
+    if (ln < z) acc += val;
+    cnt += (unsigned long long)z;
+    if (z < 64) return xcur + z - 1;
+    xcur += 64;
+  }
+}
+
+// Maximal hit runs of the row that meet [s, e].  The row is evaluated over a
+// window that pads [s, e] to whole 64-pixel steps on the sides where runs
+// may continue (extL, extR); elsewhere the pixel next to [s, e] is a known
+// miss.  Runs inside the padding that do not reach [s, e] are ignored; runs
+// that reach the window's edge are followed past it.  Each run is handed to
+// handle_run once both ends are known.
+__device__ __forceinline__ void scan_window(const RowCtx& R, long long s, long long e, bool extL, bool extR,
+                            TaskState& S) {
+  const int ln = __lane_id();
+  const long long len = e - s + 1;
+  const long long extra = ((len + 63) / 64) * 64 - len;
+  long long padL = 0, padR = 0;
+  if (extL && extR) {
+    padL = extra / 2;
+    padR = extra - padL;
+  } else if (extL) {
+    padL = extra;
+  } else if (extR) {
+    padR = extra;
+  }
+  long long ws = s - padL, we = e + padR;
+  if (ws < 0) ws = 0;
+  if (we > R.N - 1) we = R.N - 1;
+  bool open = false;
+  long long runL = 0, acc = 0;
+  unsigned long long cnt = 0;
+  for (long long x0 = ws; x0 <= we; x0 += 64) {
+    const long long x = x0 + ln;
+    const bool valid = x <= we;
+    long long val = 0;
+    const bool hit = valid && eval_px(R, x, val);
+    const unsigned long long m = __ballot(hit);
+    const int nb = (int)(we - x0 + 1 < 64 ? we - x0 + 1 : 64);
+    S.evals += (unsigned long long)nb;
+    int pos = 0;
+    while (pos < nb) {
+      if (open) {
+        const unsigned long long rest = ~(m >> pos);
+        int z = rest == 0 ? 64 - pos : __builtin_ctzll(rest);
+        if (z > nb - pos) z = nb - pos;
+        if (ln >= pos && ln < pos + z) acc += val;
+        cnt += (unsigned long long)z;
+        pos += z;
+        if (pos < nb) {
+          const long long runR = x0 + pos - 1;
+          if (runR >= s && runL <= e) {
+            if (runL == ws && extL && ws > 0) runL = extend_left(R, ws - 1, acc, cnt, S.evals);
+            handle_run(S, runL, runR, wave_sum(acc), cnt);
+          }
+          acc = 0;
+          cnt = 0;
+          open = false;
+        }
+      } else {
+        const unsigned long long rest = m >> pos;
+        if (rest == 0) break;
+        const int tz = __builtin_ctzll(rest);
+        if (pos + tz >= nb) break;
+        runL = x0 + pos + tz;
+        open = true;
+        pos += tz;
+      }
+    }
+  }
+  if (open && we >= s && runL <= e) {
+    long long runR = we;
+    if (runL == ws && extL && ws > 0) runL = extend_left(R, ws - 1, acc, cnt, S.evals);
+    if (extR && we < R.N - 1) runR = extend_right(R, we + 1, acc, cnt, S.evals);
+    handle_run(S, runL, runR, wave_sum(acc), cnt);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fill
+// ---------------------------------------------------------------------------
+// Scans [lo, hi] of the current row without the known pixels [k1, k2].
+__device__ __forceinline__ void scan_excluding(const RowCtx& R, long long lo, long long hi, long long k1,
+                               long long k2, TaskState& S) {
+  if (lo < k1) scan_window(R, lo, hi < k1 - 1 ? hi : k1 - 1, true, hi < k1 - 1, S);
+  if (hi > k2) scan_window(R, lo > k2 + 1 ? lo : k2 + 1, hi, lo > k2 + 1, true, S);
+}
+
+// Processes one task and the look-backs it records; the follow-up kept for
+// this wave (if any) is left in S.cont / S.haveCont.
+__device__ __forceinline__ void process_task(TaskState& S) {
+  const FillArgs& A = *S.A;
+  S.haveCont = false;
+  S.nBack = 0;
+  S.kind = S.t.kd & 3;
+  S.dir = (S.t.kd & 4) ? -1 : 1;
+  S.gp = A.batchPts[S.t.b];
+  const CurveDev& C = A.curves[A.ptCurve[S.gp]];
+  S.N = C.N;
+  if (S.t.row < 0 || (long long)S.t.row >= S.N) return;
+  S.xa = S.t.xa;
+  S.xb = S.t.xb;
+  if (S.kind == 1 && (S.xa < 0 || S.xa > S.N - 1)) {  // seed column outside the grid
+    flag_point(A, S.gp, kFlagEdge);
+    return;
+  }
+  const long long lo = S.xa < 0 ? 0 : S.xa, hi = S.xb > S.N - 1 ? S.N - 1 : S.xb;
+  if (lo > hi) return;
+
+  RowCtx R;
+  R.C = &C;
+  R.N = S.N;
+  R.y = ny_to_y(C, S.t.row);
+  const double2 pos = A.ptPos[S.gp];
+  R.px = pos.x;
+  R.py = pos.y;
+  if (S.kind == 3) scan_excluding(R, lo, hi, S.t.f1, S.t.f2, S);
+  else scan_window(R, lo, hi, true, true, S);
+
+  // look-backs recorded during the scan; runs they find queue their own
+  // follow-ups, so no further look-backs are recorded here
+  for (int k = 0; k < kMaxLookBacks; k++) {
+    if (k >= S.nBack) break;
+    const LookBack lb = k == 0 ? S.back0 : S.back1;
+    S.t.row = lb.row;
+    S.t.label = lb.label;
+    S.kind = 3;
+    S.dir = lb.dir;
+    S.xa = lb.xa;
+    S.xb = lb.xb;
+    R.y = ny_to_y(C, lb.row);
+    const long long l2 = S.xa < 0 ? 0 : S.xa, h2 = S.xb > S.N - 1 ? S.N - 1 : S.xb;
+    if (l2 <= h2) scan_excluding(R, l2, h2, lb.k1, lb.k2, S);
+  }
+  S.nBack = 0;
+}
+
+__global__ void __launch_bounds__(256) fill_kernel(FillArgs A) {
+  const int ln = __lane_id();
+  const long long t0 = (long long)__builtin_amdgcn_s_memrealtime();
+  TaskState S;
+  S.A = &A;
+  S.evals = 0;
+  S.nIns = 0;
+  S.nDup = 0;
+  S.accLabel = -1;
+  S.accSum = 0;
+  S.accCnt = 0;
+  S.insTicks = 0;
+  unsigned long long nTasks = 0, busyTicks = 0, idlePolls = 0;
+  bool have = false;
+  int backoff = 0;
+  while (true) {
+    if (!have) {
+      if (aldi(A.abortFlag)) break;
+      int st = A.nSeedTasks;
+      if (ln == 0 && aldi(A.seedNext) < A.nSeedTasks) st = atomicAdd(A.seedNext, 1);
+      st = bcast0(st);
+      if (st < A.nSeedTasks) {
+        const int2 sk = A.seedTasks[st];
+        const int gp = A.batchPts[sk.x];
+        const int2 px = A.seeds[(size_t)gp * kMaxSeeds + sk.y];
+        const CurveDev& C = A.curves[A.ptCurve[gp]];
+        // the reference skips seeds outside rows 1..N-1 (lcirs.cc:386-389)
+        const bool rowOk = px.y > 0 && (long long)px.y < C.N;
+        S.t = make_task(sk.x, rowOk ? px.y : -1, px.x, px.x, sk.x, 1, 1, 0, 0);
+        have = true;
+      } else if (q_pop(A, S.t)) {
+        have = true;
+      } else {
+        long long pend = 0;
+        if (ln == 0) pend = (long long)ald((unsigned long long*)A.pending);
+        pend = bcast0(pend);
+        if (pend <= 0) break;
+        const long long now = (long long)__builtin_amdgcn_s_memrealtime();
+        if (now - t0 > A.timeLimitTicks) {
+          if (ln == 0) atomicExch(A.abortFlag, 1);
+          break;
+        }
+        // idle: back off so that polling does not crowd the shared counters
+        idlePolls++;
+        if (backoff < 6) backoff++;
+        for (int k = 0; k < (1 << backoff); k++) __builtin_amdgcn_s_sleep(32);
+        continue;
+      }
+    }
+    backoff = 0;
+    nTasks++;
+    const unsigned long long tb0 = __builtin_amdgcn_s_memrealtime();
+    process_task(S);
+    busyTicks += __builtin_amdgcn_s_memrealtime() - tb0;
+    if (S.haveCont) {
+      S.t = S.cont;  // the kept follow-up inherits this task's pending unit
+    } else {
+      flush_acc(S);
+      if (ln == 0) {
+        mem_order();
+        afaddl(A.pending, -1);
+      }
+      have = false;
+    }
+    if ((nTasks & 255) == 0) {
+      const long long now = (long long)__builtin_amdgcn_s_memrealtime();
+      if (now - t0 > A.timeLimitTicks) {
+        if (ln == 0) atomicExch(A.abortFlag, 1);
+        break;
+      }
+    }
+  }
+  flush_acc(S);
+  if (ln == 0) {
+    atomicAdd(&A.stats[0], nTasks);
+    atomicAdd(&A.stats[1], S.evals);
+    atomicAdd(&A.stats[2], S.nIns);
+    atomicAdd(&A.stats[3], S.nDup);
+    atomicAdd(&A.stats[4], busyTicks);
+    atomicAdd(&A.stats[5], S.insTicks);
+    atomicAdd(&A.stats[6], idlePolls);
+    atomicAdd(&A.stats[7], 1ull);
+  }
+}
+
+int launch_fill(const FillArgs& a, int nWorkgroups, hipStream_t s) {
+  hipLaunchKernelGGL(fill_kernel, dim3(nWorkgroups), dim3(256), 0, s, a);
+  return (int)hipGetLastError();
+}
+
+__global__ void clear_u64_kernel(unsigned long long* p, unsigned long long n, unsigned long long v) {
+  for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < n;
+       i += (unsigned long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+int launch_clear_u64(unsigned long long* p, unsigned long long n, unsigned long long v,
+                     hipStream_t s) {
+  if (n == 0) return 0;
+  unsigned long long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(clear_u64_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+  return (int)hipGetLastError();
+}
+
+// Resets the run table (word0 = 0, word1 = ~0), labels, queue slots and
+// counters before a fill launch.  A plain kernel (no memset nodes), so that
+// the sequence can be captured in a graph.
+__global__ void fill_reset_kernel(FillArgs A) {
+  const unsigned long long tid = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x;
+  const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+  for (unsigned long long i = tid; i <= A.hmask; i += stride) {
+    A.htab[2 * i] = 0;
+    A.htab[2 * i + 1] = kEmpty1;
+  }
+  for (unsigned long long i = tid; i < (unsigned long long)A.labelCap; i += stride) {
+    A.lparent[i] = (int)i;
+    A.lpoint[i] = i < (unsigned long long)A.nBatch ? (int)i : -1;
+    A.lcount[i] = 0;
+    A.lsum[i] = 0;
+  }
+  for (unsigned long long i = tid; i < A.qcap; i += stride) A.qseq[i] = i;
+  if (tid == 0) {
+    *A.qhead = 0;
+    *A.qtail = 0;
+    *A.pending = A.nSeedTasks;
+    *A.seedNext = 0;
+    *A.labelNext = A.nBatch;
+    *A.abortFlag = 0;
+    for (int k = 0; k < 8; k++) A.stats[k] = 0;
+  }
+}
+
+int launch_fill_reset(const FillArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(fill_reset_kernel, dim3(2048), dim3(256), 0, s, a);
+  return (int)hipGetLastError();
+}
+
+// Adds every label whose class holds the reference-seed label of its point
+// to that point's totals.
+__global__ void resolve_kernel(FillArgs A, int nLabels, unsigned long long* ptCount,
+                               long long* ptSum) {
+  const int L = blockIdx.x * blockDim.x + threadIdx.x;
+  if (L >= nLabels) return;
+  const int b = A.lpoint[L];
+  if (b < 0 || b >= A.nBatch) return;
+  if (A.lcount[L] == 0) return;
+  const int r = find_root(A.lparent, L);
+  const int rb = find_root(A.lparent, b);
+  if (r != rb) return;
+  atomicAdd(&ptCount[b], A.lcount[L]);
+  atomicAdd((unsigned long long*)&ptSum[b], (unsigned long long)A.lsum[L]);
+}
+
+int launch_resolve(const FillArgs& a, int nLabels, unsigned long long* ptCount, long long* ptSum,
+                   hipStream_t s) {
+  if (nLabels <= 0) return 0;
+  hipLaunchKernelGGL(resolve_kernel, dim3((nLabels + 255) / 256), dim3(256), 0, s, a, nLabels,
+                     ptCount, ptSum);
+  return (int)hipGetLastError();
+}
+
+}  // namespace tl

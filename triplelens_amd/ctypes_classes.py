@@ -1,0 +1,71 @@
+"""Python classes over liblcirs.so with the names, arguments and behaviour of
+the reference's ctypes wrapper (Examples/ctypes_classes.py:128-193).
+
+``LC_irs`` is the reference class: same constructor arguments (the eighth,
+named ``img_plane_size`` there, is pointsPerRadius, lcirs.cc:524), same
+methods, output written into a caller-allocated float64 array of length
+``lc_len`` by ``copy_lc``.  The only differences are that the library is
+found next to this package instead of at '../bin/liblcirs.so' and that
+errors raise instead of being printed.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+from numpy.ctypeslib import ndpointer
+
+from . import engine as _engine
+
+
+class LC_irs(object):
+    def __init__(self, a, b, th, m2, m3, source_size, lc_len, img_plane_size):
+        self.lib_lc = _engine.load()
+        self.lib_lc.copy_lc.argtypes = [ctypes.c_void_p,
+                                        ndpointer(dtype=np.double, flags="C_CONTIGUOUS")]
+        self.obj = self.lib_lc.lcirs_new(a, b, th, m2, m3, source_size, lc_len, img_plane_size)
+        if not self.obj:
+            raise _engine.EngineError(_engine.last_error() or "lcirs_new failed")
+        self.lc_len = int(lc_len)
+
+    def __del__(self):
+        obj = getattr(self, "obj", None)
+        if obj:
+            self.lib_lc.lcirs_delete(obj)
+            self.obj = None
+
+    def get_lc(self, iniX, iniY, finX, finY):
+        self.lib_lc.get_lc(self.obj, iniX, iniY, finX, finY)
+        _engine._check()
+
+    def get_lc_irs(self, iniX, iniY, finX, finY):
+        self.lib_lc.get_lc_irs(self.obj, iniX, iniY, finX, finY)
+        _engine._check()
+
+    def copy_lc(self, lc_vec):
+        self.lib_lc.copy_lc(self.obj, lc_vec)
+
+    def set_limb_darkening(self, model_type, model_parameter):
+        if isinstance(model_type, str):
+            model_type = model_type.encode()
+        self.lib_lc.set_limb_darkening(self.obj, model_type, model_parameter)
+        _engine._check()
+
+    def set_amoeba_printout(self, amoeba_filename, par_filename):
+        if isinstance(amoeba_filename, str):
+            amoeba_filename = amoeba_filename.encode()
+        if isinstance(par_filename, str):
+            par_filename = par_filename.encode()
+        self.lib_lc.set_amoeba_printout(self.obj, amoeba_filename, par_filename)
+        _engine._check()
+
+    # additions
+    def ray_counts(self):
+        out = np.zeros(self.lc_len + 1, np.uint64)
+        _engine._check(self.lib_lc.lcirs_copy_ray_counts(self.obj, out.ctypes.data))
+        return out
+
+    def lc_full(self):
+        out = np.zeros(self.lc_len + 1)
+        _engine._check(self.lib_lc.lcirs_copy_lc_full(self.obj, out.ctypes.data))
+        return out

@@ -1,0 +1,188 @@
+"""ctypes binding of liblcirs.so (the C ABI in include/tl_lcirs.h).
+
+The library is the product: a HIP engine for gfx950.  Loading fails loudly if
+it has not been built, and every GPU call raises when the engine reports an
+error; there is no CPU path behind it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import build as _build
+
+LIB_PATH = _build.LIB
+MAX_SEEDS = 64
+
+FLAG_EDGE = 1
+FLAG_SEED_OVERFLOW = 2
+FLAG_HASH_FULL = 4
+FLAG_LABEL_FULL = 8
+FLAG_QUEUE_FULL = 16
+FLAG_TIMEOUT = 32
+FLAG_ROW_OVERFLOW = 64
+
+_D = ctypes.c_double
+_P = ctypes.c_void_p
+_lib = None
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Loads liblcirs.so (once) and declares its functions."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise EngineError(f"{path} is missing: build it with triplelens_amd/build.py")
+    lib = ctypes.CDLL(path)
+    lib.lcirs_new.argtypes = [_D] * 6 + [ctypes.c_uint, ctypes.c_long]
+    lib.lcirs_new.restype = _P
+    lib.lcirs_delete.argtypes = [_P]
+    for f in ("get_lc", "get_lc_irs"):
+        getattr(lib, f).argtypes = [_P, _D, _D, _D, _D]
+        getattr(lib, f).restype = None
+    lib.copy_lc.argtypes = [_P, _P]
+    lib.set_limb_darkening.argtypes = [_P, ctypes.c_char_p, _D]
+    lib.set_amoeba_printout.argtypes = [_P, ctypes.c_char_p, ctypes.c_char_p]
+    for f in ("lcirs_copy_lc_full", "lcirs_copy_ray_counts", "lcirs_copy_flags",
+              "lcirs_get_grid"):
+        getattr(lib, f).argtypes = [_P, _P]
+        getattr(lib, f).restype = ctypes.c_int
+    lib.lcirs_copy_cc_ca.argtypes = [_P, _P, _P]
+    lib.lcirs_copy_seeds.argtypes = [_P, _P, _P]
+    lib.lcirs_batch_irs.argtypes = [ctypes.c_int, _P, _P]
+    lib.lcirs_fill_seeds.argtypes = [_P, ctypes.c_int, _P, _P, _P, _P, _P, _P]
+    lib.tl_last_stats.argtypes = [_P]
+    lib.tl_set_option.argtypes = [ctypes.c_char_p, _D]
+    lib.tl_last_error.restype = ctypes.c_char_p
+    lib.tl_device_info.argtypes = [ctypes.c_char_p, ctypes.c_int, _P]
+    _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    return load().tl_last_error().decode()
+
+
+def _check(rc: int = 0):
+    err = last_error()
+    if rc != 0 or err:
+        raise EngineError(err or f"engine call failed ({rc})")
+
+
+def set_option(name: str, value: float) -> None:
+    if load().tl_set_option(name.encode(), float(value)) != 0:
+        raise ValueError(f"unknown engine option {name}")
+
+
+def last_stats() -> dict:
+    out = np.zeros(12)
+    load().tl_last_stats(out.ctypes.data)
+    keys = ["tasks", "evals", "runs", "dups", "seed_ms", "fill_ms", "fill_launches", "total_ms",
+            "busy_ms", "insert_ms", "idle_polls", "waves"]
+    return dict(zip(keys, out.tolist()))
+
+
+def device_info():
+    name = ctypes.create_string_buffer(128)
+    cus = ctypes.c_int(0)
+    _check(load().tl_device_info(name, 128, ctypes.byref(cus)))
+    return name.value.decode(), cus.value
+
+
+class LightCurve:
+    """One LightCurveIRS object of the engine."""
+
+    def __init__(self, a, b, th, m2, m3, rho, lc_len, ppr):
+        lib = load()
+        self._lib = lib
+        self.h = lib.lcirs_new(a, b, th, m2, m3, rho, lc_len, ppr)
+        if not self.h:
+            raise EngineError(last_error() or "lcirs_new failed")
+        self.lc_len = int(lc_len)
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            self._lib.lcirs_delete(h)
+            self.h = None
+
+    def set_limb_darkening(self, model: str, v: float):
+        self._lib.set_limb_darkening(self.h, model.encode(), float(v))
+        _check()
+
+    def lc_irs(self, sx, sy, ex, ey):
+        """Runs get_lc_irs; returns (magnification, rays, flags), lcLength+1 each."""
+        self._lib.get_lc_irs(self.h, sx, sy, ex, ey)
+        _check()
+        return self.results()
+
+    def results(self):
+        n = self.lc_len + 1
+        amp = np.zeros(n)
+        rays = np.zeros(n, np.uint64)
+        flags = np.zeros(n, np.uint32)
+        _check(self._lib.lcirs_copy_lc_full(self.h, amp.ctypes.data))
+        _check(self._lib.lcirs_copy_ray_counts(self.h, rays.ctypes.data))
+        _check(self._lib.lcirs_copy_flags(self.h, flags.ctypes.data))
+        return amp, rays, flags
+
+    def point_lc(self, sx, sy, ex, ey):
+        self._lib.get_lc(self.h, sx, sy, ex, ey)
+        _check()
+        out = np.zeros(self.lc_len)
+        self._lib.copy_lc(self.h, out.ctypes.data)
+        return out
+
+    def copy_lc(self):
+        out = np.zeros(self.lc_len)
+        self._lib.copy_lc(self.h, out.ctypes.data)
+        return out
+
+    def grid(self):
+        g = np.zeros(36)
+        _check(self._lib.lcirs_get_grid(self.h, g.ctypes.data))
+        return g
+
+    def cc_ca(self):
+        cc = np.zeros(6 * 500, np.complex128)
+        ca = np.zeros(6 * 500, np.complex128)
+        _check(self._lib.lcirs_copy_cc_ca(self.h, cc.ctypes.data, ca.ctypes.data))
+        return cc.reshape(6, 500), ca.reshape(6, 500)
+
+    def seeds(self):
+        n = self.lc_len + 1
+        ns = np.zeros(n, np.int32)
+        pix = np.zeros((n, MAX_SEEDS, 2), np.int32)
+        _check(self._lib.lcirs_copy_seeds(self.h, ns.ctypes.data, pix.ctypes.data))
+        return ns, pix
+
+    def fill_seeds(self, pos, nseeds, pix):
+        """GPU fill of given pixel seeds at given source positions."""
+        pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 2)
+        n = len(pos)
+        nseeds = np.ascontiguousarray(nseeds, np.int32)
+        pix = np.ascontiguousarray(pix, np.int32).reshape(n, MAX_SEEDS, 2)
+        rays = np.zeros(n, np.uint64)
+        amp = np.zeros(n)
+        flags = np.zeros(n, np.uint32)
+        _check(self._lib.lcirs_fill_seeds(self.h, n, pos.ctypes.data, nseeds.ctypes.data,
+                                          pix.ctypes.data, rays.ctypes.data, amp.ctypes.data,
+                                          flags.ctypes.data))
+        return amp, rays, flags
+
+
+def batch_irs(curves, trajs):
+    """One engine run over many LightCurve objects; trajs: (n, 4)."""
+    lib = load()
+    n = len(curves)
+    arr = (ctypes.c_void_p * n)(*[c.h for c in curves])
+    t = np.ascontiguousarray(trajs, np.float64).reshape(n, 4)
+    _check(lib.lcirs_batch_irs(n, arr, t.ctypes.data))
+    return [c.results() for c in curves]
