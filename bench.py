@@ -1,0 +1,197 @@
+#!/usr/bin/env python3
+"""Light-curve benchmark of the MI355X IRS engine.
+
+One step = one extended-source light curve of the C2 workload computed by
+the engine (get_lc_irs through the C ABI: seed search, flood fill and ray
+map on the GPU): a seeded triple lens (separations log-uniform [0.5, 2],
+m2, m3 log-uniform [1e-3, 0.3]), linear limb darkening u = 0.5,
+rho = 5e-4, pixel rho/200, 4096 positions on a straight line of length 2
+through the central caustic.  The printed value is light-curve points per
+second.
+
+The dominant kernel's roofline figures are measured in the same run: the
+fill kernel's duration comes from HIP events around it on the engine's
+stream, and its algorithmic work is 43 FP64 operations per counted ray.
+
+The CPU baseline is the reference LightCurveIRS compiled from its own
+sources (oracle/_ref, pinned flags -O2 -ffp-contract=off -mavx) run over the
+same curve on the host cores, one contiguous part of the trajectory per
+core; it is timed before the GPU is first used.  Where the reference build
+is absent the C restatement in oracle/ is timed instead.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import multiprocessing as mp
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+FLOP_PER_RAY = 43          # irsSIMD 35 + pixel position 2 + profile and sum 6
+FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 vector/matrix peak
+METRIC = "IRS light-curve points/s at ≤1e-3 rel err: surrogate+IRS vs plain IRS, 1 GPU"
+
+
+def c2_curve():
+    """C2 geometry; the central caustic is located with the engine's own
+    critical curves (host set-up, no GPU work)."""
+    from triplelens_amd import engine as E
+    from triplelens_amd import workloads as W
+
+    def ca_fn(a, b, th, m2, m3):
+        return E.LightCurve(a, b, th, m2, m3, 5e-4, 1, 200).cc_ca()[1]
+
+    return W.c2(seed=2, ca_fn=ca_fn)
+
+
+# ---------------------------------------------------------------- CPU baseline
+def _ref_worker(args):
+    import ctypes
+    lib_path, lens, rho, ppr, u, traj, n = args
+    D = ctypes.c_double
+    lib = ctypes.CDLL(lib_path)
+    lib.tlref_lc_irs.argtypes = [D] * 6 + [ctypes.c_uint, ctypes.c_long, ctypes.c_int, D, D, D,
+                                           D, D, ctypes.c_void_p, ctypes.c_void_p]
+    amp = np.zeros(n)
+    rays = np.zeros(n, np.uint64)
+    t = time.perf_counter()
+    lib.tlref_lc_irs(*lens, rho, n - 1, ppr, 1, u, *traj, amp.ctypes.data, rays.ctypes.data)
+    return time.perf_counter() - t, int(rays.sum()), n
+
+
+def _split(c, parts):
+    """Contiguous parts of the trajectory: (start, end, points) per part."""
+    n = c.n_points
+    bounds = np.linspace(0, n, parts + 1).astype(int)
+    out = []
+    for k in range(parts):
+        i0, i1 = bounds[k], bounds[k + 1]
+        if i1 - i0 < 2:
+            continue
+        t0, t1 = i0 / c.lc_len, (i1 - 1) / c.lc_len
+        out.append(((c.sx + (c.ex - c.sx) * t0, c.sy + (c.ey - c.sy) * t0,
+                     c.sx + (c.ex - c.sx) * t1, c.sy + (c.ey - c.sy) * t1), i1 - i0))
+    return out
+
+
+def cpu_baseline(c, cores):
+    from oracle import oracle as orc
+    parts = _split(c, 8 * cores)
+    if os.path.exists(orc.REF_SO):
+        jobs = [(orc.REF_SO, c.lens(), c.rho, c.ppr, c.u, tr, n) for tr, n in parts]
+        # largest parts first so that the pool stays balanced
+        jobs.sort(key=lambda j: -j[-1])
+        t = time.perf_counter()
+        with mp.get_context("fork").Pool(cores) as pool:
+            res = list(pool.imap_unordered(_ref_worker, jobs, chunksize=1))
+        wall = time.perf_counter() - t
+        kind = "reference"
+        one_core = sum(r[2] for r in res) / sum(r[0] for r in res)
+    else:
+        O = orc.Oracle()
+        params = np.array([[*c.lens(), c.rho, c.ppr, c.u] for _ in parts])
+        trajs = np.array([tr for tr, _ in parts])
+        lens = np.array([n - 1 for _, n in parts])
+        t = time.perf_counter()
+        O.batch(params, trajs, lens, cores)
+        wall = time.perf_counter() - t
+        kind = "port"
+        one_core = None
+    pts = sum(n for _, n in parts)
+    out = {"value": float(pts / wall), "unit": "points/s", "cores": cores, "kind": kind,
+           "sample": f"the full {c.n_points}-point C2 curve split into {len(parts)} contiguous "
+                     f"parts over {cores} worker processes, wall {wall:.1f} s"}
+    if one_core is not None:
+        out["value_1core"] = float(one_core)
+    return out
+
+
+# ---------------------------------------------------------------- main
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-cores", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box in this setup has a single MI355X and the "
+              "engine has no multi-GPU path; run with --gpus 1", file=sys.stderr)
+        return 2
+
+    c = c2_curve()
+    cpu = None if args.no_cpu else cpu_baseline(c, args.cpu_cores)
+
+    from triplelens_amd import engine as E
+    name, cus = E.device_info()
+    lc = E.LightCurve(*c.lens(), c.rho, c.lc_len, c.ppr)
+    lc.set_limb_darkening("linear", c.u)
+    for _ in range(args.warmup):
+        lc.lc_irs(*c.traj())
+    fill_ms = launches = rays = 0.0
+    flags_all = 0
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        _, r, f = lc.lc_irs(*c.traj())
+        st = E.last_stats()
+        fill_ms += st["fill_ms"]
+        launches += st["fill_launches"]
+        rays += float(r.sum())
+        flags_all |= int(np.bitwise_or.reduce(f))
+    wall = time.perf_counter() - t
+    ms_per_step = 1e3 * wall / args.steps
+    value = c.n_points * args.steps / wall
+    fill_launch_ms = fill_ms / launches
+    rays_per_launch = rays / launches
+    achieved = FLOP_PER_RAY * rays_per_launch / (fill_launch_ms * 1e-3) / 1e12
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "C2 plain IRS: seeded triple lens, rho=5e-4, linear limb darkening "
+                        "u=0.5, pixel rho/200, 4096-point trajectory through the central caustic",
+            "n_points": c.n_points,
+            "geometry_seed": 2,
+            "lens": {"a": c.a, "b": c.b, "th": c.th, "m2": c.m2, "m3": c.m3},
+            "mode": "plain IRS (no surrogate)",
+            "device": name,
+            "cus": cus,
+            "rays_per_step": rays / args.steps,
+            "status_flags": flags_all,
+        },
+        "roofline": {
+            "bound": "mfma",
+            "achieved": achieved,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS,
+            "traffic": None,
+            "kernel": "fill_kernel",
+            "kernel_ms": fill_launch_ms,
+            "flop_per_ray": FLOP_PER_RAY,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -65,6 +65,12 @@ class Oracle:
         lib.tlo_csqrt.argtypes = [_D, _D, _P]
         lib.tlo_cdiv.argtypes = [_D, _D, _D, _D, _P]
         lib.tlo_batch.argtypes = [ctypes.c_int, _P, _P, _P, _P, ctypes.c_int, _P, _P]
+        lib.tlo_libm_hypot.argtypes = [_D, _D]
+        lib.tlo_libm_hypot.restype = _D
+        lib.tlo_libm_csqrt.argtypes = [_D, _D, _P]
+        lib.tlo_libm_cdiv.argtypes = [_D, _D, _D, _D, _P]
+        lib.tlo_seeds_margin.argtypes = [_P, _D, _D, _D, _D, ctypes.c_int, _P, _P, _P, _P]
+        lib.tlo_fill_stats.argtypes = [_P]
         self.lib = lib
 
     # -- light curve object -------------------------------------------------
@@ -171,6 +177,18 @@ class LC:
         ni = np.zeros(n, np.int32)
         self._o.lib.tlo_seeds(self.h, sx, sy, ex, ey, max_seeds, _ptr(xy), _ptr(ns), _ptr(ni))
         return xy, ns, ni
+
+    def seeds_margin(self, sx, sy, ex, ey, max_seeds=64):
+        """Seeds as seeds(), plus per point the distance in decades between
+        the closest root residual and the 1e-6 image threshold."""
+        n = self.lc_len + 1
+        xy = np.zeros((n, max_seeds, 2))
+        ns = np.zeros(n, np.int32)
+        ni = np.zeros(n, np.int32)
+        mg = np.zeros(n)
+        self._o.lib.tlo_seeds_margin(self.h, sx, sy, ex, ey, max_seeds, _ptr(xy), _ptr(ns),
+                                     _ptr(ni), _ptr(mg))
+        return xy, ns, ni, mg
 
     def fill_point(self, px, py, seeds_xy):
         s = np.ascontiguousarray(seeds_xy, np.float64).reshape(-1, 2)

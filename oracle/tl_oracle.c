@@ -36,6 +36,7 @@
 #define _GNU_SOURCE
 #include "tl_oracle.h"
 
+#include <complex.h>
 #include <float.h>
 #include <math.h>
 #include <pthread.h>
@@ -182,6 +183,19 @@ static cx csqrt_(cx z) {
 }
 
 double tlo_hypot(double x, double y) { return hyp(x, y); }
+
+/* The system library's own routines, for tests of the restatements above:
+ * libm hypot and csqrt, and C complex division (libgcc __divdc3). */
+double tlo_libm_hypot(double x, double y) { return hypot(x, y); }
+void tlo_libm_csqrt(double re, double im, double* out) {
+  double complex r = csqrt(CMPLX(re, im));
+  out[0] = creal(r); out[1] = cimag(r);
+}
+void tlo_libm_cdiv(double a, double b, double c, double d, double* out) {
+  volatile double complex n = CMPLX(a, b), q = CMPLX(c, d);
+  double complex r = n / q;
+  out[0] = creal(r); out[1] = cimag(r);
+}
 void tlo_csqrt(double re, double im, double* out) {
   cx r = csqrt_(mk(re, im));
   out[0] = r.re; out[1] = r.im;
@@ -498,11 +512,27 @@ static int img_check(const lens_t* L, cx img, cx zeta, double tol) {
   return cabs_(csub(t, zeta)) < tol;
 }
 
+/* Smallest distance, in decades, between a root's lens-equation residual
+ * and the 1e-6 acceptance threshold over the image tests since the last
+ * reset (test instrumentation: a seed decision this close to the threshold
+ * depends on rounding). */
+static __thread double g_margin = 1e300;
+
+static double img_residual(const lens_t* L, cx img, cx zeta) {
+  cx t = csub(img, cdiv_rc(L->m1, cconj(csub(img, L->z1))));
+  t = csub(t, cdiv_rc(L->m2, cconj(csub(img, L->z2))));
+  t = csub(t, cdiv_rc(L->m3, cconj(csub(img, L->z3))));
+  return cabs_(csub(t, zeta));
+}
+
 static void img_classify(imgpoint_t* ip, cx zeta) {
   ip->nImg = 0;
   for (int k = 0; k < 10; k++) {
     ip->isImg[k] = img_check(&ip->L, ip->roots[k], zeta, 1.0e-6);
     if (ip->isImg[k]) ip->imgs[ip->nImg++] = ip->roots[k];
+    double r = img_residual(&ip->L, ip->roots[k], zeta);
+    double m = r > 0.0 ? fabs(log10(r / 1.0e-6)) : 1e300;
+    if (m < g_margin) g_margin = m;
   }
 }
 
@@ -1060,16 +1090,27 @@ int tlo_get_lc_irs(tlo_lc* lc, double sx, double sy, double ex, double ey,
   return 0;
 }
 
+int tlo_seeds_margin(tlo_lc* lc, double sx, double sy, double ex, double ey,
+                     int maxSeeds, double* seedXY, int* nSeeds, int* nImg, double* margin);
 int tlo_seeds(tlo_lc* lc, double sx, double sy, double ex, double ey,
               int maxSeeds, double* seedXY, int* nSeeds, int* nImg) {
+  return tlo_seeds_margin(lc, sx, sy, ex, ey, maxSeeds, seedXY, nSeeds, nImg, NULL);
+}
+
+/* As tlo_seeds, and margin[i] = the smallest distance in decades between any
+ * root residual of point i's solves and the 1e-6 image threshold. */
+int tlo_seeds_margin(tlo_lc* lc, double sx, double sy, double ex, double ey,
+                     int maxSeeds, double* seedXY, int* nSeeds, int* nImg, double* margin) {
   if (!lc || maxSeeds < 0) return -1;
   cx s = mk(sx, sy), e = mk(ex, ey);
   cx* seeds = NULL;
   int cap = 0;
   for (unsigned i = 0; i <= lc->lcLength; i++) {
     cx pos = traj_pos(s, e, i, lc->lcLength);
+    g_margin = 1e300;
     int n = point_seeds(lc, pos, &seeds, &cap, nImg ? &nImg[i] : NULL);
     if (n < 0) { free(seeds); return -2; }
+    if (margin) margin[i] = g_margin;
     nSeeds[i] = n;
     for (int k = 0; k < n && k < maxSeeds; k++) {
       seedXY[2 * ((size_t)i * maxSeeds + k)] = seeds[k].re;

@@ -47,6 +47,12 @@ int tlo_get_lc_irs(tlo_lc* lc, double sx, double sy, double ex, double ey,
 int tlo_seeds(tlo_lc* lc, double sx, double sy, double ex, double ey,
               int maxSeeds, double* seedXY, int* nSeeds, int* nImg);
 
+/* As tlo_seeds, plus margin[i]: the smallest distance, in decades, between
+ * the lens-equation residual of any root found for point i (centre and
+ * caustic trial points) and the 1e-6 image threshold. */
+int tlo_seeds_margin(tlo_lc* lc, double sx, double sy, double ex, double ey,
+                     int maxSeeds, double* seedXY, int* nSeeds, int* nImg, double* margin);
+
 /* Fill one source position from given seeds (image-plane coordinates), as the
  * seed loop of getLCIRS does (lcirs.cc:168-181).  Returns rays, sum of
  * brightness (before ampScale) and the edge flag. */
@@ -88,10 +94,18 @@ double tlo_irs(const tlo_lc* lc, double x, double y, double sx, double sy);
 int tlo_point_amp(double a, double b, double th, double m2, double m3, int n,
                   const double* pos, double* out);
 
-/* The complex-arithmetic helpers, exposed for tests against libm/libgcc. */
+/* The complex-arithmetic helpers, exposed for tests against libm/libgcc, and
+ * the system routines they restate. */
 double tlo_hypot(double x, double y);
 void tlo_csqrt(double re, double im, double* out);
 void tlo_cdiv(double a, double b, double c, double d, double* out);
+double tlo_libm_hypot(double x, double y);
+void tlo_libm_csqrt(double re, double im, double* out);
+void tlo_libm_cdiv(double a, double b, double c, double d, double* out);
+
+/* Amoeba statistics of the last fill (ranges, rows, max ranges per row, min
+ * row, max row). */
+int tlo_fill_stats(long* out);
 
 /* Multi-threaded batch of independent light curves (CPU baseline):
  * curve k uses params[8*k .. 8*k+7] = a, b, th, m2, m3, rho, ppr, v and

@@ -1,0 +1,149 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures of tests/golden/ from the reference itself.
+
+Runs in the container that holds the reference sources: oracle/Makefile
+compiles them (unchanged, pinned flags -O2 -ffp-contract=off -mavx) into
+oracle/_ref/libtlref.so, and this script drives that library through
+oracle/ref_driver.cc.  The fixtures are data only (inputs and the
+reference's outputs); the tests read them without the reference.
+
+Fixtures:
+  c1.npz          C1: grid, critical curves, per-point
+                  ray counts, magnifications and seeds of all 200 points.
+  c2_ppr20.npz    C2 geometry, all 4096 points, pixel rho/20 (rays and A).
+  c2_window.npz   C2 geometry, 65 points around the central caustic at the
+                  full pixel rho/200.
+  c3_ppr20.npz    the first 16 C3 curves at pixel rho/20.
+  laguerre.npz    solveRoots on random degree-6 and degree-10 polynomials,
+                  fresh and from initial roots.
+  coeffs.npz      image-polynomial coefficients at random source positions.
+  images.npz      the image chain (roots, accepted flags) along C1.
+  point_amp.npz   point-source magnification (getPointAmp) along C1.
+"""
+from __future__ import annotations
+
+import math
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle import oracle as orc  # noqa: E402
+from triplelens_amd import workloads as W  # noqa: E402
+
+
+def pack_seeds(xy, ns):
+    """Ragged seeds as (offsets, flat xy)."""
+    off = np.zeros(len(ns) + 1, np.int64)
+    off[1:] = np.cumsum(np.minimum(ns, xy.shape[1]))
+    flat = np.concatenate([xy[i, :min(n, xy.shape[1])] for i, n in enumerate(ns)])
+    return off, flat
+
+
+def curve_fixture(R, c, ppr=None):
+    ppr = c.ppr if ppr is None else ppr
+    amp, rays = R.lc_irs(*c.lens(), c.rho, c.lc_len, ppr, c.u, *c.traj())
+    grid = R.grid(*c.lens(), c.rho, c.lc_len, ppr, c.u)
+    return amp, rays, grid
+
+
+def main():
+    orc.build(ref=True)
+    R = orc.Reference()
+    O = orc.Oracle()
+    rng = np.random.default_rng(12345)
+    out = {}
+
+    # C1 ------------------------------------------------------------------
+    c = W.c1()
+    amp, rays, grid = curve_fixture(R, c)
+    xy, ns, ni = R.seeds(*c.lens(), c.rho, c.lc_len, c.ppr, *c.traj(), max_seeds=64)
+    off, flat = pack_seeds(xy, ns)
+    cc, ca = R.ccc(*c.lens())
+    np.savez_compressed(os.path.join(HERE, "c1.npz"), params=np.array(
+        [*c.lens(), c.rho, c.ppr, c.u, *c.traj(), c.n_points]), amp=amp, rays=rays, grid=grid,
+        seed_off=off, seed_xy=flat, n_img=ni, cc=cc, ca=ca)
+    out["c1"] = int(rays.sum())
+
+    # C2 ------------------------------------------------------------------
+    def ca_fn(a, b, th, m2, m3):
+        return R.ccc(a, b, th, m2, m3)[1]
+
+    c2 = W.c2(seed=2, ca_fn=ca_fn)
+    amp, rays, grid = curve_fixture(R, c2, ppr=20)
+    np.savez_compressed(os.path.join(HERE, "c2_ppr20.npz"), params=np.array(
+        [*c2.lens(), c2.rho, 20, c2.u, *c2.traj(), c2.n_points]), amp=amp, rays=rays, grid=grid)
+    out["c2_ppr20"] = int(rays.sum())
+    # a 65-point window where the magnification peaks, at full resolution
+    i0 = int(np.argmax(amp)) - 32
+    i0 = max(0, min(i0, c2.n_points - 65))
+    t0, t1 = i0 / c2.lc_len, (i0 + 64) / c2.lc_len
+    win = (c2.sx + (c2.ex - c2.sx) * t0, c2.sy + (c2.ey - c2.sy) * t0,
+           c2.sx + (c2.ex - c2.sx) * t1, c2.sy + (c2.ey - c2.sy) * t1)
+    amp, rays = R.lc_irs(*c2.lens(), c2.rho, 64, c2.ppr, c2.u, *win)
+    np.savez_compressed(os.path.join(HERE, "c2_window.npz"), params=np.array(
+        [*c2.lens(), c2.rho, c2.ppr, c2.u, *win, 65]), amp=amp, rays=rays)
+    out["c2_window"] = int(rays.sum())
+
+    # C3 ------------------------------------------------------------------
+    rows = []
+    amps, raysl = [], []
+    for k, cc3 in enumerate(W.c3()[:16]):
+        amp, rays, _ = curve_fixture(R, cc3, ppr=20)
+        rows.append([*cc3.lens(), cc3.rho, 20, cc3.u, *cc3.traj(), cc3.n_points])
+        amps.append(amp)
+        raysl.append(rays)
+    off = np.zeros(len(rows) + 1, np.int64)
+    off[1:] = np.cumsum([len(a) for a in amps])
+    np.savez_compressed(os.path.join(HERE, "c3_ppr20.npz"), params=np.array(rows), off=off,
+                        amp=np.concatenate(amps), rays=np.concatenate(raysl))
+    out["c3_ppr20"] = int(sum(r.sum() for r in raysl))
+
+    # Laguerre --------------------------------------------------------------
+    cases = []
+    for deg in (6, 10):
+        for k in range(40):
+            roots = (rng.normal(size=deg) + 1j * rng.normal(size=deg)) * rng.uniform(0.2, 3.0)
+            coeffs = np.poly(roots)[::-1].astype(np.complex128)  # lowest order first
+            init = None
+            if k % 2:
+                init = np.concatenate([roots + 1e-3 * (rng.normal(size=deg) + 1j * rng.normal(size=deg)),
+                                       [0.0 + 0.0j]])
+            res = R.laguerre_solve(coeffs, init)
+            cases.append((deg, coeffs, init, res))
+    np.savez_compressed(os.path.join(HERE, "laguerre.npz"),
+                        deg=np.array([cs[0] for cs in cases]),
+                        coeffs=np.concatenate([cs[1] for cs in cases]),
+                        has_init=np.array([cs[2] is not None for cs in cases]),
+                        init=np.concatenate([cs[2] if cs[2] is not None else np.zeros(cs[0] + 1, complex)
+                                             for cs in cases]),
+                        roots=np.concatenate([cs[3] for cs in cases]))
+
+    # image-polynomial coefficients ----------------------------------------
+    rowsc = []
+    for k in range(30):
+        a, b = math.exp(rng.uniform(math.log(0.3), math.log(3))), math.exp(rng.uniform(math.log(0.3), math.log(3)))
+        th = rng.uniform(0, 2 * math.pi)
+        m2, m3 = math.exp(rng.uniform(math.log(1e-4), math.log(0.3))), math.exp(rng.uniform(math.log(1e-4), math.log(0.3)))
+        z = complex(rng.uniform(-1.5, 1.5), rng.uniform(-1.5, 1.5))
+        cr = R.coeffs(a, b, th, m2, m3, z)
+        rowsc.append(np.concatenate([[a, b, th, m2, m3, z.real, z.imag], cr.view(np.float64)]))
+    np.savez_compressed(os.path.join(HERE, "coeffs.npz"), rows=np.array(rowsc))
+
+    # images and point magnification along C1 -------------------------------
+    t = np.arange(c.n_points) / c.lc_len
+    pos = (c.sx + (c.ex - c.sx) * t) + 1j * (c.sy + (c.ey - c.sy) * t)
+    roots, is_img, n_img = R.images(*c.lens(), pos)
+    np.savez_compressed(os.path.join(HERE, "images.npz"), pos=pos, roots=roots, is_img=is_img,
+                        n_img=n_img)
+    pa = R.point_amp(*c.lens(), pos)
+    np.savez_compressed(os.path.join(HERE, "point_amp.npz"), pos=pos, amp=pa)
+    print(out)
+
+
+if __name__ == "__main__":
+    main()

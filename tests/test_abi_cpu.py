@@ -1,0 +1,85 @@
+"""The drop-in library liblcirs.so on the CPU side: it loads, exports every
+entry point include/tl_lcirs.h declares, builds the reference's grid and
+critical curves on the host bit for bit, and reports errors through
+tl_last_error instead of crashing.  No GPU work is started here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, curve_from_params, golden
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from triplelens_amd import build, engine
+    if not os.path.exists(engine.LIB_PATH):
+        build.build()
+    return engine.load()
+
+
+def test_exports_every_declared_symbol(lib):
+    hdr = open(os.path.join(ROOT, "include", "tl_lcirs.h")).read()
+    names = re.findall(r"^\s*(?:void\*|void|int|const char\*)\s+(\w+)\s*\(", hdr, re.M)
+    assert len(names) >= 16
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_reference_abi_names_present(lib):
+    # lcirs.cc:515-609 and the ctypes class LC_irs (ctypes_classes.py:128-193)
+    for n in ["lcirs_new", "get_lc", "get_lc_irs", "copy_lc", "set_limb_darkening",
+              "set_amoeba_printout"]:
+        assert hasattr(lib, n)
+
+
+def test_host_setup_matches_reference(lib, oracle):
+    from triplelens_amd import engine as E
+    for name in ["c1.npz", "c2_ppr20.npz"]:
+        g = golden(name)
+        c = curve_from_params(g["params"])
+        lc = E.LightCurve(*c.lens(), c.rho, c.lc_len, c.ppr)
+        lc.set_limb_darkening("linear", c.u)
+        assert np.array_equal(lc.grid()[:32], g["grid"])
+    g = golden("c1.npz")
+    c = curve_from_params(g["params"])
+    cc, ca = E.LightCurve(*c.lens(), c.rho, c.lc_len, c.ppr).cc_ca()
+    assert np.array_equal(cc, g["cc"]) and np.array_equal(ca, g["ca"])
+
+
+def test_c3_host_setup_matches_oracle(lib, oracle):
+    from triplelens_amd import engine as E, workloads as W
+    for c in W.c3()[:24]:
+        lc = E.LightCurve(*c.lens(), c.rho, c.lc_len, c.ppr)
+        lc.set_limb_darkening("linear", c.u)
+        lo = oracle.new(*c.lens(), c.rho, c.lc_len, c.ppr, c.u)
+        assert np.array_equal(lc.grid(), lo.grid())
+
+
+def test_invalid_input_returns_null_with_error(lib):
+    from triplelens_amd import engine as E
+    with pytest.raises(E.EngineError, match="sourceSize"):
+        E.LightCurve(1.0, 1.0, 1.0, 0.1, 0.1, 0.0, 10, 50)
+    with pytest.raises(E.EngineError, match="binary"):
+        E.LightCurve(1.0, 1.0, 1.0, 0.1, 0.0, 1e-3, 10, 50)
+
+
+def test_limb_darkening_models(lib):
+    from triplelens_amd import engine as E
+    lc = E.LightCurve(1.0, 0.9, 1.0, 0.1, 0.1, 1e-3, 10, 50)
+    g = lc.grid()
+    assert (g[32], g[33]) == (0.6, 0.4)  # default profile (limbdarkeningmodel.cc:8-16)
+    lc.set_limb_darkening("linear", 0.5)
+    i0 = 1.0 / (1.0 - 0.5 / 3.0)
+    g = lc.grid()
+    assert (g[32], g[33]) == (i0 * 0.5, i0 * 0.5)
+    assert g[7] == 1.0 / np.pi / 2500.0
+
+
+def test_ctypes_class_mirrors_reference(lib):
+    from triplelens_amd.ctypes_classes import LC_irs
+    lc = LC_irs(1.0, 0.9433981132056605, 1.0121970114513341, 0.1, 0.1, 1e-3, 199, 50)
+    lc.set_limb_darkening(b"linear", 0.0)
+    lc.set_limb_darkening("default", 0.0)

@@ -217,7 +217,7 @@ struct Options {
   int fillWorkgroups = 0;  // 0: 2 per compute unit (8 resident waves)
   int probeLimit = 4096;
   int queueLog2 = 21;
-  int scoutRows = 0;
+  int scoutRows = 8;
   int scoutJump = 256;
   double timeLimitS = 60.0;
 };

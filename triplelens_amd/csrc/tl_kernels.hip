@@ -60,6 +60,18 @@ __device__ __forceinline__ long long wave_sum(long long v) {
 template <typename T>
 __device__ __forceinline__ T bcast0(T v) { return __shfl(v, 0, 64); }
 
+// Wave-uniform values in scalar registers (the first active lane's value;
+// used where every lane holds the same value or lane 0 holds the result).
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ long long uni64(long long v) {
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)(unsigned long long)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)((unsigned long long)v >> 32));
+  return (long long)(((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ double unid(double v) {
+  return __longlong_as_double(uni64(__double_as_longlong(v)));
+}
+
 // ---------------------------------------------------------------------------
 // Pixel <-> plane mapping (lcirs.cc:355-378)
 // ---------------------------------------------------------------------------
@@ -396,17 +408,16 @@ int launch_seed(const SeedArgs& a, hipStream_t s) {
 // Run table: open addressing over 16-byte entries.
 //   word0 = (row+1) << 39 | xL << 14 | b     (0 = empty)
 //   word1 = label << 32 | xR                 (~0 = not yet written)
-// A maximal run is identified by (b, row, xL): two floods that reach the same
-// run compute the same (xL, xR), and the compare-and-swap on word0 lets
-// exactly one of them record it.
+// Rows and columns are those of the grid (0 <= row, xL < 2^25 - 1; scans
+// never leave the columns [0, N-1]) and b < 2^14.  A maximal run is
+// identified by (b, row, xL): two floods that reach the same run compute the
+// same (xL, xR), and the compare-and-swap on word0 lets exactly one of them
+// record it.
 // ---------------------------------------------------------------------------
 constexpr unsigned long long kEmpty1 = ~0ull;
 
-// word0 = (row+1) << 39 | xL << 14 | b, with rows and columns of the grid
-// (0 <= row, xL < 2^25 - 1; scans never leave the columns [0, N-1]) and
-// b < 2^14.  word1 = label << 32 | xR.
-__device__ __forceinline__ unsigned long long run_key(int b, int row, long long xl) {
-  return ((unsigned long long)(row + 1) << 39) | ((unsigned long long)xl << 14) |
+__device__ __forceinline__ unsigned long long run_key(int b, int row, int xl) {
+  return ((unsigned long long)(row + 1) << 39) | ((unsigned long long)(unsigned)xl << 14) |
          (unsigned long long)b;
 }
 __device__ __forceinline__ unsigned long long hash_run(unsigned long long key) {
@@ -421,11 +432,13 @@ __device__ __forceinline__ unsigned long long hash_run(unsigned long long key) {
 
 enum { kInsNew = 1, kInsDup = 0, kInsFull = -1 };
 
-// Records run (b, row, xl, xr) with `label`; lane 0 does the work and the
-// result is broadcast.  On a duplicate *dupLabel receives the stored label.
-__device__ __forceinline__ int insert_run(const FillArgs& A, int b, int row, long long xl, long long xr, int label,
-                          int* dupLabel) {
-  int res = kInsFull, dl = -1;
+// Records run (b, row, xl, xr) with `label` (lane 0 works, the result is
+// broadcast).  Returns kInsNew, kInsFull, or kInsDup with the stored label in
+// the upper half: (label << 8) | kInsDup.  The slow path probes from the
+// run's home slot on.
+__device__ __forceinline__ int insert_run_slow(const FillArgs& A, int b, int row, int xl, int xr,
+                                            int label) {
+  int res = kInsFull;
   if (__lane_id() == 0) {
     const unsigned long long key = run_key(b, row, xl);
     const unsigned long long w1 =
@@ -446,14 +459,31 @@ __device__ __forceinline__ int insert_run(const FillArgs& A, int b, int row, lon
           __builtin_amdgcn_s_sleep(2);
           v = ald(&A.htab[2 * idx + 1]);
         }
-        dl = v == kEmpty1 ? -1 : (int)(v >> 32);
-        res = kInsDup;
+        const int dl = v == kEmpty1 ? -1 : (int)(v >> 32);
+        res = dl < 0 ? kInsFull : ((dl << 8) | kInsDup);
         break;
       }
     }
   }
-  res = bcast0(res);
-  *dupLabel = bcast0(dl);
+  return uni(res);
+}
+
+// Fast path: the run's home slot is free (the common case at load <= 1/2).
+__device__ __forceinline__ int insert_run(const FillArgs& A, int b, int row, int xl, int xr,
+                                          int label) {
+  int res = -2;
+  if (__lane_id() == 0) {
+    const unsigned long long key = run_key(b, row, xl);
+    const unsigned long long idx = hash_run(key) & A.hmask;
+    unsigned long long expected = 0;
+    if (acas(&A.htab[2 * idx], &expected, key)) {
+      ast(&A.htab[2 * idx + 1],
+          ((unsigned long long)(unsigned)label << 32) | (unsigned long long)(unsigned)xr);
+      res = kInsNew;
+    }
+  }
+  res = uni(res);
+  if (res == -2) res = insert_run_slow(A, b, row, xl, xr, label);
   return res;
 }
 
@@ -488,7 +518,7 @@ __device__ __forceinline__ void unite(const FillArgs& A, int a, int b) {
 // publishes it is written (mem_order between them), and a slot is handed back
 // only after its payload has been read.  Lane 0 operates.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bool q_push(const FillArgs& A, const Task& t) {
+__device__ bool q_push(const FillArgs& A, const Task& t) {
   int ok = 0;
   if (__lane_id() == 0) {
     unsigned long long pos = ald(A.qtail);
@@ -517,7 +547,7 @@ __device__ __forceinline__ bool q_push(const FillArgs& A, const Task& t) {
   return bcast0(ok) != 0;
 }
 
-__device__ __forceinline__ bool q_pop(const FillArgs& A, Task& t) {
+__device__ bool q_pop(const FillArgs& A, Task& t) {
   int ok = 0;
   unsigned long long w[4] = {0, 0, 0, 0};
   if (__lane_id() == 0) {
@@ -550,25 +580,84 @@ __device__ __forceinline__ bool q_pop(const FillArgs& A, Task& t) {
   return ok != 0;
 }
 
+__device__ __forceinline__ void flag_point(const FillArgs& A, int gp, uint32_t f) {
+  if (__lane_id() == 0) atomicOr(&A.flags[gp], f);
+}
+
+// Queues a task.  The pending count is raised, and that increment completed,
+// before the task becomes visible to other waves.
+__device__ __forceinline__ void push_task(const FillArgs& A, Task f, int gp) {
+  if (__lane_id() == 0) {
+    afaddl(A.pending, 1);
+    mem_order();
+  }
+  if (!q_push(A, f)) {
+    flag_point(A, gp, kFlagQueueFull);
+    if (__lane_id() == 0) afaddl(A.pending, -1);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Row scanning
 // ---------------------------------------------------------------------------
+// Everything the ray test of one row needs, held by value so that it stays
+// in registers across the scan.
 struct RowCtx {
-  const CurveDev* C;
+  Lens L;
+  double blx, bly, step, rho2, invRho2, ldA, ldB;
   double y, px, py;
-  long long N;
+  int N;
 };
 
-// Ray test and fixed-point brightness of pixel x of the row
-// (lcirs.cc:400-407, limbdarkeningmodel.cc:45-49).
-__device__ __forceinline__ bool eval_px(const RowCtx& R, long long x, long long& val) {
-  const CurveDev& C = *R.C;
-  const double rsq = irs_ray(C.L, nx_to_x(C, x), R.y, R.px, R.py);
-  if (!(rsq < C.rho2)) return false;
-  const double r = rsq / C.rho2;
-  const double v = C.ldA + C.ldB * sqrt(1.0 - r);
-  val = (long long)(v * kSumScale + 0.5);
+// Ray test of pixel x of the row (lcirs.cc:400-402) and its brightness
+// A + B sqrt(1 - r^2) (lcirs.cc:406-407, limbdarkeningmodel.cc:45-49) as a
+// multiple of 2^-34.  The test is the reference's to the bit; the
+// brightness (a sum that only has to match to 1e-6) uses the reciprocal of
+// rho^2 and the hardware reciprocal square root, within a few ulps of the
+// reference's value, and is rounded to fixed point by adding 2^52.
+__device__ __forceinline__ bool eval_px(const RowCtx& R, int x, long long& val) {
+  const double rsq = irs_ray(R.L, R.blx + R.step * (double)x, R.y, R.px, R.py);
+  if (!(rsq < R.rho2)) return false;
+  const double y1 = 1.0 - rsq * R.invRho2;
+  const double v = R.ldA + R.ldB * (y1 * __builtin_amdgcn_rsq(y1));
+  const double t = fma(v, kSumScale, 4503599627370496.0);  // 2^52: integer in the low bits
+  val = __double_as_longlong(t) - 0x4330000000000000ll;
   return true;
+}
+
+// Result of following a run past the evaluated window.
+struct Ext {
+  int end;
+  long long sum;   // lane-local brightness of the pixels taken in
+  int cnt;
+  int evals;
+};
+
+// Follows a run from column `from` in direction dir (-1 left, +1 right) until
+// its first miss or the grid edge.
+__device__ __forceinline__ Ext extend_run(RowCtx R, int from, int dir) {
+  const int ln = __lane_id();
+  Ext r;
+  r.sum = 0;
+  r.cnt = 0;
+  r.evals = 0;
+  int xcur = from;
+  while (true) {
+    if (xcur < 0) { r.end = 0; return r; }
+    if (xcur > R.N - 1) { r.end = R.N - 1; return r; }
+    // lane l looks at xcur + dir*l
+    const int x = xcur + dir * ln;
+    const bool valid = x >= 0 && x <= R.N - 1;
+    long long val = 0;
+    const bool hit = valid && eval_px(R, x, val);
+    r.evals += __builtin_popcountll(__ballot(valid));
+    const unsigned long long m = __ballot(hit);
+    const int z = ~m == 0 ? 64 : __builtin_ctzll(~m);  // hits counted from lane 0
+    if (ln < z) r.sum += val;
+    r.cnt += z;
+    if (z < 64) { r.end = xcur + dir * (z - 1); return r; }
+    xcur += dir * 64;
+  }
 }
 
 // A look-back recorded while a row is scanned, processed after the scan:
@@ -583,8 +672,11 @@ constexpr int kMaxLookBacks = 2;  // kept in two named slots (no indexed array)
 struct TaskState {
   const FillArgs* A;
   Task t;
-  int kind, dir, gp;
-  long long N, xa, xb;      // task interval (kind 3: the child run's extent)
+  int curB;                 // batch point whose constants R holds (-1: none)
+  RowCtx R;
+  int kind, dir, gp, N;
+  int xa, xb;               // the known run next to the scanned row (kind 0:
+                            // the parent, kind 3: the child)
   Task cont;                // follow-up kept by this wave (the next sweep step)
   bool haveCont;
   int nBack;
@@ -592,13 +684,8 @@ struct TaskState {
   int accLabel;             // totals of new runs, credited to accLabel when
   long long accSum;         // the label changes or the sweep ends
   unsigned long long accCnt;
-  unsigned long long evals, nIns, nDup;
-  unsigned long long insTicks;  // time spent in run-table inserts
+  unsigned long long evals, nIns, nDup, insTicks;
 };
-
-__device__ __forceinline__ void flag_point(const FillArgs& A, int gp, uint32_t f) {
-  if (__lane_id() == 0) atomicOr(&A.flags[gp], f);
-}
 
 __device__ __forceinline__ void flush_acc(TaskState& S) {
   if (S.accCnt && __lane_id() == 0) {
@@ -609,54 +696,35 @@ __device__ __forceinline__ void flush_acc(TaskState& S) {
   S.accSum = 0;
 }
 
-__device__ __forceinline__ void credit(TaskState& S, int label, unsigned long long cnt,
-                                       long long sum) {
-  if (label != S.accLabel) {
-    flush_acc(S);
-    S.accLabel = label;
-  }
-  S.accCnt += cnt;
-  S.accSum += sum;
-}
-
-__device__ __forceinline__ Task make_task(int b, long long row, long long xa, long long xb,
-                                          int label, int kind, int dir, long long f1,
-                                          long long f2) {
+__device__ __forceinline__ Task make_task(int b, int row, int xa, int xb, int label, int kind,
+                                          int dir, int f1, int f2) {
   Task f;
-  f.b = b; f.row = (int)row; f.xa = (int)xa; f.xb = (int)xb; f.label = label;
+  f.b = b; f.row = row; f.xa = xa; f.xb = xb; f.label = label;
   f.kd = kind | (dir < 0 ? 4 : 0);
-  f.f1 = (int)f1; f.f2 = (int)f2;
+  f.f1 = f1; f.f2 = f2;
   return f;
 }
 
-// Queues a task.  The pending count is raised, and that increment
-// completed, before the task becomes visible to other waves.
-__device__ __forceinline__ void push_task(TaskState& S, const Task& f) {
-  const FillArgs& A = *S.A;
-  if (__lane_id() == 0) {
-    afaddl(A.pending, 1);
-    mem_order();
-  }
-  if (!q_push(A, f)) {
-    flag_point(A, S.gp, kFlagQueueFull);
-    if (__lane_id() == 0) afaddl(A.pending, -1);
-  }
+// Follow-ups of one run, at most three, kept in named slots and dispatched
+// from one place: the first one that may stay with the wave becomes its next
+// step, the others are queued.
+struct Followups {
+  Task f0, f1, f2;
+  int n;
+};
+constexpr int kQueueOnly = 8;  // kd flag: never kept by the wave
+
+__device__ __forceinline__ void fu_add(Followups& F, const Task& f) {
+  if (F.n == 0) F.f0 = f;
+  else if (F.n == 1) F.f1 = f;
+  else F.f2 = f;
+  F.n++;
 }
 
-// Keeps a follow-up as this wave's next step, or queues it.
-__device__ __forceinline__ void add_followup(TaskState& S, const Task& f) {
-  if (!S.haveCont) {
-    S.cont = f;
-    S.haveCont = true;
-    return;
-  }
-  push_task(S, f);
-}
-
-// A maximal hit run [xl, xr] of the task's row was found: record it, credit
-// its pixels to its label and schedule the neighbouring rows.
-__device__ __forceinline__ void handle_run(TaskState& S, long long xl, long long xr, long long sum,
-                           unsigned long long cnt) {
+// A maximal hit run [xl, xr] of the row being scanned was found: record it,
+// credit its pixels to its label and schedule the neighbouring rows.
+__device__ __forceinline__ void handle_run(TaskState& S, int xl, int xr, long long sum,
+                                           unsigned long long cnt) {
   const FillArgs& A = *S.A;
   const Task& t = S.t;
   int label = t.label;
@@ -664,7 +732,7 @@ __device__ __forceinline__ void handle_run(TaskState& S, long long xl, long long
     // a scout run starts a class of its own
     int nl = 0;
     if (__lane_id() == 0) nl = atomicAdd(A.labelNext, 1);
-    nl = bcast0(nl);
+    nl = uni(nl);
     if (nl >= A.labelCap) {
       flag_point(A, S.gp, kFlagLabelFull);
       return;
@@ -672,125 +740,105 @@ __device__ __forceinline__ void handle_run(TaskState& S, long long xl, long long
     if (__lane_id() == 0) A.lpoint[nl] = t.b;
     label = nl;
   }
-  int dupLabel = -1;
   const unsigned long long ti0 = __builtin_amdgcn_s_memrealtime();
-  const int ins = insert_run(A, t.b, t.row, xl, xr, label, &dupLabel);
+  const int ins = insert_run(A, t.b, t.row, xl, xr, label);
   S.insTicks += __builtin_amdgcn_s_memrealtime() - ti0;
   if (ins == kInsFull) {
     flag_point(A, S.gp, kFlagHashFull);
     return;
   }
-  if (ins == kInsDup) {
+  if ((ins & 0xFF) == kInsDup) {
     S.nDup++;
-    if (S.kind != 2) unite(A, label, dupLabel);
+    if (S.kind != 2) unite(A, label, ins >> 8);
     return;
   }
   S.nIns++;
   if (xl <= 1 || xr >= S.N - 2) flag_point(A, S.gp, kFlagEdge);
-  credit(S, label, cnt, sum);
-  const long long row = t.row, N = S.N, center = (xl + xr) / 2;
-  if (S.kind == 1 || S.kind == 2) {
-    for (int d = 1; d >= -1; d -= 2)
-      if (row + d >= 0 && row + d < N)
-        add_followup(S, make_task(t.b, row + d, xl, xr, label, 0, d, center, 0));
-    return;
+  if (label != S.accLabel) {
+    flush_acc(S);
+    S.accLabel = label;
   }
+  S.accCnt += cnt;
+  S.accSum += sum;
+  const int row = t.row, N = S.N, center = (int)(((long long)xl + xr) / 2);
+  Followups F;
+  F.n = 0;
   const int d = S.dir;  // direction of travel
-  if (S.kind == 3) {
+  if (S.kind == 1 || S.kind == 2) {
+    if (row + 1 < N) fu_add(F, make_task(t.b, row + 1, xl, xr, label, 0, 1, center, 0));
+    if (row - 1 >= 0) fu_add(F, make_task(t.b, row - 1, xl, xr, label, 0, -1, center, 0));
+  } else if (S.kind == 3) {
     // a run found looking back: sweep on away from the run it was reached
     // from, and look back towards that run's row where this one reaches past it
     if (row + d >= 0 && row + d < N)
-      add_followup(S, make_task(t.b, row + d, xl, xr, label, 0, d, center, 0));
-    if (row - d >= 0 && row - d < N && (xl < S.xa - 1 || xr > S.xb + 1))
-      push_task(S, make_task(t.b, row - d, xl, xr, label, 3, -d, S.xa - 1, S.xb + 1));
-    return;
-  }
-  // sweep step: 1. the next row in the direction of travel
-  if (row + d >= 0 && row + d < N) {
-    // f2 >= 0: rows since the sweep's anchor, no scout sent yet;
-    // f2 < 0: -(rows since the last scout) - 1
-    long long a1 = t.f1, a2 = t.f2 >= 0 ? t.f2 + 1 : t.f2 - 1;
-    const long long rows = a2 >= 0 ? a2 : -a2 - 1;
-    const bool due = A.scoutRows > 0 && ((a2 >= 0 && rows >= A.scoutRows) ||
-                                         (a2 < 0 && rows >= A.scoutJump));
-    if (due) {
-      // send a scout ahead along the run's drift, then re-anchor
-      const long long rt = row + (long long)d * A.scoutJump;
-      if (rt >= 0 && rt < N) {
-        const long long drift = rows > 0 ? (center - a1) / rows : 0;
-        const long long pc = center + drift * A.scoutJump;
-        const long long adr = drift < 0 ? -drift : drift;
-        const long long hw = (xr - xl) / 2 + adr * A.scoutJump / 4 + 32;
-        long long sa = pc - hw, sb = pc + hw;
-        if (sa < 0) sa = 0;
-        if (sb > N - 1) sb = N - 1;
-        if (sa <= sb) push_task(S, make_task(t.b, rt, sa, sb, -1, 2, 0, 0, 0));
+      fu_add(F, make_task(t.b, row + d, xl, xr, label, 0, d, center, 0));
+    if (row - d >= 0 && row - d < N && (xl < S.xa - 1 || xr > S.xb + 1)) {
+      Task f = make_task(t.b, row - d, xl, xr, label, 3, -d, S.xa - 1, S.xb + 1);
+      f.kd |= kQueueOnly;
+      fu_add(F, f);
+    }
+  } else {
+    // sweep step: 1. the next row in the direction of travel
+    if (row + d >= 0 && row + d < N) {
+      // f2 >= 0: rows since the sweep's anchor, no scout sent yet;
+      // f2 < 0: -(rows since the last scout) - 1
+      int a1 = t.f1, a2 = t.f2 >= 0 ? t.f2 + 1 : t.f2 - 1;
+      const int rows = a2 >= 0 ? a2 : -a2 - 1;
+      const bool due = A.scoutRows > 0 && ((a2 >= 0 && rows >= A.scoutRows) ||
+                                           (a2 < 0 && rows >= A.scoutJump));
+      if (due) {
+        // send a scout ahead along the run's drift, then re-anchor
+        const long long rt = (long long)row + (long long)d * A.scoutJump;
+        if (rt >= 0 && rt < N) {
+          const long long drift = rows > 0 ? ((long long)center - a1) / rows : 0;
+          const long long pc = center + drift * A.scoutJump;
+          const long long adr = drift < 0 ? -drift : drift;
+          const long long hw = ((long long)xr - xl) / 2 + adr * A.scoutJump / 4 + 32;
+          long long sa = pc - hw, sb = pc + hw;
+          if (sa < 0) sa = 0;
+          if (sb > N - 1) sb = N - 1;
+          if (sa <= sb) {
+            Task f = make_task(t.b, (int)rt, (int)sa, (int)sb, -1, 2, 0, 0, 0);
+            f.kd |= kQueueOnly;
+            fu_add(F, f);
+          }
+        }
+        a1 = center;
+        a2 = -1;
       }
-      a1 = center;
-      a2 = -1;
+      fu_add(F, make_task(t.b, row + d, xl, xr, label, 0, d, a1, a2));
     }
-    add_followup(S, make_task(t.b, row + d, xl, xr, label, 0, d, a1, a2));
-  }
-  // 2. the row the sweep came from, where the new run reaches past the parent
-  //    run [xa, xb] and its two bounding misses: looked at after this scan
-  if (row - d >= 0 && row - d < N && (xl < S.xa - 1 || xr > S.xb + 1)) {
-    if (S.nBack < kMaxLookBacks) {
+    // 2. the row the sweep came from, where the new run reaches past the
+    //    parent run [xa, xb] and its two bounding misses: looked at after
+    //    this scan (or queued when two look-backs are already waiting)
+    if (row - d >= 0 && row - d < N && (xl < S.xa - 1 || xr > S.xb + 1)) {
       LookBack lb;
-      lb.row = (int)(row - d); lb.xa = (int)xl; lb.xb = (int)xr;
-      lb.k1 = (int)(S.xa - 1); lb.k2 = (int)(S.xb + 1); lb.dir = -d; lb.label = label;
-      if (S.nBack == 0) S.back0 = lb;
-      else S.back1 = lb;
-      S.nBack++;
-    } else {
-      push_task(S, make_task(t.b, row - d, xl, xr, label, 3, -d, S.xa - 1, S.xb + 1));
+      lb.row = row - d; lb.xa = xl; lb.xb = xr;
+      lb.k1 = S.xa - 1; lb.k2 = S.xb + 1; lb.dir = -d; lb.label = label;
+      if (S.nBack == 0) {
+        S.back0 = lb;
+        S.nBack = 1;
+      } else if (S.nBack == 1) {
+        S.back1 = lb;
+        S.nBack = 2;
+      } else {
+        Task f = make_task(t.b, lb.row, xl, xr, label, 3, -d, lb.k1, lb.k2);
+        f.kd |= kQueueOnly;
+        fu_add(F, f);
+      }
     }
   }
-}
-
-// Extends a run leftwards from column `from`, adding the pixels it takes in
-// to acc/cnt.  Returns the run's left end.
-__device__ __forceinline__ long long extend_left(const RowCtx& R, long long from, long long& acc,
-                                 unsigned long long& cnt, unsigned long long& evals) {
-  const int ln = __lane_id();
-  long long xcur = from;
-  while (true) {
-    if (xcur < 0) return 0;
-    const long long x0 = xcur - 63;
-    const long long x = x0 + ln;
-    const bool valid = x >= 0;
-    long long val = 0;
-    const bool hit = valid && eval_px(R, x, val);
-    evals += (unsigned long long)(xcur - (x0 < 0 ? 0 : x0) + 1);
-    const unsigned long long m = __ballot(hit);
-    const unsigned long long inv = ~m;
-    const int z = inv == 0 ? 64 : __builtin_clzll(inv);  // hits from bit 63 down
-    if (ln >= 64 - z) acc += val;
-    cnt += (unsigned long long)z;
-    if (z < 64) return xcur - z + 1;
-    xcur -= 64;
-  }
-}
-
-// Extends a run rightwards from column `from`.  Returns the run's right end.
-__device__ __forceinline__ long long extend_right(const RowCtx& R, long long from, long long& acc,
-                                  unsigned long long& cnt, unsigned long long& evals) {
-  const int ln = __lane_id();
-  long long xcur = from;
-  while (true) {
-    if (xcur > R.N - 1) return R.N - 1;
-    const long long x = xcur + ln;
-    const bool valid = x <= R.N - 1;
-    long long val = 0;
-    const bool hit = valid && eval_px(R, x, val);
-    const long long hiEnd = xcur + 63 < R.N - 1 ? xcur + 63 : R.N - 1;
-    evals += (unsigned long long)(hiEnd - xcur + 1);
-    const unsigned long long m = __ballot(hit);
-    const unsigned long long inv = ~m;
-    const int z = inv == 0 ? 64 : __builtin_ctzll(inv);
-    if (ln < z) acc += val;
-    cnt += (unsigned long long)z;
-    if (z < 64) return xcur + z - 1;
-    xcur += 64;
+  for (int k = 0; k < 3; k++) {
+    if (k >= F.n) break;
+    Task f = k == 0 ? F.f0 : (k == 1 ? F.f1 : F.f2);
+    const bool queueOnly = (f.kd & kQueueOnly) != 0;
+    f.kd &= ~kQueueOnly;
+    if (!queueOnly && !S.haveCont) {
+      S.cont = f;
+      S.haveCont = true;
+    } else {
+      push_task(A, f, S.gp);
+    }
   }
 }
 
@@ -799,128 +847,171 @@ __device__ __forceinline__ long long extend_right(const RowCtx& R, long long fro
 // may continue (extL, extR); elsewhere the pixel next to [s, e] is a known
 // miss.  Runs inside the padding that do not reach [s, e] are ignored; runs
 // that reach the window's edge are followed past it.  Each run is handed to
-// handle_run once both ends are known.
-__device__ __forceinline__ void scan_window(const RowCtx& R, long long s, long long e, bool extL, bool extR,
-                            TaskState& S) {
+// handle_run once both its ends are known.
+__device__ __forceinline__ void scan_window(const RowCtx& R, int s, int e, bool extL, bool extR,
+                                            TaskState& S) {
   const int ln = __lane_id();
-  const long long len = e - s + 1;
-  const long long extra = ((len + 63) / 64) * 64 - len;
-  long long padL = 0, padR = 0;
-  if (extL && extR) {
-    padL = extra / 2;
-    padR = extra - padL;
-  } else if (extL) {
-    padL = extra;
-  } else if (extR) {
-    padR = extra;
-  }
-  long long ws = s - padL, we = e + padR;
-  if (ws < 0) ws = 0;
-  if (we > R.N - 1) we = R.N - 1;
+  const int len = e - s + 1;
+  const int extra = ((len + 63) / 64) * 64 - len;
+  const int padL = extL ? (extR ? extra / 2 : extra) : 0;
+  const int padR = extR ? extra - padL : 0;
+  const int ws = s - padL < 0 ? 0 : s - padL;
+  const int we = e + padR > R.N - 1 ? R.N - 1 : e + padR;
   bool open = false;
-  long long runL = 0, acc = 0;
-  unsigned long long cnt = 0;
-  for (long long x0 = ws; x0 <= we; x0 += 64) {
-    const long long x = x0 + ln;
-    const bool valid = x <= we;
-    long long val = 0;
-    const bool hit = valid && eval_px(R, x, val);
-    const unsigned long long m = __ballot(hit);
-    const int nb = (int)(we - x0 + 1 < 64 ? we - x0 + 1 : 64);
-    S.evals += (unsigned long long)nb;
-    int pos = 0;
-    while (pos < nb) {
-      if (open) {
-        const unsigned long long rest = ~(m >> pos);
-        int z = rest == 0 ? 64 - pos : __builtin_ctzll(rest);
-        if (z > nb - pos) z = nb - pos;
-        if (ln >= pos && ln < pos + z) acc += val;
-        cnt += (unsigned long long)z;
-        pos += z;
-        if (pos < nb) {
-          const long long runR = x0 + pos - 1;
-          if (runR >= s && runL <= e) {
-            if (runL == ws && extL && ws > 0) runL = extend_left(R, ws - 1, acc, cnt, S.evals);
-            handle_run(S, runL, runR, wave_sum(acc), cnt);
-          }
-          acc = 0;
-          cnt = 0;
-          open = false;
-        }
+  int runL = 0, cb = ws, nb = 0, pos = 0;
+  unsigned long long m = 0, cnt = 0;
+  long long acc = 0, val = 0;
+  bool hit = false;
+  while (true) {
+    bool emit = false, last = false;
+    int eR = 0;
+    if (pos >= nb) {
+      // next 64 pixels
+      if (nb > 0) cb += 64;
+      if (cb > we) {
+        last = true;
+        emit = open && runL <= e;
+        eR = we;
       } else {
-        const unsigned long long rest = m >> pos;
-        if (rest == 0) break;
-        const int tz = __builtin_ctzll(rest);
-        if (pos + tz >= nb) break;
-        runL = x0 + pos + tz;
+        const int x = cb + ln;
+        val = 0;
+        hit = x <= we && eval_px(R, x, val);
+        m = __ballot(hit);
+        nb = we - cb + 1 < 64 ? we - cb + 1 : 64;
+        S.evals += (unsigned long long)nb;
+        pos = 0;
+        continue;
+      }
+    } else if (open) {
+      const unsigned long long rest = ~(m >> pos);
+      int z = rest == 0 ? 64 - pos : __builtin_ctzll(rest);
+      if (z > nb - pos) z = nb - pos;
+      if (ln >= pos && ln < pos + z) acc += val;
+      cnt += (unsigned long long)z;
+      pos += z;
+      if (pos < nb) {
+        eR = cb + pos - 1;
+        open = false;
+        emit = eR >= s && runL <= e;
+        if (!emit) { acc = 0; cnt = 0; }
+      }
+    } else {
+      const unsigned long long rest = m >> pos;
+      const int tz = rest == 0 ? 64 : __builtin_ctzll(rest);
+      if (pos + tz >= nb) {
+        pos = nb;
+      } else {
+        runL = cb + pos + tz;
         open = true;
         pos += tz;
       }
     }
-  }
-  if (open && we >= s && runL <= e) {
-    long long runR = we;
-    if (runL == ws && extL && ws > 0) runL = extend_left(R, ws - 1, acc, cnt, S.evals);
-    if (extR && we < R.N - 1) runR = extend_right(R, we + 1, acc, cnt, S.evals);
-    handle_run(S, runL, runR, wave_sum(acc), cnt);
+    if (emit) {
+      for (int side = 0; side < 2; side++) {
+        const bool need = side == 0 ? (runL == ws && extL && ws > 0)
+                                    : (last && extR && we < R.N - 1);
+        if (!need) continue;
+        const Ext x = extend_run(R, side == 0 ? ws - 1 : we + 1, side == 0 ? -1 : 1);
+        if (side == 0) runL = x.end;
+        else eR = x.end;
+        acc += x.sum;
+        cnt += (unsigned long long)x.cnt;
+        S.evals += x.evals;
+      }
+      handle_run(S, runL, eR, wave_sum(acc), cnt);
+      acc = 0;
+      cnt = 0;
+      open = false;
+    }
+    if (last) break;
   }
 }
 
 // ---------------------------------------------------------------------------
 // Fill
 // ---------------------------------------------------------------------------
-// Scans [lo, hi] of the current row without the known pixels [k1, k2].
-__device__ __forceinline__ void scan_excluding(const RowCtx& R, long long lo, long long hi, long long k1,
-                               long long k2, TaskState& S) {
-  if (lo < k1) scan_window(R, lo, hi < k1 - 1 ? hi : k1 - 1, true, hi < k1 - 1, S);
-  if (hi > k2) scan_window(R, lo > k2 + 1 ? lo : k2 + 1, hi, lo > k2 + 1, true, S);
-}
-
-// Processes one task and the look-backs it records; the follow-up kept for
-// this wave (if any) is left in S.cont / S.haveCont.
+// Processes one task and the look-backs it records, as up to six scans of one
+// row each: the task's row (one interval, or two around a known run), then
+// each look-back (the same).  The follow-up kept for this wave (if any) is
+// left in S.cont / S.haveCont.
 __device__ __forceinline__ void process_task(TaskState& S) {
   const FillArgs& A = *S.A;
   S.haveCont = false;
   S.nBack = 0;
+  S.t.b = uni(S.t.b);
+  S.t.row = uni(S.t.row);
+  S.t.xa = uni(S.t.xa);
+  S.t.xb = uni(S.t.xb);
+  S.t.label = uni(S.t.label);
+  S.t.kd = uni(S.t.kd);
+  S.t.f1 = uni(S.t.f1);
+  S.t.f2 = uni(S.t.f2);
   S.kind = S.t.kd & 3;
   S.dir = (S.t.kd & 4) ? -1 : 1;
-  S.gp = A.batchPts[S.t.b];
-  const CurveDev& C = A.curves[A.ptCurve[S.gp]];
-  S.N = C.N;
-  if (S.t.row < 0 || (long long)S.t.row >= S.N) return;
-  S.xa = S.t.xa;
-  S.xb = S.t.xb;
-  if (S.kind == 1 && (S.xa < 0 || S.xa > S.N - 1)) {  // seed column outside the grid
+  if (S.t.b != S.curB) {
+    // constants of this source position, kept while the wave stays on it
+    S.curB = S.t.b;
+    S.gp = uni(A.batchPts[S.t.b]);
+    const CurveDev& C = A.curves[uni(A.ptCurve[S.gp])];
+    S.N = uni((int)C.N);
+    S.R.L.m1 = unid(C.L.m1); S.R.L.m2 = unid(C.L.m2); S.R.L.m3 = unid(C.L.m3);
+    S.R.L.z1 = mk(unid(C.L.z1.re), unid(C.L.z1.im));
+    S.R.L.z2 = mk(unid(C.L.z2.re), unid(C.L.z2.im));
+    S.R.L.z3 = mk(unid(C.L.z3.re), unid(C.L.z3.im));
+    S.R.blx = unid(C.blx);
+    S.R.bly = unid(C.bly);
+    S.R.step = unid(C.step);
+    S.R.rho2 = unid(C.rho2);
+    S.R.invRho2 = 1.0 / S.R.rho2;
+    S.R.ldA = unid(C.ldA);
+    S.R.ldB = unid(C.ldB);
+    S.R.N = S.N;
+    const double2 pos = A.ptPos[S.gp];
+    S.R.px = unid(pos.x);
+    S.R.py = unid(pos.y);
+  }
+  if (S.t.row < 0 || S.t.row >= S.N) return;
+  if (S.kind == 1 && (S.t.xa < 0 || S.t.xa > S.N - 1)) {  // seed column outside the grid
     flag_point(A, S.gp, kFlagEdge);
     return;
   }
-  const long long lo = S.xa < 0 ? 0 : S.xa, hi = S.xb > S.N - 1 ? S.N - 1 : S.xb;
-  if (lo > hi) return;
-
-  RowCtx R;
-  R.C = &C;
-  R.N = S.N;
-  R.y = ny_to_y(C, S.t.row);
-  const double2 pos = A.ptPos[S.gp];
-  R.px = pos.x;
-  R.py = pos.y;
-  if (S.kind == 3) scan_excluding(R, lo, hi, S.t.f1, S.t.f2, S);
-  else scan_window(R, lo, hi, true, true, S);
-
-  // look-backs recorded during the scan; runs they find queue their own
-  // follow-ups, so no further look-backs are recorded here
-  for (int k = 0; k < kMaxLookBacks; k++) {
-    if (k >= S.nBack) break;
-    const LookBack lb = k == 0 ? S.back0 : S.back1;
-    S.t.row = lb.row;
-    S.t.label = lb.label;
-    S.kind = 3;
-    S.dir = lb.dir;
-    S.xa = lb.xa;
-    S.xb = lb.xb;
-    R.y = ny_to_y(C, lb.row);
-    const long long l2 = S.xa < 0 ? 0 : S.xa, h2 = S.xb > S.N - 1 ? S.N - 1 : S.xb;
-    if (l2 <= h2) scan_excluding(R, l2, h2, lb.k1, lb.k2, S);
+  int k1 = S.kind == 3 ? S.t.f1 : 0, k2 = S.kind == 3 ? S.t.f2 : -1;
+  S.xa = S.t.xa;
+  S.xb = S.t.xb;
+  for (int job = 0; job < 6; job++) {
+    if (job >= 2) {
+      const int li = (job - 2) >> 1;
+      if (li >= S.nBack) break;
+      if ((job & 1) == 0) {
+        const LookBack lb = li == 0 ? S.back0 : S.back1;
+        S.t.row = lb.row;
+        S.t.label = lb.label;
+        S.kind = 3;
+        S.dir = lb.dir;
+        S.xa = lb.xa;
+        S.xb = lb.xb;
+        k1 = lb.k1;
+        k2 = lb.k2;
+      }
+    } else if (job == 1 && k1 > k2) {
+      continue;  // a task without a known run inside its interval scans once
+    }
+    if ((job & 1) == 0) S.R.y = S.R.bly + S.R.step * (double)S.t.row;
+    const int lo = S.xa < 0 ? 0 : S.xa, hi = S.xb > S.N - 1 ? S.N - 1 : S.xb;
+    int s = lo, e = hi;
+    bool extL = true, extR = true;
+    if (k1 <= k2) {
+      // two sides of the known pixels [k1, k2]
+      if ((job & 1) == 0) {
+        e = hi < k1 - 1 ? hi : k1 - 1;
+        extR = hi < k1 - 1;
+      } else {
+        s = lo > k2 + 1 ? lo : k2 + 1;
+        extL = lo > k2 + 1;
+      }
+    }
+    if (s > e) continue;
+    scan_window(S.R, s, e, extL, extR, S);
   }
   S.nBack = 0;
 }
@@ -930,13 +1021,14 @@ __global__ void __launch_bounds__(256) fill_kernel(FillArgs A) {
   const long long t0 = (long long)__builtin_amdgcn_s_memrealtime();
   TaskState S;
   S.A = &A;
+  S.curB = -1;
   S.evals = 0;
   S.nIns = 0;
   S.nDup = 0;
+  S.insTicks = 0;
   S.accLabel = -1;
   S.accSum = 0;
   S.accCnt = 0;
-  S.insTicks = 0;
   unsigned long long nTasks = 0, busyTicks = 0, idlePolls = 0;
   bool have = false;
   int backoff = 0;
@@ -953,7 +1045,7 @@ __global__ void __launch_bounds__(256) fill_kernel(FillArgs A) {
         const CurveDev& C = A.curves[A.ptCurve[gp]];
         // the reference skips seeds outside rows 1..N-1 (lcirs.cc:386-389)
         const bool rowOk = px.y > 0 && (long long)px.y < C.N;
-        S.t = make_task(sk.x, rowOk ? px.y : -1, px.x, px.x, sk.x, 1, 1, 0, 0);
+        S.t = make_task(sk.x, rowOk ? px.y : -1, px.x, px.x, sk.x, 1, 1, 0, -1);
         have = true;
       } else if (q_pop(A, S.t)) {
         have = true;
