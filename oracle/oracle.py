@@ -71,6 +71,8 @@ class Oracle:
         lib.tlo_libm_cdiv.argtypes = [_D, _D, _D, _D, _P]
         lib.tlo_seeds_margin.argtypes = [_P, _D, _D, _D, _D, ctypes.c_int, _P, _P, _P, _P]
         lib.tlo_fill_stats.argtypes = [_P]
+        lib.tlo_hit_mask.argtypes = [_P, ctypes.c_long, ctypes.c_long, ctypes.c_long,
+                                     ctypes.c_long, _D, _D, _P]
         self.lib = lib
 
     # -- light curve object -------------------------------------------------
@@ -203,6 +205,13 @@ class LC:
 
     def irs(self, x, y, sx, sy):
         return self._o.lib.tlo_irs(self.h, x, y, sx, sy)
+
+    def hit_mask(self, x0, x1, y0, y1, sx, sy):
+        """Boolean hit mask of pixels [y0..y1] x [x0..x1] for source (sx, sy)."""
+        out = np.zeros((y1 - y0 + 1, x1 - x0 + 1), np.uint8)
+        if self._o.lib.tlo_hit_mask(self.h, x0, x1, y0, y1, sx, sy, _ptr(out)):
+            raise ValueError("bad hit-mask box")
+        return out.astype(bool)
 
 
 class Reference:

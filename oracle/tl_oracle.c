@@ -534,6 +534,12 @@ static void img_classify(imgpoint_t* ip, cx zeta) {
     double m = r > 0.0 ? fabs(log10(r / 1.0e-6)) : 1e300;
     if (m < g_margin) g_margin = m;
   }
+  /* Two roots converged onto one image: the solver lost a root, and which
+   * one it lost follows the iteration path, i.e. the last bits of the
+   * coefficients.  Reported as margin 0. */
+  for (int i = 0; i < ip->nImg; i++)
+    for (int j = i + 1; j < ip->nImg; j++)
+      if (cabs_(csub(ip->imgs[i], ip->imgs[j])) < 1.0e-8) g_margin = 0.0;
 }
 
 /* ImgPoint::getImages(pos) for a triple lens (imgpoint.cc:208-274,281-286) */
@@ -541,6 +547,8 @@ static int img_get(imgpoint_t* ip, cx zeta) {
   img_roots(ip, zeta, 0);
   img_classify(ip, zeta);
   if (ip->nImg % 2 != 0 || ip->nImg < 4) {
+    /* the chained solve gave an invalid image count: also a lost root */
+    g_margin = 0.0;
     img_roots(ip, zeta, 1);
     img_classify(ip, zeta);
   }
@@ -802,6 +810,18 @@ static inline double irs_ray(const lens_t* L, double x, double y, double sx, dou
 
 double tlo_irs(const tlo_lc* lc, double x, double y, double sx, double sy) {
   return irs_ray(&lc->ip.L, x, y, sx, sy);
+}
+
+int tlo_hit_mask(const tlo_lc* lc, long x0, long x1, long y0, long y1, double sx, double sy,
+                 uint8_t* out) {
+  if (!lc || !out || x1 < x0 || y1 < y0) return -1;
+  const long w = x1 - x0 + 1;
+  for (long j = y0; j <= y1; j++) {
+    const double y = ny_to_y(lc, j);
+    for (long i = x0; i <= x1; i++)
+      out[(j - y0) * w + (i - x0)] = irs_ray(&lc->ip.L, nx_to_x(lc, i), y, sx, sy) < lc->rho2;
+  }
+  return 0;
 }
 
 /* ---- amoeba (amoeba.cc) ------------------------------------------------- */

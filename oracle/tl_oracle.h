@@ -89,6 +89,11 @@ int tlo_images(double a, double b, double th, double m2, double m3, int n,
  * irsSIMD (lcirs.cc:290-341). */
 double tlo_irs(const tlo_lc* lc, double x, double y, double sx, double sy);
 
+/* Hit mask of the pixels x0..x1, y0..y1 (row-major, 1 = rsq < rho^2) for
+ * source position (sx, sy), pixel centres as nxToX/nyToY (lcirs.cc:355-364). */
+int tlo_hit_mask(const tlo_lc* lc, long x0, long x1, long y0, long y1, double sx, double sy,
+                 uint8_t* out);
+
 /* Point-source magnification (LightCurveBase::getPointAmp, lcbase.cc:17-38)
  * along n positions with one root chain. */
 int tlo_point_amp(double a, double b, double th, double m2, double m3, int n,

@@ -1,0 +1,228 @@
+"""Parity of the HIP engine with the oracle and the reference's fixtures.
+
+Every test here runs the engine on the GPU through the C ABI (liblcirs.so):
+ray counts must be identical to the oracle's (and to the reference's golden
+vectors wherever the oracle matches them), magnifications within 1e-10
+relative (the engine sums brightness in fixed point, the oracle in double;
+the reference's own amplitudes are compared at 1e-6).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import curve_from_params, golden, pixel_seeds
+
+pytestmark = pytest.mark.gpu
+
+AMP_RTOL = 1e-10
+
+
+@pytest.fixture(scope="module")
+def E():
+    from triplelens_amd import engine
+    engine.load()
+    engine.set_option("time_limit_s", 60)
+    return engine
+
+
+def _new(E, c, lc_len=None, ppr=None):
+    lc = E.LightCurve(*c.lens(), c.rho, c.lc_len if lc_len is None else lc_len,
+                      c.ppr if ppr is None else ppr)
+    lc.set_limb_darkening("linear", c.u)
+    return lc
+
+
+def _assert_no_status(E, flags):
+    bad = int(np.bitwise_or.reduce(flags)) & ~E.FLAG_EDGE
+    assert bad == 0, f"engine status flags {bad:#x}"
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+
+
+# --------------------------------------------------------------------- C1
+def test_c1_matches_reference_golden(E):
+    """The 200-point workload through the reference-style class LC_irs."""
+    from triplelens_amd.ctypes_classes import LC_irs
+    g = golden("c1.npz")
+    c = curve_from_params(g["params"])
+    lc = LC_irs(*c.lens(), c.rho, c.lc_len, c.ppr)
+    lc.set_limb_darkening(b"linear", c.u)
+    lc.get_lc_irs(*c.traj())
+    out = np.zeros(c.lc_len)
+    lc.copy_lc(out)
+    rays = lc.ray_counts()
+    full = lc.lc_full()
+    assert np.array_equal(rays, g["rays"]), np.where(rays != g["rays"])[0][:10]
+    assert _rel(full, g["amp"]) < 1e-6
+    assert np.array_equal(out, full[:c.lc_len])
+
+
+def test_c1_matches_oracle(E, oracle):
+    from triplelens_amd import workloads as W
+    c = W.c1()
+    amp, rays, flags = _new(E, c).lc_irs(*c.traj())
+    amp_o, rays_o, edge_o = oracle.new(*c.lens(), c.rho, c.lc_len, c.ppr, c.u).lc_irs(*c.traj())
+    assert np.array_equal(rays, rays_o)
+    assert _rel(amp, amp_o) < AMP_RTOL
+    assert np.array_equal((flags & E.FLAG_EDGE) != 0, edge_o)
+    _assert_no_status(E, flags)
+
+
+def test_c1_seeds_match_golden(E):
+    """Seed pixels of every point: the reference's seeds mapped to pixels."""
+    g = golden("c1.npz")
+    c = curve_from_params(g["params"])
+    lc = _new(E, c)
+    lc.lc_irs(*c.traj())
+    ns, pix = lc.seeds()
+    off, xy = g["seed_off"], g["seed_xy"]
+    px, py = pixel_seeds(g["grid"], xy)
+    for i in range(c.n_points):
+        ref = set(zip(px[off[i]:off[i + 1]].tolist(), py[off[i]:off[i + 1]].tolist()))
+        got = set(map(tuple, pix[i, :ns[i]].tolist()))
+        assert ns[i] == off[i + 1] - off[i] and got == ref, i
+
+
+def test_c1_default_limb_darkening(E, oracle):
+    """Without set_limb_darkening the profile is A=0.6, B=0.4."""
+    from triplelens_amd import workloads as W
+    c = W.c1()
+    lc = E.LightCurve(*c.lens(), c.rho, c.lc_len, c.ppr)
+    amp, rays, _ = lc.lc_irs(*c.traj())
+    amp_o, rays_o, _ = oracle.new(*c.lens(), c.rho, c.lc_len, c.ppr).lc_irs(*c.traj())
+    assert np.array_equal(rays, rays_o)
+    assert _rel(amp, amp_o) < AMP_RTOL
+
+
+def test_c1_point_source_curve(E, oracle):
+    """get_lc: point-source magnification at t = i/(lcLength-1)."""
+    from triplelens_amd import workloads as W
+    c = W.c1()
+    got = _new(E, c).point_lc(*c.traj())
+    t = np.arange(c.lc_len) / (c.lc_len - 1)
+    pos = (c.sx + (c.ex - c.sx) * t) + 1j * (c.sy + (c.ey - c.sy) * t)
+    ref = oracle.point_amp(*c.lens(), pos)
+    assert _rel(got, ref) < 1e-12
+
+
+# --------------------------------------------------------------------- C2
+def test_c2_window_full_resolution_golden(E):
+    """65 points around the magnification peak at pixel rho/200."""
+    g = golden("c2_window.npz")
+    c = curve_from_params(g["params"])
+    amp, rays, flags = _new(E, c).lc_irs(*c.traj())
+    assert np.array_equal(rays, g["rays"]), np.where(rays != g["rays"])[0][:10]
+    assert _rel(amp, g["amp"]) < 1e-6
+    _assert_no_status(E, flags)
+
+
+def test_c2_ppr20_matches_oracle_and_golden(E, oracle):
+    """All 4096 C2 points at pixel rho/20: identical to the oracle, and to the
+    reference wherever the oracle is."""
+    g = golden("c2_ppr20.npz")
+    c = curve_from_params(g["params"])
+    amp, rays, flags = _new(E, c).lc_irs(*c.traj())
+    amp_o, rays_o, _ = oracle.new(*c.lens(), c.rho, c.lc_len, c.ppr, c.u).lc_irs(*c.traj())
+    assert np.array_equal(rays, rays_o), np.where(rays != rays_o)[0][:10]
+    assert _rel(amp, amp_o) < AMP_RTOL
+    same_o = rays_o == g["rays"]
+    assert np.array_equal(rays[same_o], g["rays"][same_o])
+    _assert_no_status(E, flags)
+
+
+def test_c3_batch_matches_oracle(E, oracle):
+    """16 curves of C3 at pixel rho/20 in one batched engine run."""
+    g = golden("c3_ppr20.npz")
+    curves = [curve_from_params(p) for p in g["params"]]
+    lcs = [_new(E, c) for c in curves]
+    res = E.batch_irs(lcs, np.array([c.traj() for c in curves]))
+    off = g["off"]
+    for k, (c, (amp, rays, flags)) in enumerate(zip(curves, res)):
+        amp_o, rays_o, _ = oracle.new(*c.lens(), c.rho, c.lc_len, c.ppr, c.u).lc_irs(*c.traj())
+        assert np.array_equal(rays, rays_o), (k, np.where(rays != rays_o)[0][:10])
+        assert _rel(amp, amp_o) < AMP_RTOL, k
+        gr = g["rays"][off[k]:off[k + 1]]
+        same_o = rays_o == gr
+        assert np.array_equal(rays[same_o], gr[same_o]), k
+        _assert_no_status(E, flags)
+
+
+def test_full_c2_curve_properties(E, oracle):
+    """The full benchmark curve (4096 points, pixel rho/200): no status
+    flags, bitwise-identical repeat runs, every point's seed set equal to the
+    oracle's, and sampled points filled by the oracle from those seeds."""
+    from triplelens_amd import workloads as W
+    c = W.c2(seed=2, ca_fn=lambda *p: oracle.ccc(*p)[1])
+    lc = _new(E, c)
+    amp, rays, flags = lc.lc_irs(*c.traj())
+    _assert_no_status(E, flags)
+    amp2, rays2, _ = lc.lc_irs(*c.traj())
+    assert np.array_equal(rays, rays2) and np.array_equal(amp, amp2)
+    ns, pix = lc.seeds()
+    lo = oracle.new(*c.lens(), c.rho, c.lc_len, c.ppr, c.u)
+    xy, ns_o, _ = lo.seeds(*c.traj())
+    g = lo.grid()
+    px, py = pixel_seeds(g, xy)
+    for i in range(c.n_points):
+        ref = set(zip(px[i, :ns_o[i]].tolist(), py[i, :ns_o[i]].tolist()))
+        assert ns[i] == ns_o[i] and set(map(tuple, pix[i, :ns[i]].tolist())) == ref, i
+    picks = sorted(set(np.linspace(0, c.n_points - 1, 12).astype(int).tolist()
+                       + np.argsort(amp)[-4:].tolist()))
+    for i in picks:
+        t = i / c.lc_len
+        r, s, _ = lo.fill_point(c.sx + (c.ex - c.sx) * t, c.sy + (c.ey - c.sy) * t,
+                                xy[i, :ns_o[i]])
+        assert rays[i] == r, i
+        assert abs(amp[i] - s * g[7]) <= AMP_RTOL * abs(s * g[7]), i
+
+
+# ------------------------------------------------------------ seeded fills
+def test_fill_from_given_seeds(E, oracle):
+    """lcirs_fill_seeds on hand-made seed lists: none, duplicates, misses,
+    seeds of one image twice over, and the seed search's own seeds."""
+    from triplelens_amd import workloads as W
+    c = W.c1()
+    lc = _new(E, c)
+    lc.lc_irs(*c.traj())
+    ns, pix = lc.seeds()
+    lo = oracle.new(*c.lens(), c.rho, c.lc_len, c.ppr, c.u)
+    g = lo.grid()
+    N, L, blx, bly = int(g[0]), g[1], g[3], g[4]
+    rng = np.random.default_rng(7)
+    idx = [0, 25, 99, 150, 199]
+    pos, nsl, pixl = [], [], []
+    for i in idx:
+        t = i / c.lc_len
+        p = (c.sx + (c.ex - c.sx) * t, c.sy + (c.ey - c.sy) * t)
+        own = pix[i, :ns[i]]
+        cases = [own[:0], own, np.concatenate([own, own]), own[:1],
+                 rng.integers(0, N, size=(5, 2)), np.concatenate([own, rng.integers(0, N, (3, 2))])]
+        for s in cases:
+            s = np.asarray(s, np.int32)[:E.MAX_SEEDS]
+            buf = np.zeros((E.MAX_SEEDS, 2), np.int32)
+            buf[:len(s)] = s
+            pos.append(p)
+            nsl.append(len(s))
+            pixl.append(buf)
+    amp, rays, flags = lc.fill_seeds(np.array(pos), np.array(nsl), np.array(pixl))
+    _assert_no_status(E, flags)
+    for k in range(len(pos)):
+        s = pixl[k][:nsl[k]]
+        # pixel centres in image coordinates map back to the same pixels
+        sxy = np.stack([blx + s[:, 0] * (L / N), bly + s[:, 1] * (L / N)], axis=1)
+        px, py = pixel_seeds(g, sxy)
+        assert np.array_equal(px, s[:, 0]) and np.array_equal(py, s[:, 1])
+        r, b, _ = lo.fill_point(pos[k][0], pos[k][1], sxy)
+        assert rays[k] == r, k
+        assert abs(amp[k] - b * g[7]) <= AMP_RTOL * max(abs(b * g[7]), 1e-300), k
+
+
+def test_fill_seeds_rejects_bad_arguments(E):
+    from triplelens_amd import workloads as W
+    c = W.c1()
+    lc = _new(E, c)
+    with pytest.raises(E.EngineError):
+        lc.fill_seeds(np.zeros((1, 2)), np.array([E.MAX_SEEDS + 1]), np.zeros((1, E.MAX_SEEDS, 2)))

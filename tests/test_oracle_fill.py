@@ -17,27 +17,26 @@ from triplelens_amd import workloads as W
 
 
 def _brute_count(lc, grid, px, py, seeds_pix, pad):
-    N, step, blx, bly = int(grid[0]), grid[2], grid[3], grid[4]
-    xs = [p[0] for p in seeds_pix]
-    ys = [p[1] for p in seeds_pix]
-    x0, x1 = max(1, min(xs) - pad), min(N - 1, max(xs) + pad)
-    y0, y1 = max(1, min(ys) - pad), min(N - 1, max(ys) + pad)
-    rho2 = grid[34]
-    hit = np.zeros((y1 - y0 + 1, x1 - x0 + 1), bool)
-    for j in range(y0, y1 + 1):
-        y = bly + step * j
-        for i in range(x0, x1 + 1):
-            hit[j - y0, i - x0] = lc.irs(blx + step * i, y, px, py) < rho2
-    lab, _ = ndimage.label(hit)  # 4-connectivity
-    keep = set()
+    """Hit pixels of the 4-connected hit components holding a hit seed, found
+    in a box of +-pad pixels around each seed (columns and rows 1..N-1)."""
+    N = int(grid[0])
+    kept = set()
     for sx, sy in seeds_pix:
-        if y0 <= sy <= y1 and x0 <= sx <= x1 and hit[sy - y0, sx - x0]:
-            keep.add(lab[sy - y0, sx - x0])
-    # every kept component must lie inside the box
-    for k in keep:
-        m = lab == k
-        assert not (m[0].any() or m[-1].any() or m[:, 0].any() or m[:, -1].any()), "box too small"
-    return int(sum((lab == k).sum() for k in keep))
+        x0, x1 = max(1, sx - pad), min(N - 1, sx + pad)
+        y0, y1 = max(1, sy - pad), min(N - 1, sy + pad)
+        if not (x0 <= sx <= x1 and y0 <= sy <= y1):
+            continue
+        hit = lc.hit_mask(x0, x1, y0, y1, px, py)
+        if not hit[sy - y0, sx - x0]:
+            continue
+        lab, _ = ndimage.label(hit)  # 4-connectivity
+        m = lab == lab[sy - y0, sx - x0]
+        # the component must lie inside the box (or end at the grid's edge)
+        assert not ((m[0].any() and y0 > 1) or (m[-1].any() and y1 < N - 1) or
+                    (m[:, 0].any() and x0 > 1) or (m[:, -1].any() and x1 < N - 1)), "box too small"
+        ys, xs = np.nonzero(m)
+        kept.update(zip((ys + y0).tolist(), (xs + x0).tolist()))
+    return len(kept)
 
 
 @pytest.mark.parametrize("idx", [0, 26, 60, 120])
@@ -52,7 +51,7 @@ def test_fill_equals_components_holding_seeds(oracle, idx):
     sxy = xy[idx, :ns[idx]]
     sx, sy = pixel_seeds(g, sxy)
     rays, bsum, edge = lc.fill_point(px, py, sxy)
-    want = _brute_count(lc, g, px, py, list(zip(sx, sy)), pad=60)
+    want = _brute_count(lc, g, px, py, list(zip(sx.tolist(), sy.tolist())), pad=400)
     assert rays == want
     assert not edge
     # uniform disk: brightness 1 per ray

@@ -2,14 +2,20 @@
 tests/golden/make_golden.py from the reference compiled from its sources).
 
 C1 must match completely: grid, critical curves, seeds, every ray count and
-every magnification bit.  On the C2 and C3 geometries the seed decisions at
-a few points are ill-conditioned: a root's lens-equation residual sits
-within a factor of a few of the 1e-6 image threshold (imgpoint.h:107), so
-whether the reference accepts it depends on the last bits of its
-coefficients; the oracle's coefficients are the exact values rounded once
-and the reference's carry a few ulps of rounding.  Ray counts must match at
-every other point, and differences may occur only where the oracle reports
-such a root (seed margin below one decade)."""
+every magnification bit.  On the C2 and C3 geometries a few seed decisions
+are ill-conditioned, and the ray counts may differ only there.  The oracle's
+coefficients are the exact values rounded once; the reference's carry a few
+ulps of rounding.  Two kinds of decision then depend on those last bits:
+  * a root's lens-equation residual sits within a decade of the 1e-6 image
+    threshold (imgpoint.h:107).  This happens at every caustic trial point
+    (lcirs.cc:195-255), which lies on the caustic, where two images merge
+    into a double root that Laguerre only resolves to about sqrt(eps);
+  * the chained solve loses a root: two roots converge onto one image, or
+    the image count is invalid and the solve restarts (imgpoint.cc:262-270).
+The oracle reports both as a seed margin below one decade (0 for a lost
+root).  Every mismatch must sit at such a point, and mismatches must stay
+below 1 in 500 points (13 of 27735 C3 points and 3 of 4096 C2 points were
+observed)."""
 import numpy as np
 import pytest
 
@@ -66,7 +72,7 @@ def test_c2_curve_ppr20(oracle):
     g = golden("c2_ppr20.npz")
     c = curve_from_params(g["params"])
     nbad, n = _compare_curve(oracle, c, g["rays"], g["amp"])
-    assert nbad <= n // 200
+    assert nbad <= n // 500
 
 
 def test_c2_window_full_resolution(oracle):
@@ -85,7 +91,7 @@ def test_c3_curves_ppr20(oracle):
         nbad, n = _compare_curve(oracle, c, g["rays"][s:e], g["amp"][s:e])
         total_bad += nbad
         total += n
-    assert total_bad <= total // 200
+    assert total_bad <= total // 500
 
 
 def test_images_along_c1(oracle):

@@ -769,6 +769,9 @@ int lcirs_fill_seeds(void* lc, int n, const double* pos, const int* nSeeds, cons
                      uint64_t* rays, double* amp, uint32_t* flags) {
   return guarded([&] {
     if (!lc || n <= 0 || !pos || !nSeeds || !pix) throw std::invalid_argument("bad arguments");
+    for (int i = 0; i < n; i++)
+      if (nSeeds[i] < 0 || nSeeds[i] > kMaxSeeds)
+        throw std::invalid_argument("seed count out of range (0 .. 64 per position)");
     Curve* c = static_cast<Curve*>(lc);
     Engine& E = engine();
     std::lock_guard<std::mutex> lock(E.mu);
