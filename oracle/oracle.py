@@ -71,6 +71,8 @@ class Oracle:
         lib.tlo_libm_cdiv.argtypes = [_D, _D, _D, _D, _P]
         lib.tlo_seeds_margin.argtypes = [_P, _D, _D, _D, _D, ctypes.c_int, _P, _P, _P, _P]
         lib.tlo_fill_stats.argtypes = [_P]
+        lib.tlo_last_segments.argtypes = [_P, _P, ctypes.c_long]
+        lib.tlo_last_segments.restype = ctypes.c_long
         lib.tlo_hit_mask.argtypes = [_P, ctypes.c_long, ctypes.c_long, ctypes.c_long,
                                      ctypes.c_long, _D, _D, _P]
         self.lib = lib
@@ -203,6 +205,15 @@ class LC:
             raise RuntimeError(f"tlo_fill_point failed ({rc})")
         return int(rays[0]), float(ssum[0]), bool(edge[0])
 
+    def last_segments(self):
+        """The amoeba of the last fill_point: (n, 3) rows of ny, xleft, xright,
+        sorted by row and column."""
+        n = self._o.lib.tlo_last_segments(self.h, None, 0)
+        out = np.zeros((max(n, 0), 3), np.int64)
+        if n > 0:
+            self._o.lib.tlo_last_segments(self.h, _ptr(out), n)
+        return out[np.lexsort((out[:, 1], out[:, 0]))] if n > 0 else out
+
     def irs(self, x, y, sx, sy):
         return self._o.lib.tlo_irs(self.h, x, y, sx, sy)
 
@@ -232,6 +243,9 @@ class Reference:
         lib.tlref_point_amp.argtypes = [_D] * 5 + [ctypes.c_int, _P, _P]
         lib.tlref_laguerre_solve.argtypes = [ctypes.c_int, _P, _P, _P]
         lib.tlref_coeffs.argtypes = [_D] * 7 + [_P]
+        lib.tlref_lc_irs_print.argtypes = [_D] * 6 + [ctypes.c_uint, ctypes.c_long, ctypes.c_int,
+                                                      _D, _D, _D, _D, _D, ctypes.c_char_p,
+                                                      ctypes.c_char_p]
         self.lib = lib
 
     @staticmethod
@@ -243,6 +257,12 @@ class Reference:
         self.lib.tlref_grid(a, b, th, m2, m3, rho, lc_len, ppr,
                             0 if ld_v is None else 1, 0.0 if ld_v is None else ld_v, _ptr(g))
         return g
+
+    def lc_irs_print(self, a, b, th, m2, m3, rho, lc_len, ppr, ld_v, sx, sy, ex, ey,
+                     amoeba_prefix, par_file):
+        """getLCIRS with the reference's amoeba and parameter print-out on."""
+        self.lib.tlref_lc_irs_print(a, b, th, m2, m3, rho, lc_len, ppr, 1, ld_v, sx, sy, ex, ey,
+                                    amoeba_prefix.encode(), par_file.encode())
 
     def lc_irs(self, a, b, th, m2, m3, rho, lc_len, ppr, ld_v, sx, sy, ex, ey):
         n = lc_len + 1

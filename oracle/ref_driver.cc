@@ -140,6 +140,26 @@ int tlref_lc_irs(double a, double b, double th, double m2, double m3, double rho
   return parsed;
 }
 
+// getLCIRS with the amoeba and parameter print-out switched on
+// (setAmoebaPrintOut, lcirs.cc:469-477): the reference writes parFile and one
+// amoeba file per point, amoebaPrefix + i.
+int tlref_lc_irs_print(double a, double b, double th, double m2, double m3, double rho,
+                       unsigned lcLength, long ppr, int ldModel, double v,
+                       double sx, double sy, double ex, double ey,
+                       const char* amoebaPrefix, const char* parFile) {
+  run_big_stack([&] {
+    CountParser sink;
+    std::streambuf* old = std::cout.rdbuf(&sink);
+    Probe lc(a, b, th, m2, m3, rho, lcLength, ppr);
+    set_ld(lc, ldModel, v);
+    lc.lcVec.resize(lcLength + 1);
+    lc.setAmoebaPrintOut(true, amoebaPrefix, parFile);
+    lc.getLCIRS({sx, sy}, {ex, ey});
+    std::cout.rdbuf(old);
+  });
+  return 0;
+}
+
 // Seeds of every point, in the order getLCIRS visits them: the same calls on
 // the same ImgPoint object (so the root-reuse chain is the reference's own),
 // following the loop of lcirs.cc:116-165.  For point i, seeds are written to

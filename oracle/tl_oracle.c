@@ -978,8 +978,7 @@ static int fill_seeds(tlo_lc* lc, double px, double py, int n, const cx* seeds,
   fillacc_t acc = {lc, px, py, 0.0, 0, 0};
   for (int k = 0; k < n; k++)
     if (flood(lc, &acc, x_to_nx(lc, seeds[k].re), y_to_ny(lc, seeds[k].im))) return -1;
-  amoeba_stats(lc);
-  amoeba_reset(lc);
+  amoeba_stats(lc);  /* the amoeba stays until the next fill (tlo_last_segments) */
   *rays = acc.count;
   *sum = acc.sum;
   if (edge) *edge = acc.edge;
@@ -1003,6 +1002,24 @@ static void amoeba_stats(const tlo_lc* lc) {
   g_stats[3] = mn; g_stats[4] = mxr;
 }
 int tlo_fill_stats(long* out) { for (int i = 0; i < 5; i++) out[i] = g_stats[i]; return 0; }
+
+long tlo_last_segments(const tlo_lc* lc, long* out, long cap) {
+  if (!lc) return -1;
+  long n = 0;
+  if (!lc->head) return 0;
+  for (long t = 0; t < lc->nTouched; t++) {
+    const long ny = lc->touched[t];
+    for (int32_t r = lc->head[ny]; r >= 0; r = lc->rnext[r]) {
+      if (out && n < cap) {
+        out[3 * n] = ny;
+        out[3 * n + 1] = lc->rxl[r];
+        out[3 * n + 2] = lc->rxr[r];
+      }
+      n++;
+    }
+  }
+  return n;
+}
 
 int tlo_fill_point(const tlo_lc* clc, double px, double py, int nSeeds,
                    const double* seedXY, uint64_t* rays, double* brightSum, int* edge) {

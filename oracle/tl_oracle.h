@@ -108,6 +108,11 @@ double tlo_libm_hypot(double x, double y);
 void tlo_libm_csqrt(double re, double im, double* out);
 void tlo_libm_cdiv(double a, double b, double c, double d, double* out);
 
+/* The amoeba of the last fill (Amoeba::amoebae after lineFloodFill,
+ * amoeba.cc:15-30): (ny, xleft, xright) triples into out (room for cap);
+ * returns their number. */
+long tlo_last_segments(const tlo_lc* lc, long* out, long cap);
+
 /* Amoeba statistics of the last fill (ranges, rows, max ranges per row, min
  * row, max row). */
 int tlo_fill_stats(long* out);

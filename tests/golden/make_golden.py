@@ -19,6 +19,11 @@ Fixtures:
   coeffs.npz      image-polynomial coefficients at random source positions.
   images.npz      the image chain (roots, accepted flags) along C1.
   point_amp.npz   point-source magnification (getPointAmp) along C1.
+  amoeba_c1.npz   the reference's amoeba print-out (setAmoebaPrintOut) for
+                  five C1 points (20..24 of 199): every (ny, xleft, xright)
+                  run per point, sorted, and the parameter file's text.
+
+Usage: make_golden.py [name ...] writes only the named fixtures.
 """
 from __future__ import annotations
 
@@ -51,12 +56,41 @@ def curve_fixture(R, c, ppr=None):
     return amp, rays, grid
 
 
-def main():
+def amoeba_fixture(R):
+    import tempfile
+    c = W.c1()
+    i0, n = 20, 4
+    t0, t1 = i0 / c.lc_len, (i0 + n) / c.lc_len
+    tr = (c.sx + (c.ex - c.sx) * t0, c.sy + (c.ey - c.sy) * t0,
+          c.sx + (c.ex - c.sx) * t1, c.sy + (c.ey - c.sy) * t1)
+    with tempfile.TemporaryDirectory() as d:
+        R.lc_irs_print(*c.lens(), c.rho, n, c.ppr, c.u, *tr, os.path.join(d, "Amoeba_"),
+                       os.path.join(d, "Par"))
+        segs = []
+        for i in range(n + 1):
+            a = np.loadtxt(os.path.join(d, f"Amoeba_{i}"), delimiter=",", dtype=np.int64)
+            a = a.reshape(-1, 3)
+            segs.append(a[np.lexsort((a[:, 1], a[:, 0]))])
+        par = open(os.path.join(d, "Par")).read()
+    off = np.zeros(n + 2, np.int64)
+    off[1:] = np.cumsum([len(a) for a in segs])
+    np.savez_compressed(os.path.join(HERE, "amoeba_c1.npz"), params=np.array(
+        [*c.lens(), c.rho, c.ppr, c.u, *tr, n + 1]), off=off, segs=np.concatenate(segs),
+        par=np.array(par))
+    return int(off[-1])
+
+
+def main(only=()):
     orc.build(ref=True)
     R = orc.Reference()
     O = orc.Oracle()
     rng = np.random.default_rng(12345)
     out = {}
+    if only:
+        if "amoeba_c1" in only:
+            out["amoeba_c1"] = amoeba_fixture(R)
+        print(out)
+        return
 
     # C1 ------------------------------------------------------------------
     c = W.c1()
@@ -142,8 +176,9 @@ def main():
                         n_img=n_img)
     pa = R.point_amp(*c.lens(), pos)
     np.savez_compressed(os.path.join(HERE, "point_amp.npz"), pos=pos, amp=pa)
+    out["amoeba_c1"] = amoeba_fixture(R)
     print(out)
 
 
 if __name__ == "__main__":
-    main()
+    main(tuple(sys.argv[1:]))

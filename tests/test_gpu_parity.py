@@ -226,3 +226,21 @@ def test_fill_seeds_rejects_bad_arguments(E):
     lc = _new(E, c)
     with pytest.raises(E.EngineError):
         lc.fill_seeds(np.zeros((1, 2)), np.array([E.MAX_SEEDS + 1]), np.zeros((1, E.MAX_SEEDS, 2)))
+
+
+def test_amoeba_print_out_matches_reference(E, tmp_path):
+    """set_amoeba_printout: one file per point with the runs of its fill, and
+    the parameter file, as the reference writes them (five C1 points)."""
+    from triplelens_amd.ctypes_classes import LC_irs
+    g = golden("amoeba_c1.npz")
+    c = curve_from_params(g["params"])
+    lc = LC_irs(*c.lens(), c.rho, c.lc_len, c.ppr)
+    lc.set_limb_darkening("linear", c.u)
+    lc.set_amoeba_printout(str(tmp_path / "Amoeba_"), str(tmp_path / "Par"))
+    lc.get_lc_irs(*c.traj())
+    off = g["off"]
+    for i in range(c.n_points):
+        a = np.loadtxt(tmp_path / f"Amoeba_{i}", delimiter=",", dtype=np.int64).reshape(-1, 3)
+        a = a[np.lexsort((a[:, 1], a[:, 0]))]
+        assert np.array_equal(a, g["segs"][off[i]:off[i + 1]]), i
+    assert (tmp_path / "Par").read_text() == str(g["par"])

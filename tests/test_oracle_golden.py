@@ -129,3 +129,17 @@ def test_against_live_reference_if_present(oracle, reference, name):
         assert np.array_equal(rays, g["rays"][s:e])
     else:
         assert np.array_equal(rays, g["rays"])
+
+
+def test_amoeba_print_out_matches_reference(oracle):
+    """The runs of every fill (Amoeba::amoebae, amoeba.cc:15-30) equal the
+    reference's amoeba print-out for five C1 points."""
+    g = golden("amoeba_c1.npz")
+    c = curve_from_params(g["params"])
+    lc = oracle.new(*c.lens(), c.rho, c.lc_len, c.ppr, c.u)
+    xy, ns, _ = lc.seeds(*c.traj())
+    off = g["off"]
+    for i in range(c.n_points):
+        t = i / c.lc_len
+        lc.fill_point(c.sx + (c.ex - c.sx) * t, c.sy + (c.ey - c.sy) * t, xy[i, :ns[i]])
+        assert np.array_equal(lc.last_segments(), g["segs"][off[i]:off[i + 1]]), i

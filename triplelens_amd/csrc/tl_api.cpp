@@ -227,6 +227,23 @@ struct Stats {
          totalMs = 0, busyMs = 0, insertMs = 0, idlePolls = 0, waves = 0;
 };
 
+// One recorded run of the amoeba print-out: row, leftmost and rightmost pixel.
+struct Segment {
+  long long ny, xl, xr;
+};
+
+// Amoeba::printOut (amoeba.cc:90-108) for point i of the last curve, into
+// amoebaPrefix + i as getLCIRS names it (lcirs.cc:183-186): one line
+// "ny,xleft,xright" per run.  The reference lists runs in hash-map order;
+// here they are sorted by row, then column.
+void write_amoeba_file(const Curve& c, int i, std::vector<Segment>& segs) {
+  std::sort(segs.begin(), segs.end(), [](const Segment& x, const Segment& y) {
+    return x.ny != y.ny ? x.ny < y.ny : x.xl < y.xl;
+  });
+  std::ofstream f(c.amoebaPrefix + std::to_string(i));
+  for (const Segment& g : segs) f << g.ny << "," << g.xl << "," << g.xr << "\n";
+}
+
 struct Job {          // one engine run over a set of curves
   std::vector<Curve*> curves;
   std::vector<cx> starts, ends;
@@ -288,7 +305,42 @@ struct Engine {
 
   void run(Job& job);
   void point_lc(Curve& c, cx s, cx e);
+  void collect_amoebae(const Job& job, const std::vector<int>& ptCurve,
+                       const std::vector<int>& batchPts, unsigned long long hcap, int labelCap,
+                       std::vector<std::vector<Segment>>& out);
 };
+
+// Reads the run table of the last fill back and files each counted run under
+// its point: a run counts when its label's class holds the point's seed label
+// (runs that only scouts reached, in components without a seed, do not).
+void Engine::collect_amoebae(const Job& job, const std::vector<int>& ptCurve,
+                             const std::vector<int>& batchPts, unsigned long long hcap,
+                             int labelCap, std::vector<std::vector<Segment>>& out) {
+  std::vector<unsigned long long> tab(2 * hcap);
+  std::vector<int> parent(labelCap);
+  HIPCHK(hipMemcpy(tab.data(), dHtab.p, tab.size() * sizeof(unsigned long long),
+                   hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(parent.data(), dLParent.p, parent.size() * sizeof(int), hipMemcpyDeviceToHost));
+  auto root = [&](int x) {
+    while (x >= 0 && x < labelCap && parent[x] != x) x = parent[x];
+    return x;
+  };
+  const int nb = (int)batchPts.size();
+  for (unsigned long long e = 0; e < hcap; e++) {
+    const unsigned long long w0 = tab[2 * e], w1 = tab[2 * e + 1];
+    if (w0 == 0) continue;
+    const int b = (int)(w0 & 0x3FFF);
+    const long long xl = (long long)((w0 >> 14) & 0x1FFFFFF);
+    const long long row = (long long)(w0 >> 39) - 1;
+    const int label = (int)(w1 >> 32);
+    const long long xr = (long long)(w1 & 0xFFFFFFFFull);
+    if (b >= nb) continue;
+    const int gp = batchPts[b];
+    if (!job.curves[ptCurve[gp]]->amoebaOut) continue;
+    if (label < 0 || label >= labelCap || root(label) != root(b)) continue;
+    out[gp].push_back(Segment{row, xl, xr});
+  }
+}
 
 Engine& engine() {
   static Engine e;
@@ -405,6 +457,10 @@ void Engine::run(Job& job) {
 
   std::vector<uint64_t> ptRays(nPts, 0);
   std::vector<long long> ptSumAll(nPts, 0);
+  // amoeba print-out: the runs of every point of the curves that asked for it
+  bool wantAmoebae = false;
+  for (auto* c : job.curves) wantAmoebae |= c->amoebaOut && job.runSeeds;
+  std::vector<std::vector<Segment>> amoebae(wantAmoebae ? nPts : 0);
   const unsigned long long qcap = 1ull << opt.queueLog2;
   dQBuf.ensure(qcap);
   dQSeq.ensure(qcap);
@@ -520,6 +576,7 @@ void Engine::run(Job& job) {
         ptRays[batchPts[b]] = cnt[b];
         ptSumAll[batchPts[b]] = sum[b];
       }
+      if (wantAmoebae) collect_amoebae(job, ptCurve, batchPts, hcap, labelCap, amoebae);
       break;
     }
   }
@@ -550,6 +607,8 @@ void Engine::run(Job& job) {
                   (size_t)n * kMaxSeeds * sizeof(int2));
       for (unsigned i = 0; i < c.lcLength && i < (unsigned)n; i++) c.lcVec[i] = c.full[i];
       c.hasIRS = true;
+      if (wantAmoebae && c.amoebaOut)
+        for (int i = 0; i < n; i++) write_amoeba_file(c, i, amoebae[pointBase[k] + i]);
     }
   }
   stats.totalMs =
