@@ -604,7 +604,7 @@ __device__ __forceinline__ void push_task(const FillArgs& A, Task f, int gp) {
 // in registers across the scan.
 struct RowCtx {
   Lens L;
-  double blx, bly, step, rho2, invRho2, ldA, ldB;
+  double blx, bly, step, rho2, ldA, ldB;
   double y, px, py;
   int N;
 };
@@ -618,8 +618,9 @@ struct RowCtx {
 __device__ __forceinline__ bool eval_px(const RowCtx& R, int x, long long& val) {
   const double rsq = irs_ray(R.L, R.blx + R.step * (double)x, R.y, R.px, R.py);
   if (!(rsq < R.rho2)) return false;
-  const double y1 = 1.0 - rsq * R.invRho2;
-  const double v = R.ldA + R.ldB * (y1 * __builtin_amdgcn_rsq(y1));
+  // sourceBrightnessRSq(rsq / rho^2) (lcirs.cc:406-407,
+  // limbdarkeningmodel.cc:45-49): IEEE division and square root
+  const double v = R.ldA + R.ldB * sqrt(1.0 - rsq / R.rho2);
   const double t = fma(v, kSumScale, 4503599627370496.0);  // 2^52: integer in the low bits
   val = __double_as_longlong(t) - 0x4330000000000000ll;
   return true;
@@ -962,7 +963,6 @@ __device__ __forceinline__ void process_task(TaskState& S) {
     S.R.bly = unid(C.bly);
     S.R.step = unid(C.step);
     S.R.rho2 = unid(C.rho2);
-    S.R.invRho2 = 1.0 / S.R.rho2;
     S.R.ldA = unid(C.ldA);
     S.R.ldB = unid(C.ldB);
     S.R.N = S.N;

@@ -20,10 +20,21 @@ from . import engine as _engine
 
 class LC_irs(object):
     def __init__(self, a, b, th, m2, m3, source_size, lc_len, img_plane_size):
-        self.lib_lc = _engine.load()
-        self.lib_lc.copy_lc.argtypes = [ctypes.c_void_p,
-                                        ndpointer(dtype=np.double, flags="C_CONTIGUOUS")]
-        self.obj = self.lib_lc.lcirs_new(a, b, th, m2, m3, source_size, lc_len, img_plane_size)
+        _engine.load()  # raises when the library has not been built
+        # a handle of its own, declared as the reference declares it
+        self.lib_lc = lib = ctypes.CDLL(_engine.LIB_PATH)
+        D = ctypes.c_double
+        lib.lcirs_new.argtypes = [D, D, D, D, D, D, ctypes.c_uint, ctypes.c_long]
+        lib.lcirs_new.restype = ctypes.c_void_p
+        lib.get_lc.argtypes = [ctypes.c_void_p, D, D, D, D]
+        lib.get_lc_irs.argtypes = [ctypes.c_void_p, D, D, D, D]
+        lib.copy_lc.argtypes = [ctypes.c_void_p, ndpointer(dtype=np.double, flags="C_CONTIGUOUS")]
+        lib.set_limb_darkening.argtypes = [ctypes.c_void_p, ctypes.c_char_p, D]
+        lib.set_amoeba_printout.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_char_p]
+        lib.lcirs_delete.argtypes = [ctypes.c_void_p]
+        lib.lcirs_copy_ray_counts.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        lib.lcirs_copy_lc_full.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.obj = lib.lcirs_new(a, b, th, m2, m3, source_size, lc_len, img_plane_size)
         if not self.obj:
             raise _engine.EngineError(_engine.last_error() or "lcirs_new failed")
         self.lc_len = int(lc_len)
