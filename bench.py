@@ -112,6 +112,22 @@ def cpu_baseline(c, cores):
     return out
 
 
+# ---------------------------------------------------------------- copy kernel
+def calibrate_copy(mib):
+    """Three device-to-device copies of `mib` MiB by torch's copy kernel: each
+    reads and writes exactly mib MiB of HBM, so the FETCH_SIZE and WRITE_SIZE
+    that rocprofv3 reports for it give the counters' scale."""
+    import torch
+    n = mib * (1 << 20)
+    src = torch.ones(n, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n, "copies": 3}), flush=True)
+    return 0
+
+
 # ---------------------------------------------------------------- main
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
@@ -120,7 +136,12 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-cores", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--calibrate-copy", type=int, metavar="MIB", default=0,
+                    help="only run a plain device copy of MIB MiB three times (the reference "
+                         "kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
     args = ap.parse_args(argv)
+    if args.calibrate_copy:
+        return calibrate_copy(args.calibrate_copy)
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has a single MI355X and the "
               "engine has no multi-GPU path; run with --gpus 1", file=sys.stderr)

@@ -101,6 +101,12 @@ int lcirs_fill_seeds(void* lc, int n, const double* pos, const int* nSeeds, cons
  * out must hold 12 values. */
 int tl_last_stats(double* out);
 
+/* Host self-test of the branch-free complex routines the kernels use (modulus,
+ * division, square root) against the general ones on n random operand pairs,
+ * normal, tiny, huge, zero, infinite and NaN; returns the number of pairs
+ * whose results differ in any bit.  No counterpart in the reference. */
+int tl_math_selftest(int n, unsigned long long seed);
+
 /* Engine settings: name/value pairs ("fill_workgroups", "probe_limit",
  * "queue_log2", "scout_rows", "scout_jump", "time_limit_s").  Returns 0 or
  * -1 for an unknown name. */

@@ -71,6 +71,8 @@ class Oracle:
         lib.tlo_libm_cdiv.argtypes = [_D, _D, _D, _D, _P]
         lib.tlo_seeds_margin.argtypes = [_P, _D, _D, _D, _D, ctypes.c_int, _P, _P, _P, _P]
         lib.tlo_fill_stats.argtypes = [_P]
+        lib.tlo_chain_states.argtypes = [_P, _D, _D, _D, _D, ctypes.c_uint, ctypes.c_uint,
+                                         ctypes.c_int, _P]
         lib.tlo_last_segments.argtypes = [_P, _P, ctypes.c_long]
         lib.tlo_last_segments.restype = ctypes.c_long
         lib.tlo_hit_mask.argtypes = [_P, ctypes.c_long, ctypes.c_long, ctypes.c_long,
@@ -204,6 +206,15 @@ class LC:
         if rc:
             raise RuntimeError(f"tlo_fill_point failed ({rc})")
         return int(rays[0]), float(ssum[0]), bool(edge[0])
+
+    def chain_states(self, sx, sy, ex, ey, i0, i1, fresh):
+        """Root-chain state after each point i0..i1: (n, 11) complex roots and
+        (n,) have-roots flags."""
+        n = i1 - i0 + 1
+        out = np.zeros((n, 23))
+        if self._o.lib.tlo_chain_states(self.h, sx, sy, ex, ey, i0, i1, int(fresh), _ptr(out)):
+            raise ValueError("bad chain range")
+        return out[:, :22].copy().view(np.complex128), out[:, 22].astype(int)
 
     def last_segments(self):
         """The amoeba of the last fill_point: (n, 3) rows of ny, xleft, xright,

@@ -219,10 +219,15 @@ void tlo_cdiv(double a, double b, double c, double d, double* out) {
  * table once i >= 64; here the index wraps instead. */
 static const double lag_frac[8] = {0.5, 0.23, 0.73, 0.13, 0.37, 0.61, 0.89, 1.0};
 
+/* Work counters of the seed search (test instrumentation): [0] image solves
+ * (getImages calls), [1] Laguerre calls, [2] Laguerre iterations. */
+static __thread long g_work[3];
+
 /* Laguerre::laguerre (laguerre.cc:18-99).  poly has m+1 coefficients. */
 static int lag_iter(const cx* poly, int m, cx* xp) {
   cx x = *xp;
   const double md = (double)m;
+  g_work[1]++;
   if (m == 1) {
     if (cabs_(poly[1]) != 0.0) {
       *xp = cdiv(cneg(poly[0]), poly[1]);
@@ -232,6 +237,7 @@ static int lag_iter(const cx* poly, int m, cx* xp) {
     return 0;
   }
   for (int i = 1; i <= LAG_MAX_IT; i++) {
+    g_work[2]++;
     cx b = poly[m];
     double err = cabs_(b);
     cx s = mk(0.0, 0.0), f = mk(0.0, 0.0);
@@ -544,6 +550,7 @@ static void img_classify(imgpoint_t* ip, cx zeta) {
 
 /* ImgPoint::getImages(pos) for a triple lens (imgpoint.cc:208-274,281-286) */
 static int img_get(imgpoint_t* ip, cx zeta) {
+  g_work[0]++;
   img_roots(ip, zeta, 0);
   img_classify(ip, zeta);
   if (ip->nImg % 2 != 0 || ip->nImg < 4) {
@@ -1003,6 +1010,12 @@ static void amoeba_stats(const tlo_lc* lc) {
 }
 int tlo_fill_stats(long* out) { for (int i = 0; i < 5; i++) out[i] = g_stats[i]; return 0; }
 
+int tlo_work_counts(long* out, int reset) {
+  for (int i = 0; i < 3; i++) out[i] = g_work[i];
+  if (reset) for (int i = 0; i < 3; i++) g_work[i] = 0;
+  return 0;
+}
+
 long tlo_last_segments(const tlo_lc* lc, long* out, long cap) {
   if (!lc) return -1;
   long n = 0;
@@ -1105,6 +1118,32 @@ static int point_seeds(tlo_lc* lc, cx pos, cx** seeds, int* cap, int* nImgOut) {
 
 static inline cx traj_pos(cx s, cx e, unsigned i, unsigned len) {
   return cadd(s, cscale(csub(e, s), (double)i / (double)len));
+}
+
+int tlo_chain_states(tlo_lc* lc, double sx, double sy, double ex, double ey, unsigned i0,
+                     unsigned i1, int fresh, double* states) {
+  if (!lc || i1 < i0 || i1 > lc->lcLength) return -1;
+  if (fresh) {
+    for (int k = 0; k <= MAXDEG; k++) lc->ip.tempRoots[k] = mk(0.0, 0.0);
+    lc->ip.haveRoots = 0;
+  }
+  cx s = mk(sx, sy), e = mk(ex, ey);
+  cx* seeds = NULL;
+  int cap = 0;
+  for (unsigned i = i0; i <= i1; i++) {
+    if (point_seeds(lc, traj_pos(s, e, i, lc->lcLength), &seeds, &cap, NULL) < 0) {
+      free(seeds);
+      return -2;
+    }
+    double* o = states + (size_t)(i - i0) * (2 * (MAXDEG + 1) + 1);
+    for (int k = 0; k <= MAXDEG; k++) {
+      o[2 * k] = lc->ip.tempRoots[k].re;
+      o[2 * k + 1] = lc->ip.tempRoots[k].im;
+    }
+    o[2 * (MAXDEG + 1)] = lc->ip.haveRoots;
+  }
+  free(seeds);
+  return 0;
 }
 
 int tlo_get_lc_irs(tlo_lc* lc, double sx, double sy, double ex, double ey,

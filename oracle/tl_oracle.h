@@ -47,6 +47,13 @@ int tlo_get_lc_irs(tlo_lc* lc, double sx, double sy, double ex, double ey,
 int tlo_seeds(tlo_lc* lc, double sx, double sy, double ex, double ey,
               int maxSeeds, double* seedXY, int* nSeeds, int* nImg);
 
+/* Root-reuse chain states (ImgPoint tempRoots[0..10] as 22 doubles, then the
+ * have-roots flag) after each point i0..i1 of the seed search, continuing
+ * from the object's current chain or, with fresh != 0, from the state of a
+ * new object.  23 doubles per point. */
+int tlo_chain_states(tlo_lc* lc, double sx, double sy, double ex, double ey, unsigned i0,
+                     unsigned i1, int fresh, double* states);
+
 /* As tlo_seeds, plus margin[i]: the smallest distance, in decades, between
  * the lens-equation residual of any root found for point i (centre and
  * caustic trial points) and the 1e-6 image threshold. */
@@ -112,6 +119,10 @@ void tlo_libm_cdiv(double a, double b, double c, double d, double* out);
  * amoeba.cc:15-30): (ny, xleft, xright) triples into out (room for cap);
  * returns their number. */
 long tlo_last_segments(const tlo_lc* lc, long* out, long cap);
+
+/* Seed-search work since the last reset on this thread: out[0] image solves,
+ * out[1] Laguerre calls, out[2] Laguerre iterations (test instrumentation). */
+int tlo_work_counts(long* out, int reset);
 
 /* Amoeba statistics of the last fill (ranges, rows, max ranges per row, min
  * row, max row). */

@@ -83,3 +83,15 @@ def test_ctypes_class_mirrors_reference(lib):
     lc = LC_irs(1.0, 0.9433981132056605, 1.0121970114513341, 0.1, 0.1, 1e-3, 199, 50)
     lc.set_limb_darkening(b"linear", 0.0)
     lc.set_limb_darkening("default", 0.0)
+
+
+def test_branch_free_complex_routines_match_general_ones():
+    """The kernels' branch-free modulus, division and square root give the
+    general routines' results to the bit (random normal, tiny, huge, zero,
+    infinite and NaN operands; run on the host)."""
+    import ctypes
+    from triplelens_amd import engine as E
+    lib = E.load()
+    lib.tl_math_selftest.argtypes = [ctypes.c_int, ctypes.c_ulonglong]
+    for seed in (1, 12345, 2 ** 40 + 7):
+        assert lib.tl_math_selftest(300000, seed) == 0

@@ -214,12 +214,14 @@ struct DBuf {
 };
 
 struct Options {
-  int fillWorkgroups = 0;  // 0: 2 per compute unit (8 resident waves)
+  int fillWorkgroups = 0;  // 0: one per compute unit (4 waves), capped at the occupancy limit
   int probeLimit = 4096;
   int queueLog2 = 21;
   int scoutRows = 8;
   int scoutJump = 256;
   double timeLimitS = 60.0;
+  int seedChunks = 3;       // the seed search is cut into this many chunks of positions
+  int seedChunkMin = 64;    // ... of at least this many positions per curve
 };
 
 struct Stats {
@@ -259,19 +261,21 @@ struct Engine {
   bool ready = false;
   int cus = 0;
   std::string name;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;   // fills and copies
+  hipStream_t sstream = nullptr;  // seed search (overlaps the fills)
+  std::vector<hipEvent_t> seedEvents;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   DBuf<TrigTable> dTrig;
   DBuf<CurveDev> dCurves;
   DBuf<cx> dCaus;
   DBuf<int2> dSeeds;
-  DBuf<int> dNSeeds, dPtCurve, dBatchPts, dLParent, dLPoint, dInts;
+  DBuf<int> dNSeeds, dPtCurve, dBatchPts, dLParent, dLPoint;
   DBuf<double2> dPtPos;
   DBuf<float> dCost;
   DBuf<uint32_t> dFlags;
   DBuf<int2> dSeedTasks;
-  DBuf<unsigned long long> dHtab, dLCount, dQSeq, dCounters, dPtCount;
-  DBuf<long long> dLSum, dPending, dPtSum;
+  DBuf<unsigned long long> dHtab, dLCount, dQSeq, dSync, dPtCount;
+  DBuf<long long> dLSum, dPtSum;
   DBuf<Task> dQBuf;
   DBuf<RootChain> dChain;
   DBuf<double> dPointLC;
@@ -290,6 +294,7 @@ struct Engine {
     cus = prop.multiProcessorCount;
     name = prop.gcnArchName;
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&sstream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&ev0));
     HIPCHK(hipEventCreate(&ev1));
     dTrig.ensure(1);
@@ -413,7 +418,24 @@ void Engine::run(Job& job) {
   std::vector<int> nSeeds(nPts);
   std::vector<float> cost(nPts);
   std::vector<uint32_t> flags(nPts, 0);
+  // Seed search and fills overlap: the seed kernel walks the positions in
+  // chunks on its own stream, and each chunk's fill runs on the engine stream
+  // as soon as the chunk's seeds are there.
+  unsigned maxLen = 0;
+  for (int k = 0; k < nc; k++)
+    maxLen = std::max<unsigned>(maxLen, (unsigned)(pointBase[k + 1] - pointBase[k]));
+  // chunk ch holds positions [len ch / n, len (ch+1) / n) of every curve of
+  // len positions, so that every chain does the same share of its work
+  int nChunks = 1;
+  std::vector<hipEvent_t>& evSeed = seedEvents;
   if (job.runSeeds) {
+    nChunks = std::max(1, std::min<int>(opt.seedChunks, (int)(maxLen / std::max(1, opt.seedChunkMin))));
+    while ((int)evSeed.size() < nChunks + 1) {
+      hipEvent_t ev;
+      HIPCHK(hipEventCreate(&ev));
+      evSeed.push_back(ev);
+    }
+    dChain.ensure(nc);
     SeedArgs sa;
     sa.curves = dCurves.p;
     sa.nCurves = nc;
@@ -424,19 +446,18 @@ void Engine::run(Job& job) {
     sa.ptPos = dPtPos.p;
     sa.cost = dCost.p;
     sa.flags = dFlags.p;
-    dChain.ensure(nc);
     sa.chainOut = dChain.p;
+    // the seed stream starts after the set-up copies on the engine stream
     HIPCHK(hipEventRecord(ev0, stream));
-    HIPCHK((hipError_t)launch_seed(sa, stream));
-    HIPCHK(hipEventRecord(ev1, stream));
-    HIPCHK(hipStreamSynchronize(stream));
-    stats.seedMs = elapsed();
-    HIPCHK(hipMemcpy(nSeeds.data(), dNSeeds.p, nPts * sizeof(int), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(cost.data(), dCost.p, nPts * sizeof(float), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(flags.data(), dFlags.p, nPts * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    std::vector<RootChain> chains(nc);
-    HIPCHK(hipMemcpy(chains.data(), dChain.p, nc * sizeof(RootChain), hipMemcpyDeviceToHost));
-    for (int k = 0; k < nc; k++) job.curves[k]->chain = chains[k];
+    HIPCHK(hipStreamWaitEvent(sstream, ev0, 0));
+    HIPCHK(hipEventRecord(evSeed[nChunks], sstream));
+    for (int ch = 0; ch < nChunks; ch++) {
+      sa.chunk = ch;
+      sa.nChunks = nChunks;
+      sa.chainIn = ch == 0 ? nullptr : dChain.p;
+      HIPCHK((hipError_t)launch_seed(sa, sstream));
+      HIPCHK(hipEventRecord(evSeed[ch], sstream));
+    }
   } else {
     for (int p = 0; p < nPts; p++) {
       nSeeds[p] = std::min(job.nSeeds[p], kMaxSeeds);
@@ -448,13 +469,6 @@ void Engine::run(Job& job) {
     HIPCHK(hipMemcpy(dFlags.p, flags.data(), nPts * sizeof(uint32_t), hipMemcpyHostToDevice));
   }
 
-  // Largest predicted cost first, in batches of at most kMaxBatch positions.
-  std::vector<int> order;
-  order.reserve(nPts);
-  for (int p = 0; p < nPts; p++)
-    if (nSeeds[p] > 0) order.push_back(p);
-  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
-
   std::vector<uint64_t> ptRays(nPts, 0);
   std::vector<long long> ptSumAll(nPts, 0);
   // amoeba print-out: the runs of every point of the curves that asked for it
@@ -464,11 +478,39 @@ void Engine::run(Job& job) {
   const unsigned long long qcap = 1ull << opt.queueLog2;
   dQBuf.ensure(qcap);
   dQSeq.ensure(qcap);
-  dCounters.ensure(12);
-  dPending.ensure(1);
-  dInts.ensure(4);
-  const int nWg = opt.fillWorkgroups > 0 ? opt.fillWorkgroups : cus * 2;
+  // the fill's shared counters, each on a 128-byte line of its own: waves
+  // that poll one of them do not slow the atomics on the others
+  constexpr int kLine = 16;  // u64 per 128 bytes
+  dSync.ensure(7 * kLine);
+  // every wave of the fill must be resident at once (its end test counts
+  // them all), so the launch is capped at the occupancy limit
+  const int resident = fill_resident_workgroups();
+  if (resident <= 0) throw std::runtime_error("fill kernel occupancy query failed");
+  int nWg = opt.fillWorkgroups > 0 ? opt.fillWorkgroups : cus;
+  if (nWg > resident) nWg = resident;
 
+  std::vector<int> order;
+  for (int ch = 0; ch < nChunks; ch++) {
+  if (job.runSeeds) {
+    // this chunk's seed counts and predicted costs (the copy waits for the
+    // chunk's seed launch; later chunks are still being written)
+    HIPCHK(hipStreamWaitEvent(stream, evSeed[ch], 0));
+    HIPCHK(hipMemcpyAsync(nSeeds.data(), dNSeeds.p, nPts * sizeof(int), hipMemcpyDeviceToHost,
+                          stream));
+    HIPCHK(hipMemcpyAsync(cost.data(), dCost.p, nPts * sizeof(float), hipMemcpyDeviceToHost,
+                          stream));
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+  // the chunk's positions, largest predicted cost first, in batches of at
+  // most kMaxBatch positions
+  order.clear();
+  for (int k = 0; k < nc; k++) {
+    const long long len = pointBase[k + 1] - pointBase[k];
+    const int i0 = (int)(len * ch / nChunks), i1 = (int)(len * (ch + 1) / nChunks);
+    for (int i = i0; i < i1; i++)
+      if (nSeeds[pointBase[k] + i] > 0) order.push_back(pointBase[k] + i);
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
   for (size_t b0 = 0; b0 < order.size(); b0 += kMaxBatch) {
     const int nb = (int)std::min<size_t>(kMaxBatch, order.size() - b0);
     std::vector<int> batchPts(order.begin() + b0, order.begin() + b0 + nb);
@@ -512,18 +554,19 @@ void Engine::run(Job& job) {
       fa.lpoint = dLPoint.p;
       fa.lcount = dLCount.p;
       fa.lsum = dLSum.p;
-      fa.labelNext = dInts.p + 1;
+      fa.labelNext = reinterpret_cast<int*>(dSync.p + 4 * kLine);
       fa.labelCap = labelCap;
       fa.qbuf = dQBuf.p;
       fa.qseq = dQSeq.p;
-      fa.qhead = dCounters.p + 0;
-      fa.qtail = dCounters.p + 1;
+      fa.qhead = dSync.p + 0 * kLine;
+      fa.qtail = dSync.p + 1 * kLine;
       fa.qcap = qcap;
-      fa.pending = dPending.p;
-      fa.seedNext = dInts.p + 0;
+      fa.active = reinterpret_cast<int*>(dSync.p + 2 * kLine);
+      fa.nWaves = nWg * (kFillThreads / 64);
+      fa.seedNext = reinterpret_cast<int*>(dSync.p + 3 * kLine);
       fa.flags = dFlags.p;
-      fa.abortFlag = dInts.p + 2;
-      fa.stats = dCounters.p + 2;
+      fa.abortFlag = reinterpret_cast<int*>(dSync.p + 5 * kLine);
+      fa.stats = dSync.p + 6 * kLine;
       fa.timeLimitTicks = (long long)(opt.timeLimitS * 1.0e8);
       fa.scoutRows = opt.scoutRows;
       fa.scoutJump = opt.scoutJump;
@@ -537,10 +580,11 @@ void Engine::run(Job& job) {
       HIPCHK(hipStreamSynchronize(stream));
       stats.fillMs += elapsed();
       stats.fillLaunches += 1;
-      unsigned long long st[10];
-      int ints[4];
-      HIPCHK(hipMemcpy(st, dCounters.p, 10 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-      HIPCHK(hipMemcpy(ints, dInts.p, 3 * sizeof(int), hipMemcpyDeviceToHost));
+      std::vector<unsigned long long> sync(7 * kLine);
+      HIPCHK(hipMemcpy(sync.data(), dSync.p, sync.size() * sizeof(unsigned long long),
+                       hipMemcpyDeviceToHost));
+      const unsigned long long* st = sync.data() + 6 * kLine - 2;  // st[2..9] = stats[0..7]
+      const bool aborted = *reinterpret_cast<const int*>(sync.data() + 5 * kLine) != 0;
       std::vector<uint32_t> bflags(nPts);
       HIPCHK(hipMemcpy(bflags.data(), dFlags.p, nPts * sizeof(uint32_t), hipMemcpyDeviceToHost));
       bool hashFull = false, labelFull = false;
@@ -548,6 +592,7 @@ void Engine::run(Job& job) {
         hashFull |= (bflags[batchPts[b]] & kFlagHashFull) != 0;
         labelFull |= (bflags[batchPts[b]] & kFlagLabelFull) != 0;
       }
+      (void)hashFull;
       stats.tasks += (double)st[2];
       stats.evals += (double)st[3];
       stats.ins += (double)st[4];
@@ -556,14 +601,16 @@ void Engine::run(Job& job) {
       stats.insertMs += (double)st[7] * 1e-5;
       stats.idlePolls += (double)st[8];
       stats.waves = (double)st[9];
-      if (ints[2]) {
+      if (aborted) {
         for (int b = 0; b < nb; b++) flags[batchPts[b]] |= kFlagTimeout;
         throw std::runtime_error("fill kernel hit its time limit");
       }
       if ((hashFull || labelFull) && attempt < 3) {
-        // clear the overflow bits and run the batch again with bigger tables
-        for (int b = 0; b < nb; b++) bflags[batchPts[b]] &= ~(kFlagHashFull | kFlagLabelFull);
-        HIPCHK(hipMemcpy(dFlags.p, bflags.data(), nPts * sizeof(uint32_t), hipMemcpyHostToDevice));
+        // clear the overflow bits of the batch's points and run it again
+        // with bigger tables (other points' flags may be in use by the seed
+        // search)
+        HIPCHK((hipError_t)launch_clear_flag_bits(dFlags.p, dBatchPts.p, nb,
+                                                  kFlagHashFull | kFlagLabelFull, stream));
         if (hashFull) hcap = std::min<unsigned long long>(hcap << 2, 1ull << 29);
         if (labelFull) labelCap *= 4;
         continue;
@@ -580,7 +627,20 @@ void Engine::run(Job& job) {
       break;
     }
   }
+  }  // chunks
+  if (job.runSeeds) {
+    HIPCHK(hipStreamSynchronize(sstream));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, evSeed[nChunks], evSeed[nChunks - 1]));
+    stats.seedMs = ms;
+    std::vector<RootChain> chains(nc);
+    HIPCHK(hipMemcpy(chains.data(), dChain.p, nc * sizeof(RootChain), hipMemcpyDeviceToHost));
+    for (int k = 0; k < nc; k++) job.curves[k]->chain = chains[k];
+    HIPCHK(hipMemcpy(nSeeds.data(), dNSeeds.p, nPts * sizeof(int), hipMemcpyDeviceToHost));
+  }
   HIPCHK(hipMemcpy(flags.data(), dFlags.p, nPts * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  uint32_t allFlags = 0;
+  for (uint32_t f : flags) allFlags |= f;
 
   // results back into the objects
   std::vector<int2> seedsHost;
@@ -613,6 +673,11 @@ void Engine::run(Job& job) {
   }
   stats.totalMs =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tStart).count();
+  // results are in place (with their flags); a lost task makes them wrong
+  if (allFlags & kFlagQueueFull)
+    throw std::runtime_error("fill task queue overflow: some light-curve points are incomplete");
+  if (allFlags & (kFlagHashFull | kFlagLabelFull))
+    throw std::runtime_error("fill tables still full after enlarging: results are incomplete");
 }
 
 void Engine::point_lc(Curve& c, cx s, cx e) {
@@ -858,6 +923,46 @@ int lcirs_fill_seeds(void* lc, int n, const double* pos, const int* nSeeds, cons
   });
 }
 
+int tl_math_selftest(int n, unsigned long long seed) {
+  // the branch-free complex helpers against the general ones, on the host
+  unsigned long long st = seed ? seed : 1;
+  auto next = [&]() {
+    st ^= st << 13;
+    st ^= st >> 7;
+    st ^= st << 17;
+    return st;
+  };
+  auto val = [&]() -> double {
+    const unsigned long long r = next();
+    switch (r & 15) {
+      case 0: return 0.0;
+      case 1: return -0.0;
+      case 2: return (r & 16) ? (double)INFINITY : -(double)INFINITY;
+      case 3: return (double)NAN;
+      case 4: return ldexp((double)(next() >> 11) * 0x1p-53, (int)(next() % 2100) - 1075);
+      default: {
+        const double m = (double)(next() >> 11) * 0x1p-53 + 0.5;
+        const double v = ldexp(m, (int)(next() % 41) - 20);
+        return (r & 32) ? -v : v;
+      }
+    }
+  };
+  auto same = [](double x, double y) {
+    return (std::isnan(x) && std::isnan(y)) || std::memcmp(&x, &y, sizeof x) == 0;
+  };
+  int bad = 0;
+  for (int i = 0; i < n; i++) {
+    const cx u = mk(val(), val()), v = mk(val(), val());
+    const double h0 = hyp(u.re, u.im), h1 = hyp_sel(u.re, u.im);
+    const cx d0 = cdiv(u, v), d1 = cdiv_sel(u, v);
+    const cx q0 = csqrt_(u), q1 = csqrt_sel(u);
+    if (!same(h0, h1) || !same(d0.re, d1.re) || !same(d0.im, d1.im) || !same(q0.re, q1.re) ||
+        !same(q0.im, q1.im))
+      bad++;
+  }
+  return bad;
+}
+
 int tl_last_stats(double* out) {
   const Stats& s = engine().stats;
   out[0] = s.tasks; out[1] = s.evals; out[2] = s.ins; out[3] = s.dup;
@@ -876,6 +981,8 @@ int tl_set_option(const char* name, double value) {
   else if (n == "scout_rows") o.scoutRows = std::max(0, (int)value);
   else if (n == "scout_jump") o.scoutJump = std::max(1, (int)value);
   else if (n == "time_limit_s") o.timeLimitS = std::max(0.1, value);
+  else if (n == "seed_chunks") o.seedChunks = std::max(1, (int)value);
+  else if (n == "seed_chunk_min") o.seedChunkMin = std::max(1, (int)value);
   else return -1;
   return 0;
 }

@@ -11,6 +11,8 @@ constexpr int kCccLen = 500;       // critical-curve samples per branch (lcirs.c
 constexpr int kMaxSeeds = 64;      // seeds kept per source position
 constexpr int kMaxBatch = 16384;   // source positions per fill launch (14-bit key field)
 constexpr int kMaxCover = 16;      // known runs read per row lookup
+constexpr int kFillThreads = 256;  // threads per fill workgroup (4 waves)
+constexpr int kLocalCap = 64;      // tasks in each wave's own ring (LDS)
 
 // Per-point status bits (OR-ed into flags[p]).
 enum : uint32_t {
@@ -64,6 +66,9 @@ struct SeedArgs {
   float* cost;        // predicted rays (point-source magnification x pi ppr^2)
   uint32_t* flags;
   RootChain* chainOut; // per curve: root-reuse state after the last position
+  const RootChain* chainIn;  // per curve: state to start from (nullptr: the curve's own)
+  int chunk, nChunks;  // positions [n chunk / nChunks, n (chunk+1) / nChunks) of every
+                       // curve of n positions
 };
 
 struct FillArgs {
@@ -86,13 +91,14 @@ struct FillArgs {
   long long* lsum;
   int* labelNext;
   int labelCap;
-  // queue
+  // shared queue (tasks handed between waves) and termination
   Task* qbuf;
   unsigned long long* qseq;
   unsigned long long* qhead;
   unsigned long long* qtail;
   unsigned long long qcap;
-  long long* pending;
+  int* active;                // waves holding work (the fill ends when 0 and no queued task)
+  int nWaves;                 // waves of the launch
   int* seedNext;
   // results and status
   uint32_t* flags;            // per global point
@@ -107,6 +113,8 @@ struct FillArgs {
 // Host-side launchers (tl_kernels.hip).  All return hipError_t as int.
 int launch_seed(const SeedArgs& a, hipStream_t s);
 int launch_fill(const FillArgs& a, int nWorkgroups, hipStream_t s);
+int fill_resident_workgroups();
+int launch_clear_flag_bits(uint32_t* flags, const int* pts, int n, uint32_t bits, hipStream_t s);
 int launch_clear_u64(unsigned long long* p, unsigned long long n, unsigned long long v, hipStream_t s);
 int launch_fill_reset(const FillArgs& a, hipStream_t s);
 int launch_resolve(const FillArgs& a, int nLabels, unsigned long long* ptCount, long long* ptSum,

@@ -102,192 +102,293 @@ __device__ double point_amp_term(const Lens& L, cx z) {
 }
 
 // ---- wave-cooperative seed search -----------------------------------------
-// One wavefront per light curve.  The root-reuse chain is inherently serial
-// and every lane runs it with the same values; the independent parts (the 10
-// root polishes, the 10 image tests and the caustic-segment tests) are spread
-// over the lanes and their results broadcast, so values and their order are
-// those of the serial code.
+// One wavefront per light curve.  The root-reuse chain is serial; its vectors
+// are spread over the lanes (lane k holds coefficient k, or root k) and the
+// serial steps read single elements with readlane, so nothing is indexed at
+// run time and everything stays in registers.  The independent parts (the
+// coefficients, the 10 root polishes, the 10 image tests and the
+// caustic-segment tests) run on separate lanes.  Values and their order are
+// those of the serial code in tl_math.h, operation for operation.
 
-__device__ __forceinline__ cx shfl_cx(cx v, int src) {
-  return mk(__shfl(v.re, src, 64), __shfl(v.im, src, 64));
+// Laguerre's fractional kicks (laguerre.cc:22), read modulo 8 as lag_iter does.
+__constant__ double kLagFrac[8] = {0.5, 0.23, 0.73, 0.13, 0.37, 0.61, 0.89, 1.0};
+
+__device__ __forceinline__ double rl(double v, int k) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, k);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), k);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
-
-__device__ __forceinline__ dcx shfl_dcx(dcx v, int src) {
+__device__ __forceinline__ cx rlc(const cx& v, int k) { return mk(rl(v.re, k), rl(v.im, k)); }
+__device__ __forceinline__ dd rld(const dd& v, int k) {
+  dd r;
+  r.hi = rl(v.hi, k);
+  r.lo = rl(v.lo, k);
+  return r;
+}
+__device__ __forceinline__ dcx rldc(const dcx& v, int k) {
   dcx r;
-  r.re.hi = __shfl(v.re.hi, src, 64);
-  r.re.lo = __shfl(v.re.lo, src, 64);
-  r.im.hi = __shfl(v.im.hi, src, 64);
-  r.im.lo = __shfl(v.im.lo, src, 64);
+  r.re = rld(v.re, k);
+  r.im = rld(v.im, k);
   return r;
 }
 
-// Coefficient k of the product of p (np terms) and q (nq terms), accumulated
-// from zero in increasing order of p's index, as dpmul does.
-__device__ __forceinline__ dcx poly_coef(const dcx* p, int np, const dcx* q, int nq, int k) {
-  dcx out = dc_zero();
-  for (int i = 0; i < np; i++) {
-    const int j = k - i;
-    if (j >= 0 && j < nq) out = dc_add(out, dc_mul(p[i], q[j]));
+// Lane l receives lane l-N's value within its row of 16 lanes (DPP row_shr).
+template <int N>
+__device__ __forceinline__ double shr(double v) {
+  if constexpr (N == 0) {
+    return v;
+  } else {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)b, 0x110 + N, 0xF, 0xF, true);
+    const unsigned hi =
+        (unsigned)__builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x110 + N, 0xF, 0xF, true);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
   }
-  return out;
+}
+template <int N>
+__device__ __forceinline__ dcx shr_dc(const dcx& v) {
+  dcx r;
+  r.re.hi = shr<N>(v.re.hi);
+  r.re.lo = shr<N>(v.re.lo);
+  r.im.hi = shr<N>(v.im.hi);
+  r.im.lo = shr<N>(v.im.lo);
+  return r;
 }
 
-// lens_poly on a wavefront: each lane computes whole coefficients of one
-// stage, with the same operation sequence as the serial code, and the stage's
-// results are broadcast to all lanes.
-__device__ void wave_lens_poly(const Lens& L, const PolyBase& pb, cx zeta, cx* c) {
-  const int ln = __lane_id();
+// Coefficient k (this lane's) of the product p q as dpmul forms it: the terms
+// p[i] q[k-i] with 0 <= k-i < NQ added in increasing i.  p[i] is lane i of pv
+// and q[j] lane j of qv.
+template <int NP, int NQ, int I = 0>
+__device__ __forceinline__ void dp_lane(const dcx& pv, const dcx& qv, int k, dcx& out) {
+  if constexpr (I < NP) {
+    const dcx p = rldc(pv, I);
+    const dcx q = shr_dc<I>(qv);
+    if (k - I >= 0 && k - I < NQ) out = dc_add(out, dc_mul(p, q));
+    dp_lane<NP, NQ, I + 1>(pv, qv, k, out);
+  }
+}
+
+// lens_poly (tl_math.h) with the coefficients of every stage computed on
+// separate lanes: lane k returns c_k (k <= 10; 0 on the other lanes).  Dv and
+// Sv hold the zeta-free factors D[k] (k < 4) and S[k] (k < 3) on lane k.
+__device__ __forceinline__ cx wl_poly(const Lens& L, const dcx& Dv, const dcx& Sv, cx zeta) {
+  const int k = __lane_id();
   const dd m1 = dd_of(L.m1), m2 = dd_of(L.m2), m3 = dd_of(L.m3);
   const dcx zc = dc_of(cconj(zeta));
-  const dcx w[3] = {dc_sub(zc, dc_of(cconj(L.z1))), dc_sub(zc, dc_of(cconj(L.z2))),
-                    dc_sub(zc, dc_of(cconj(L.z3)))};
-  // stage 1: Q_k[i] = w_k D[i] + S[i], lanes 0..11
-  dcx Q[3][4];
-  {
-    dcx v = dc_zero();
-    if (ln < 12) {
-      const int k = ln >> 2, i = ln & 3;
-      const dcx sv = i < 3 ? pb.S[i] : dc_zero();
-      v = dc_add(dc_mul(w[k], pb.D[i]), sv);
-    }
-    for (int l = 0; l < 12; l++) Q[l >> 2][l & 3] = shfl_dcx(v, l);
+  const dcx w1 = dc_sub(zc, dc_of(cconj(L.z1))), w2 = dc_sub(zc, dc_of(cconj(L.z2))),
+            w3 = dc_sub(zc, dc_of(cconj(L.z3)));
+  dcx Q1 = dc_zero(), Q2 = dc_zero(), Q3 = dc_zero();
+  if (k < 4) {
+    const dcx sk = k < 3 ? Sv : dc_zero();
+    Q1 = dc_add(dc_mul(w1, Dv), sk);
+    Q2 = dc_add(dc_mul(w2, Dv), sk);
+    Q3 = dc_add(dc_mul(w3, Dv), sk);
   }
-  // stage 2: q12, q13, q23 (7 coefficients each), lanes 0..20
-  dcx q[3][7];
-  {
-    dcx v = dc_zero();
-    if (ln < 21) {
-      const int pr = ln / 7, k = ln % 7;
-      const dcx* P = pr == 2 ? Q[1] : Q[0];
-      const dcx* R = pr == 0 ? Q[1] : Q[2];
-      v = poly_coef(P, 4, R, 4, k);
-    }
-    for (int l = 0; l < 21; l++) q[l / 7][l % 7] = shfl_dcx(v, l);
+  dcx q12 = dc_zero(), q13 = dc_zero(), q23 = dc_zero();
+  dp_lane<4, 4>(Q1, Q2, k, q12);
+  dp_lane<4, 4>(Q1, Q3, k, q13);
+  dp_lane<4, 4>(Q2, Q3, k, q23);
+  if (k >= 7) {
+    q12 = dc_zero();
+    q13 = dc_zero();
+    q23 = dc_zero();
   }
-  // stage 3: q123 = q12 Q3 (lanes 0..9), mix = m1 q23 + m2 q13 + m3 q12 (lanes 10..16)
-  dcx q123[10], mix[7];
+  dcx q123 = dc_zero();
+  dp_lane<7, 4>(q12, Q3, k, q123);
+  if (k >= 10) q123 = dc_zero();
+  dcx mix = dc_zero();
+  if (k < 7) mix = dc_add(dc_add(dc_scale(q23, m1), dc_scale(q13, m2)), dc_scale(q12, m3));
+  // left = (z - zeta) q123: lin[0] = -zeta, lin[1] = 1
+  dcx left = dc_zero();
   {
-    dcx v = dc_zero();
-    if (ln < 10) {
-      v = poly_coef(q[0], 7, Q[2], 4, ln);
-    } else if (ln < 17) {
-      const int i = ln - 10;
-      v = dc_add(dc_add(dc_scale(q[2][i], m1), dc_scale(q[1][i], m2)), dc_scale(q[0][i], m3));
-    }
-    for (int l = 0; l < 10; l++) q123[l] = shfl_dcx(v, l);
-    for (int l = 0; l < 7; l++) mix[l] = shfl_dcx(v, 10 + l);
+    dcx lin = dc_zero();
+    if (k == 0) lin = dc_of(cneg(zeta));
+    if (k == 1) lin = dc_of(mk(1.0, 0.0));
+    dp_lane<2, 10>(lin, q123, k, left);
   }
-  // stage 4: c_i = D mix - (z - zeta) q123, lanes 0..10
-  {
-    cx v = mk(0.0, 0.0);
-    if (ln < 11) {
-      const dcx lin[2] = {dc_of(cneg(zeta)), dc_of(mk(1.0, 0.0))};
-      const dcx left = poly_coef(lin, 2, q123, 10, ln);
-      const dcx right = ln < 10 ? poly_coef(pb.D, 4, mix, 7, ln) : dc_zero();
-      v = dc_round(dc_sub(right, left));
-    }
-    for (int l = 0; l < 11; l++) c[l] = shfl_cx(v, l);
-  }
+  dcx right = dc_zero();
+  dp_lane<4, 7>(Dv, mix, k, right);
+  if (k >= 10) right = dc_zero();
+  return k <= 10 ? dc_round(dc_sub(right, left)) : mk(0.0, 0.0);
 }
 
-// Laguerre::laguerre (laguerre.cc:18-99) with every lane holding the same x:
-// the Horner recurrence runs on all lanes, lane j keeps b_j, the moduli
-// |b_j| are taken in parallel and the error bound is accumulated in the
-// reference's order from the broadcast values.
-__device__ bool wave_lag_iter(const cx* poly, int m, cx* xp, const TrigTable* trig) {
-  const double frac[8] = {0.5, 0.23, 0.73, 0.13, 0.37, 0.61, 0.89, 1.0};
+// Laguerre::laguerre (laguerre.cc:18-99) for an x held by every lane; the
+// polynomial's coefficient j is lane j of pv.  The Horner recurrence runs on
+// all lanes, lane j keeps b_j, the moduli |b_j| are taken in parallel and the
+// error bound is accumulated in the reference's order.
+__device__ __forceinline__ bool wl_lag(const cx& pv, int m, cx* xp, const TrigTable* trig) {
   const int ln = __lane_id();
   cx x = *xp;
   const double md = (double)m;
-  if (m == 1) return lag_iter(poly, m, xp, trig);
+  if (m == 1) {
+    const cx p0 = rlc(pv, 0), p1 = rlc(pv, 1);
+    if (cabs_sel(p1) != 0.0) {
+      *xp = cdiv_sel(cneg(p0), p1);
+      return true;
+    }
+    *xp = mk(999.0, 999.0);
+    return false;
+  }
   for (int i = 1; i <= kLagMaxIt; i++) {
-    cx b = poly[m];
-    const double err0 = cabs_(b);
+    cx b = rlc(pv, m);
+    const double err0 = cabs_sel(b);
+    const double ax = cabs_sel(x);
     cx s = mk(0.0, 0.0), f = mk(0.0, 0.0), mine = mk(0.0, 0.0);
     for (int j = m - 1; j >= 0; j--) {
       s = cadd(cmul(x, s), f);
       f = cadd(cmul(x, f), b);
-      b = cadd(cmul(x, b), poly[j]);
+      b = cadd(cmul(x, b), rlc(pv, j));
       if (ln == j) mine = b;
     }
-    const double mabs = cabs_(mine);
-    const double ax = cabs_(x);
+    const double mabs = cabs_sel(mine);
     double err = err0;
-    for (int j = m - 1; j >= 0; j--) err = __shfl(mabs, j, 64) + ax * err;
+    for (int j = m - 1; j >= 0; j--) err = rl(mabs, j) + ax * err;
     err *= kLagEps;
-    const double babs = __shfl(mabs, 0, 64);  // |b_0|, the polynomial value
-    if (babs <= err) { *xp = x; return true; }
-    cx g = cdiv(f, b);
-    cx h = csub(cmul(g, g), cdiv(cscale(s, 2.0), b));
+    if (rl(mabs, 0) <= err) { *xp = x; return true; }
+    cx g = cdiv_sel(f, b);
+    cx h = csub(cmul(g, g), cdiv_sel(cscale(s, 2.0), b));
     cx rad = cscale(csub(cscale(h, md), cmul(g, g)), md - 1.0);
-    cx sq = csqrt_(rad);
+    cx sq = csqrt_sel(rad);
     cx gp = cadd(g, sq);
     cx gm = csub(g, sq);
-    if (cabs_(gp) < cabs_(gm)) gp = gm;
+    if (cabs_sel(gp) < cabs_sel(gm)) gp = gm;
     cx dx;
-    if (cabs_(gp) > 0.0) dx = cdiv_rc(md, gp);
-    else dx = cscale(mk(trig->c[i], trig->s[i]), 1.0 + cabs_(x));
+    if (cabs_sel(gp) > 0.0) dx = cdiv_rc_sel(md, gp);
+    else dx = cscale(mk(trig->c[i], trig->s[i]), 1.0 + cabs_sel(x));
     cx x1 = csub(x, dx);
     if (x.re == x1.re && x.im == x1.im) { *xp = x; return true; }
     if (i % kLagMT) x = x1;
-    else x = csub(x, cmul(mk(frac[(i / kLagMT) & 7], 0.0), dx));
+    else x = csub(x, cmul(mk(kLagFrac[(i / kLagMT) & 7], 0.0), dx));
   }
   *xp = x;
   return false;
 }
 
-// Laguerre::solveRoots for the degree-10 image polynomial: deflation on all
-// lanes, then lane k polishes root k.
-__device__ void wave_solve10(const cx* c, const cx* init, cx* roots, const TrigTable* trig) {
-  cx tmp[11];
-  for (int i = 0; i <= 10; i++) tmp[i] = c[i];
-  int nr = 0;
+// Laguerre::laguerre with a different x on every lane (the polishing step):
+// the coefficients are the lanes of pv, read in the same order by all lanes.
+__device__ __forceinline__ bool wl_lag_lanes(const cx& pv, int m, cx* xp,
+                                             const TrigTable* trig) {
+  cx x = *xp;
+  const double md = (double)m;
+  for (int i = 1; i <= kLagMaxIt; i++) {
+    cx b = rlc(pv, m);
+    double err = cabs_sel(b);
+    const double ax = cabs_sel(x);
+    cx s = mk(0.0, 0.0), f = mk(0.0, 0.0);
+    for (int j = m - 1; j >= 0; j--) {
+      s = cadd(cmul(x, s), f);
+      f = cadd(cmul(x, f), b);
+      b = cadd(cmul(x, b), rlc(pv, j));
+      err = cabs_sel(b) + ax * err;
+    }
+    err *= kLagEps;
+    if (cabs_sel(b) <= err) { *xp = x; return true; }
+    cx g = cdiv_sel(f, b);
+    cx h = csub(cmul(g, g), cdiv_sel(cscale(s, 2.0), b));
+    cx rad = cscale(csub(cscale(h, md), cmul(g, g)), md - 1.0);
+    cx sq = csqrt_sel(rad);
+    cx gp = cadd(g, sq);
+    cx gm = csub(g, sq);
+    if (cabs_sel(gp) < cabs_sel(gm)) gp = gm;
+    cx dx;
+    if (cabs_sel(gp) > 0.0) dx = cdiv_rc_sel(md, gp);
+    else dx = cscale(mk(trig->c[i], trig->s[i]), 1.0 + cabs_sel(x));
+    cx x1 = csub(x, dx);
+    if (x.re == x1.re && x.im == x1.im) { *xp = x; return true; }
+    if (i % kLagMT) x = x1;
+    else x = csub(x, cmul(mk(kLagFrac[(i / kLagMT) & 7], 0.0), dx));
+  }
+  *xp = x;
+  return false;
+}
+
+// Laguerre::solveRoots for the degree-10 image polynomial (coefficients on
+// lanes 0..10 of cv): deflation with every lane holding the step's root, the
+// deflated coefficients on the lanes of tmp; then lane r polishes root r on
+// the full polynomial.  Returns root r on lane r (r < 10; 0 elsewhere).
+// With init, initv's lane j seeds the search of step j (j = 10..1).
+__device__ __forceinline__ cx wl_solve10(const cx& cv, bool init, const cx& initv,
+                                         const TrigTable* trig) {
+  const int ln = __lane_id();
+  cx tmp = cv;
+  cx rootv = mk(0.0, 0.0);
   for (int j = 10; j >= 1; j--) {
-    cx x = init ? init[j] : mk(kLagEps, kLagEps);
-    if (!wave_lag_iter(tmp, j, &x, trig) && init) x = mk(kLagEps, kLagEps);
-    roots[nr++] = x;
-    cx b = tmp[j];
+    cx x = init ? rlc(initv, j) : mk(kLagEps, kLagEps);
+    if (!wl_lag(tmp, j, &x, trig) && init) x = mk(kLagEps, kLagEps);
+    if (ln == 10 - j) rootv = x;
+    cx b = rlc(tmp, j);
     for (int jj = j - 1; jj >= 0; jj--) {
-      cx cc = tmp[jj];
-      tmp[jj] = b;
-      b = cadd(cmul(x, b), cc);
+      const cx c = rlc(tmp, jj);
+      if (ln == jj) tmp = b;
+      b = cadd(cmul(x, b), c);
     }
   }
-  const int ln = __lane_id();
-  cx mine = roots[ln < 10 ? ln : 0];
-  if (ln < 10) lag_iter(c, 10, &mine, trig);
-  for (int k = 0; k < 10; k++) roots[k] = shfl_cx(mine, k);
+  if (ln < 10) wl_lag_lanes(cv, 10, &rootv, trig);
+  return rootv;
+}
+
+// Laguerre::checkRoots (laguerre.cc:188-198) on roots held one per lane.
+__device__ __forceinline__ bool wl_check(const cx& cv, const cx& rootv) {
+  cx c0 = rlc(cv, 10);
+  for (int j = 0; j < 10; j++) c0 = cmul(c0, cneg(rlc(rootv, j)));
+  const cx a0 = rlc(cv, 0);
+  if (cabs_sel(a0) > 1.0e-8) return (1.0 - cabs_sel(cdiv_sel(c0, a0))) < 1.0e-5;
+  return cabs_sel(csub(c0, a0)) < 1.0e-10;
 }
 
 // getRootsPrecalculated + getImages (imgpoint.cc:160-274) on a wavefront.
-__device__ int wave_get_images(const Lens& L, const PolyBase& pb, RootChain* rc, cx zeta,
-                               cx* imgs, const TrigTable* trig) {
-  cx c[11];
-  wave_lens_poly(L, pb, zeta, c);
+// chainv holds tempRoots[k] on lane k (lane 10 is the slot read past the end
+// of the reference's vector and stays 0).  Returns the mask of the roots
+// accepted as images (bit k: root k on lane k of chainv).
+__device__ __forceinline__ unsigned long long wl_get_images(const Lens& L, const dcx& Dv,
+                                                            const dcx& Sv, cx& chainv, int& have,
+                                                            cx zeta, const TrigTable* trig) {
+  const cx cv = wl_poly(L, Dv, Sv, zeta);
   const int ln = __lane_id();
-  for (int attempt = 0; attempt < 2; attempt++) {
-    if (!rc->have || attempt == 1) {
-      wave_solve10(c, nullptr, rc->tempRoots, trig);
-      rc->have = 1;
-    } else {
-      cx out[10];
-      wave_solve10(c, rc->tempRoots, out, trig);
-      if (!lag_check<10>(c, out)) wave_solve10(c, nullptr, out, trig);
-      for (int k = 0; k < 10; k++) rc->tempRoots[k] = out[k];
+  // The reference's sequence (imgpoint.cc:181-206,262-270): a solve from the
+  // chain's roots (when it has some), replaced by a fresh solve when the
+  // roots fail checkRoots; then, if the image count is invalid, one more
+  // fresh solve.  A fresh solve of the same coefficients gives the same
+  // roots, so the second one is skipped when the first solve was fresh.
+  bool withInit = have != 0;
+  bool second = false;
+  while (true) {
+    const cx r = wl_solve10(cv, withInit, chainv, trig);
+    if (withInit && !wl_check(cv, r)) {
+      withInit = false;
+      continue;
     }
-    rc->tempRoots[10] = mk(0.0, 0.0);
-    const bool ok = ln < 10 && img_check(L, rc->tempRoots[ln < 10 ? ln : 0], zeta, 1.0e-6);
-    unsigned long long m = __ballot(ok);
-    int n = 0;
-    while (m) {
-      const int k = __builtin_ctzll(m);
-      m &= m - 1;
-      imgs[n++] = rc->tempRoots[k];
-    }
-    if (!(n % 2 != 0 || n < 4)) return n;
-    if (attempt == 1) return n;
+    chainv = ln < 10 ? r : mk(0.0, 0.0);
+    have = 1;
+    const bool ok = ln < 10 && img_check(L, chainv, zeta, 1.0e-6);
+    const unsigned long long m = __ballot(ok);
+    const int n = __builtin_popcountll(m);
+    if (!(n % 2 != 0 || n < 4) || second || !withInit) return m;
+    second = true;
+    withInit = false;
   }
-  return 0;
+}
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Seed pixels of the images in `m` (root order), appended at out[cnt..].
+__device__ __forceinline__ void emit_seeds(const CurveDev& C, const cx& chainv,
+                                           unsigned long long m, int2* out, int& cnt,
+                                           uint32_t& fl) {
+  const int ln = __lane_id();
+  const int n = __builtin_popcountll(m);
+  if ((m >> ln) & 1ull) {
+    const int slot = cnt + __builtin_popcountll(m & ((1ull << ln) - 1ull));
+    if (slot < kMaxSeeds)
+      out[slot] = make_int2(to_pixel(chainv.re, C.blx, C.Nd, C.Ldbl),
+                            to_pixel(chainv.im, C.bly, C.Nd, C.Ldbl));
+  }
+  if (cnt + n > kMaxSeeds) fl |= kFlagSeedOverflow;
+  cnt = cnt + n > kMaxSeeds ? kMaxSeeds : cnt + n;
 }
 
 __global__ void __launch_bounds__(64) seed_kernel(SeedArgs a) {
@@ -296,66 +397,82 @@ __global__ void __launch_bounds__(64) seed_kernel(SeedArgs a) {
   const int ln = __lane_id();
   const CurveDev& C = a.curves[c];
   const Lens L = C.L;
-  RootChain rc;
-  for (int k = 0; k < 11; k++) rc.tempRoots[k] = C.chain[k];
-  rc.have = C.chainHave;
-  cx imgs[10];
+  const dcx Dv = ln < 4 ? C.pb.D[ln < 4 ? ln : 0] : dc_zero();
+  const dcx Sv = ln < 3 ? C.pb.S[ln < 3 ? ln : 0] : dc_zero();
+  cx chainv = mk(0.0, 0.0);
+  int have = 0;
+  if (a.chainIn) {
+    if (ln <= 10) chainv = a.chainIn[c].tempRoots[ln <= 10 ? ln : 0];
+    have = a.chainIn[c].have;
+  } else {
+    if (ln <= 10) chainv = C.chain[ln <= 10 ? ln : 0];
+    have = C.chainHave;
+  }
   const cx s = mk(C.sx, C.sy), e = mk(C.ex, C.ey);
   const cx* caus = a.caustics + C.causticBase;
-  for (unsigned i = 0; i <= C.lcLength; i++) {
+  const long long n = (long long)C.lcLength + 1;
+  const unsigned iBegin = (unsigned)(n * a.chunk / a.nChunks);
+  const unsigned iEnd = (unsigned)(n * (a.chunk + 1) / a.nChunks);
+  for (unsigned i = iBegin; i < iEnd; i++) {
     const int p = C.pointBase + (int)i;
     const cx pos = traj_pos(s, e, i, C.lcLength);
     if (ln == 0) a.ptPos[p] = make_double2(pos.re, pos.im);
     int cnt = 0;
     uint32_t fl = 0;
     int2* out = a.seeds + (size_t)p * kMaxSeeds;
-    const int n = wave_get_images(L, C.pb, &rc, pos, imgs, a.trig);
     double amp = 0.0;
-    for (int k = 0; k < n; k++) {
-      amp += point_amp_term(L, imgs[k]);
-      if (cnt < kMaxSeeds) {
-        if (ln == 0)
-          out[cnt] = make_int2(to_pixel(imgs[k].re, C.blx, C.Nd, C.Ldbl),
-                               to_pixel(imgs[k].im, C.bly, C.Nd, C.Ldbl));
-        cnt++;
+    // The source positions solved for this point, in the reference's order
+    // (lcirs.cc:131-165): the centre, then on each caustic branch whose
+    // cumulative box meets the source square the first segment of every run
+    // of consecutive segments that meet the source (64 segments at a time).
+    int r = -1;              // caustic branch being scanned (-1: centre next)
+    bool findBranch = true;  // r has to advance to a branch whose box meets
+    int k0 = 0;              // next chunk of branch r
+    bool prevHit = false;    // hit of the segment before that chunk
+    unsigned long long f = 0;  // first-hit segments of the chunk not yet solved
+    cx trial = mk(0.0, 0.0);   // this lane's segment's trial position
+    const cx* cr = caus;
+    while (true) {
+      cx zeta;
+      const bool centre = r < 0;
+      if (centre) {
+        zeta = pos;
+        r = 0;
       } else {
-        fl |= kFlagSeedOverflow;
-      }
-    }
-    for (int r = 0; r < 6; r++) {
-      if (!(pos.re + C.rho >= C.boxMin[r].re && pos.re - C.rho <= C.boxMax[r].re &&
-            pos.im + C.rho >= C.boxMin[r].im && pos.im - C.rho <= C.boxMax[r].im))
-        continue;
-      // segments k..k+1 of this caustic branch, 64 at a time; only the first
-      // segment of every run of consecutive hits solves for images
-      const cx* cr = caus + r * kCccLen;
-      bool prevHit = false;  // hit of the segment before this chunk
-      for (int k0 = 0; k0 < kCccLen - 1; k0 += 64) {
-        const int k = k0 + ln;
-        cx trial = mk(0.0, 0.0);
-        const bool hit = k < kCccLen - 1 && intersection_check(pos, cr[k], cr[k + 1], C.rho,
-                                                               C.rho2pow, &trial);
-        const unsigned long long m = __ballot(hit);
-        const unsigned long long firsts = m & ~((m << 1) | (prevHit ? 1ull : 0ull));
-        unsigned long long f = firsts;
-        while (f) {
-          const int j = __builtin_ctzll(f);
-          f &= f - 1;
-          const cx tp = shfl_cx(trial, j);
-          const int n2 = wave_get_images(L, C.pb, &rc, tp, imgs, a.trig);
-          for (int q = 0; q < n2; q++) {
-            if (cnt < kMaxSeeds) {
-              if (ln == 0)
-                out[cnt] = make_int2(to_pixel(imgs[q].re, C.blx, C.Nd, C.Ldbl),
-                                     to_pixel(imgs[q].im, C.bly, C.Nd, C.Ldbl));
-              cnt++;
-            } else {
-              fl |= kFlagSeedOverflow;
-            }
+        while (f == 0) {
+          if (findBranch) {
+            while (r < 6 && !(pos.re + C.rho >= C.boxMin[r].re && pos.re - C.rho <= C.boxMax[r].re &&
+                              pos.im + C.rho >= C.boxMin[r].im && pos.im - C.rho <= C.boxMax[r].im))
+              r++;
+            if (r >= 6) break;
+            cr = caus + r * kCccLen;
+            k0 = 0;
+            prevHit = false;
+            findBranch = false;
           }
+          if (k0 >= kCccLen - 1) {
+            r++;
+            findBranch = true;
+            continue;
+          }
+          const int k = k0 + ln;
+          trial = mk(0.0, 0.0);
+          const bool hit = k < kCccLen - 1 && intersection_check(pos, cr[k], cr[k + 1], C.rho,
+                                                                 C.rho2pow, &trial);
+          const unsigned long long hm = __ballot(hit);
+          f = hm & ~((hm << 1) | (prevHit ? 1ull : 0ull));
+          prevHit = (hm >> 63) & 1ull;
+          k0 += 64;
         }
-        prevHit = (m >> 63) & 1ull;
+        if (f == 0) break;
+        const int j = __builtin_ctzll(f);
+        f &= f - 1;
+        zeta = rlc(trial, j);
       }
+      const unsigned long long m = wl_get_images(L, Dv, Sv, chainv, have, zeta, a.trig);
+      emit_seeds(C, chainv, m, out, cnt, fl);
+      // predicted image area (point magnification) for the fill's ordering
+      if (centre) amp = wave_sum_d(((m >> ln) & 1ull) ? point_amp_term(L, chainv) : 0.0);
     }
     if (ln == 0) {
       a.nSeeds[p] = cnt;
@@ -363,31 +480,41 @@ __global__ void __launch_bounds__(64) seed_kernel(SeedArgs a) {
       a.cost[p] = (float)(amp * 3.141592653589793 * C.ppr * C.ppr);
     }
   }
-  if (ln == 0) a.chainOut[c] = rc;
+  if (ln <= 10) a.chainOut[c].tempRoots[ln] = chainv;
+  if (ln == 0) a.chainOut[c].have = have;
 }
 
 // LightCurveBase::getLC / getPointAmp (lcbase.cc:17-52): positions at
-// t = i/(lcLength-1), magnification sum over images of 1/|1 - |sum m/(z-z_k)^2|^2|
-// (the squares as products; the reference uses std::pow(z, 2.0)).
+// t = i/(lcLength-1), magnification sum over images (in root order) of
+// 1/|1 - |sum m/(z-z_k)^2|^2| (the squares as products; the reference uses
+// std::pow(z, 2.0)).  One wavefront per curve, as the seed search.
 __global__ void __launch_bounds__(64) point_lc_kernel(const CurveDev* curves, int nCurves,
                                                       const TrigTable* trig, double* out,
                                                       RootChain* chainOut) {
   const int c = blockIdx.x;
-  if (c >= nCurves || threadIdx.x != 0) return;
+  if (c >= nCurves) return;
+  const int ln = __lane_id();
   const CurveDev& C = curves[c];
-  RootChain rc;
-  for (int k = 0; k < 11; k++) rc.tempRoots[k] = C.chain[k];
-  rc.have = C.chainHave;
-  cx imgs[10];
+  const Lens L = C.L;
+  const dcx Dv = ln < 4 ? C.pb.D[ln < 4 ? ln : 0] : dc_zero();
+  const dcx Sv = ln < 3 ? C.pb.S[ln < 3 ? ln : 0] : dc_zero();
+  cx chainv = ln <= 10 ? C.chain[ln <= 10 ? ln : 0] : mk(0.0, 0.0);
+  int have = C.chainHave;
   const cx s = mk(C.sx, C.sy), e = mk(C.ex, C.ey);
   for (unsigned i = 0; i < C.lcLength; i++) {
     const cx pos = cadd(s, cscale(csub(e, s), i / (C.lcLength - 1.0)));
-    const int n = get_images(C.L, C.pb, &rc, pos, imgs, trig);
+    unsigned long long m = wl_get_images(L, Dv, Sv, chainv, have, pos, trig);
+    const double term = ((m >> ln) & 1ull) ? point_amp_term(L, chainv) : 0.0;
     double amp = 0.0;
-    for (int k = 0; k < n; k++) amp += point_amp_term(C.L, imgs[k]);
-    out[C.pointBase + (int)i] = amp;
+    while (m) {
+      const int k = __builtin_ctzll(m);
+      m &= m - 1;
+      amp += rl(term, k);
+    }
+    if (ln == 0) out[C.pointBase + (int)i] = amp;
   }
-  chainOut[c] = rc;
+  if (ln <= 10) chainOut[c].tempRoots[ln] = chainv;
+  if (ln == 0) chainOut[c].have = have;
 }
 
 int launch_point_lc(const CurveDev* curves, int nCurves, const TrigTable* trig, double* out,
@@ -395,6 +522,18 @@ int launch_point_lc(const CurveDev* curves, int nCurves, const TrigTable* trig, 
   if (nCurves <= 0) return 0;
   hipLaunchKernelGGL(point_lc_kernel, dim3(nCurves), dim3(64), 0, s, curves, nCurves, trig, out,
                      chainOut);
+  return (int)hipGetLastError();
+}
+
+__global__ void clear_flag_bits_kernel(uint32_t* flags, const int* pts, int n, uint32_t bits) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flags[pts[i]] &= ~bits;
+}
+
+int launch_clear_flag_bits(uint32_t* flags, const int* pts, int n, uint32_t bits, hipStream_t s) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(clear_flag_bits_kernel, dim3((n + 255) / 256), dim3(256), 0, s, flags, pts, n,
+                     bits);
   return (int)hipGetLastError();
 }
 
@@ -584,17 +723,8 @@ __device__ __forceinline__ void flag_point(const FillArgs& A, int gp, uint32_t f
   if (__lane_id() == 0) atomicOr(&A.flags[gp], f);
 }
 
-// Queues a task.  The pending count is raised, and that increment completed,
-// before the task becomes visible to other waves.
-__device__ __forceinline__ void push_task(const FillArgs& A, Task f, int gp) {
-  if (__lane_id() == 0) {
-    afaddl(A.pending, 1);
-    mem_order();
-  }
-  if (!q_push(A, f)) {
-    flag_point(A, gp, kFlagQueueFull);
-    if (__lane_id() == 0) afaddl(A.pending, -1);
-  }
+__device__ __forceinline__ void afaddi(int* p, int v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, AGENT);
 }
 
 // ---------------------------------------------------------------------------
@@ -686,7 +816,37 @@ struct TaskState {
   long long accSum;         // the label changes or the sweep ends
   unsigned long long accCnt;
   unsigned long long evals, nIns, nDup, insTicks;
+  // this wave's ring of waiting tasks in LDS: lcount tasks from lhead on
+  Task* ring;
+  int lhead, lcount;
 };
+
+// Task hand-off.  Follow-ups a wave does not continue with go to its own ring
+// in LDS (no atomics: one owner); the newest is taken next (depth first).
+// Waves that run dry take tasks from the shared queue, and waves with a
+// backlog move their oldest tasks there while some wave is idle.
+__device__ __forceinline__ bool ring_donate(TaskState& S) {
+  const Task t = S.ring[S.lhead];
+  if (!q_push(*S.A, t)) return false;
+  S.lhead = (S.lhead + 1) & (kLocalCap - 1);
+  S.lcount--;
+  return true;
+}
+
+__device__ __forceinline__ void ring_push(TaskState& S, const Task& f) {
+  if (S.lcount == kLocalCap && !ring_donate(S)) {
+    flag_point(*S.A, S.gp, kFlagQueueFull);  // both full: the task is lost
+    return;
+  }
+  const int slot = (S.lhead + S.lcount) & (kLocalCap - 1);
+  if (__lane_id() == 0) S.ring[slot] = f;
+  S.lcount++;
+}
+
+__device__ __forceinline__ Task ring_pop(TaskState& S) {
+  S.lcount--;
+  return S.ring[(S.lhead + S.lcount) & (kLocalCap - 1)];
+}
 
 __device__ __forceinline__ void flush_acc(TaskState& S) {
   if (S.accCnt && __lane_id() == 0) {
@@ -838,7 +998,7 @@ __device__ __forceinline__ void handle_run(TaskState& S, int xl, int xr, long lo
       S.cont = f;
       S.haveCont = true;
     } else {
-      push_task(A, f, S.gp);
+      ring_push(S, f);
     }
   }
 }
@@ -1016,7 +1176,13 @@ __device__ __forceinline__ void process_task(TaskState& S) {
   S.nBack = 0;
 }
 
-__global__ void __launch_bounds__(256) fill_kernel(FillArgs A) {
+// Work comes from the seed list, then from this wave's ring, then from the
+// shared queue.  A wave that finds none of them leaves the active count and
+// polls; it rejoins before taking a task from the queue, so a task is never
+// queued while the count reads 0.  The fill is over when the count is 0 and
+// the queue is empty.
+__global__ void __launch_bounds__(kFillThreads) fill_kernel(FillArgs A) {
+  __shared__ Task rings[kFillThreads / 64][kLocalCap];
   const int ln = __lane_id();
   const long long t0 = (long long)__builtin_amdgcn_s_memrealtime();
   TaskState S;
@@ -1029,10 +1195,16 @@ __global__ void __launch_bounds__(256) fill_kernel(FillArgs A) {
   S.accLabel = -1;
   S.accSum = 0;
   S.accCnt = 0;
+  S.ring = rings[threadIdx.x / 64];
+  S.lhead = 0;
+  S.lcount = 0;
   unsigned long long nTasks = 0, busyTicks = 0, idlePolls = 0;
   bool have = false;
-  int backoff = 0;
   while (true) {
+    if (!have && S.lcount > 0) {
+      S.t = ring_pop(S);
+      have = true;
+    }
     if (!have) {
       if (aldi(A.abortFlag)) break;
       int st = A.nSeedTasks;
@@ -1050,36 +1222,79 @@ __global__ void __launch_bounds__(256) fill_kernel(FillArgs A) {
       } else if (q_pop(A, S.t)) {
         have = true;
       } else {
-        long long pend = 0;
-        if (ln == 0) pend = (long long)ald((unsigned long long*)A.pending);
-        pend = bcast0(pend);
-        if (pend <= 0) break;
-        const long long now = (long long)__builtin_amdgcn_s_memrealtime();
-        if (now - t0 > A.timeLimitTicks) {
-          if (ln == 0) atomicExch(A.abortFlag, 1);
-          break;
+        // idle: leave the active count, then poll
+        flush_acc(S);
+        if (ln == 0) {
+          mem_order();
+          afaddi(A.active, -1);
         }
-        // idle: back off so that polling does not crowd the shared counters
-        idlePolls++;
-        if (backoff < 6) backoff++;
-        for (int k = 0; k < (1 << backoff); k++) __builtin_amdgcn_s_sleep(32);
-        continue;
+        bool done = false;
+        int backoff = 0;
+        while (!have) {
+          // poll at most every ~30 us once idle for a while, reading the
+          // active count and the abort flag only every fourth time
+          idlePolls++;
+          if (backoff < 6) backoff++;
+          for (int k = 0; k < (1 << backoff); k++) __builtin_amdgcn_s_sleep(16);
+          const bool full = (idlePolls & 3) == 0;
+          int act = 1, abortNow = 0;
+          unsigned long long qh = 0, qt = 0;
+          if (ln == 0) {
+            if (full) {
+              act = aldi(A.active);
+              abortNow = aldi(A.abortFlag);
+              mem_order();
+            }
+            qh = ald(A.qhead);
+            qt = ald(A.qtail);
+          }
+          act = bcast0(act);
+          qh = bcast0(qh);
+          qt = bcast0(qt);
+          if (bcast0(abortNow)) { done = true; break; }
+          if ((long long)__builtin_amdgcn_s_memrealtime() - t0 > A.timeLimitTicks) {
+            if (ln == 0) atomicExch(A.abortFlag, 1);
+            done = true;
+            break;
+          }
+          if (qt > qh) {
+            // rejoin before taking the task
+            if (ln == 0) {
+              afaddi(A.active, 1);
+              mem_order();
+            }
+            if (q_pop(A, S.t)) {
+              have = true;
+            } else if (ln == 0) {
+              afaddi(A.active, -1);
+            }
+          } else if (act <= 0) {
+            done = true;  // no wave holds work and nothing is queued
+            break;
+          }
+        }
+        if (done) break;
       }
     }
-    backoff = 0;
     nTasks++;
     const unsigned long long tb0 = __builtin_amdgcn_s_memrealtime();
     process_task(S);
     busyTicks += __builtin_amdgcn_s_memrealtime() - tb0;
     if (S.haveCont) {
-      S.t = S.cont;  // the kept follow-up inherits this task's pending unit
+      S.t = S.cont;
     } else {
       flush_acc(S);
-      if (ln == 0) {
-        mem_order();
-        afaddl(A.pending, -1);
-      }
       have = false;
+    }
+    if ((nTasks & 7) == 0 && S.lcount > 0) {
+      // hand the oldest waiting tasks to idle waves
+      int act = 0;
+      if (ln == 0) act = aldi(A.active);
+      act = bcast0(act);
+      int give = A.nWaves - act;
+      if (give > 4) give = 4;
+      for (int k = 0; k < give && S.lcount > 0; k++)
+        if (!ring_donate(S)) break;
     }
     if ((nTasks & 255) == 0) {
       const long long now = (long long)__builtin_amdgcn_s_memrealtime();
@@ -1102,8 +1317,21 @@ __global__ void __launch_bounds__(256) fill_kernel(FillArgs A) {
   }
 }
 
+// Workgroups of fill_kernel that can be resident on the device at once.  The
+// fill's end test counts every wave of the launch as working until it has
+// started and found nothing to do, so the launch must not exceed this.
+int fill_resident_workgroups() {
+  int perCu = 0, dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, fill_kernel, kFillThreads, 0) !=
+      hipSuccess)
+    return 0;
+  return perCu * cus;
+}
+
 int launch_fill(const FillArgs& a, int nWorkgroups, hipStream_t s) {
-  hipLaunchKernelGGL(fill_kernel, dim3(nWorkgroups), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(fill_kernel, dim3(nWorkgroups), dim3(kFillThreads), 0, s, a);
   return (int)hipGetLastError();
 }
 
@@ -1142,7 +1370,7 @@ __global__ void fill_reset_kernel(FillArgs A) {
   if (tid == 0) {
     *A.qhead = 0;
     *A.qtail = 0;
-    *A.pending = A.nSeedTasks;
+    *A.active = A.nWaves;
     *A.seedNext = 0;
     *A.labelNext = A.nBatch;
     *A.abortFlag = 0;

@@ -101,6 +101,44 @@ TL_HD double hyp(double x, double y) {
 }
 TL_HD double cabs_(cx a) { return hyp(a.re, a.im); }
 
+// Branch-free forms of cdiv, hyp and csqrt_ for operands in the normal range,
+// with the general routines for everything else.  Each computes the values
+// of both sides of the general routine's test and selects one, so the result
+// is the general routine's to the bit; a wavefront then runs no divergent
+// branches and independent chains can overlap.
+TL_HD cx cdiv_sel(cx n, cx d) {
+  const double a = n.re, b = n.im, c = d.re, e = d.im;
+  const bool ce = fabs(c) < fabs(e);
+  const double num = ce ? c : e, den = ce ? e : c;
+  const double ratio = num / den;
+  const double denom = (num * ratio) + den;
+  const double ar = a * ratio, br = b * ratio;
+  const double xn = ce ? ar + b : br + a;
+  const double yn = ce ? br - a : b - ar;
+  return mk(xn / denom, yn / denom);
+}
+
+TL_HD double hyp_sel(double x, double y) {
+  const double LARGE = 0x1p+511, TINY = 0x1p-511, EPSH = 0x1p-54;
+  const double fx = fabs(x), fy = fabs(y);
+  const double ax = fx < fy ? fy : fx;
+  const double ay = fx < fy ? fx : fy;
+  if (!(isfinite(x) && isfinite(y) && !(ax > LARGE) && !(ay < TINY) && !(ay <= ax * EPSH)))
+    return hyp(x, y);
+  double h = sqrt(ax * ax + ay * ay);
+  const double d1 = h - ay;
+  const double t1a = ax * (2.0 * d1 - ax);
+  const double t2a = (d1 - 2.0 * (ax - ay)) * d1;
+  const double d2 = h - ax;
+  const double t1b = 2.0 * d2 * (ax - 2.0 * ay);
+  const double t2b = (4.0 * d2 - ay) * ay + d2 * d2;
+  const bool lo = h <= 2.0 * ay;
+  const double t1 = lo ? t1a : t1b, t2 = lo ? t2a : t2b;
+  h -= (t1 + t2) / (2.0 * h);
+  return h;
+}
+TL_HD double cabs_sel(cx a) { return hyp_sel(a.re, a.im); }
+
 // Complex square root, glibc csqrt algorithm for finite operands.
 TL_HD cx csqrt_(cx z) {
   double re = z.re, im = z.im;
@@ -153,6 +191,23 @@ TL_HD cx csqrt_(cx z) {
   }
   return mk(r, copysign(s, im));
 }
+
+TL_HD cx csqrt_sel(cx z) {
+  const double re = z.re, im = z.im;
+  const double QMAX = DBL_MAX / 4.0, TWOMIN = 2.0 * DBL_MIN;
+  if (!(im != 0.0 && re != 0.0 && fabs(re) <= QMAX && fabs(im) <= QMAX &&
+        !(fabs(re) < TWOMIN && fabs(im) < TWOMIN)))
+    return csqrt_(z);
+  const double d = hyp_sel(re, im);
+  // d + re for re > 0, d - re = d + |re| otherwise: the root of the larger one
+  const double q = sqrt(0.5 * (d + fabs(re)));
+  const double w = 0.5 * (im / q);
+  const bool pos = re > 0.0;
+  const double r = pos ? q : fabs(w);
+  const double s = pos ? w : q;
+  return mk(r, copysign(s, im));
+}
+TL_HD cx cdiv_rc_sel(double t, cx d) { return cdiv_sel(mk(t, 0.0), d); }
 
 // ---------------------------------------------------------------------------
 // Laguerre (laguerre.cc)
