@@ -9,9 +9,12 @@ rho = 5e-4, pixel rho/200, 4096 positions on a straight line of length 2
 through the central caustic.  The printed value is light-curve points per
 second.
 
-The dominant kernel's roofline figures are measured in the same run: the
-fill kernel's duration comes from HIP events around it on the engine's
-stream, and its algorithmic work is 43 FP64 operations per counted ray.
+Roofline figures are those of the fill kernel, where the path's arithmetic
+is (43 FP64 operations per counted ray): its launch durations come from HIP
+events around it on the engine's stream in this run, and its HBM traffic
+per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary under
+profiles/.  The seed search (the serial root chain, one wavefront per curve)
+runs concurrently on a second stream; its time is reported beside it.
 
 The CPU baseline is the reference LightCurveIRS compiled from its own
 sources (oracle/_ref, pinned flags -O2 -ffp-contract=off -mavx) run over the
@@ -78,6 +81,19 @@ def _split(c, parts):
         out.append(((c.sx + (c.ex - c.sx) * t0, c.sy + (c.ey - c.sy) * t0,
                      c.sx + (c.ex - c.sx) * t1, c.sy + (c.ey - c.sy) * t1), i1 - i0))
     return out
+
+
+def measured_traffic():
+    """HBM bytes per fill launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (scaled by a plain copy kernel's readings), or None."""
+    path = os.path.join(HERE, "profiles", "fill_kernel_c2_traffic.txt")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(c, cores):
@@ -156,13 +172,14 @@ def main(argv=None):
     lc.set_limb_darkening("linear", c.u)
     for _ in range(args.warmup):
         lc.lc_irs(*c.traj())
-    fill_ms = launches = rays = 0.0
+    fill_ms = launches = rays = seed_ms = 0.0
     flags_all = 0
     t = time.perf_counter()
     for _ in range(args.steps):
         _, r, f = lc.lc_irs(*c.traj())
         st = E.last_stats()
         fill_ms += st["fill_ms"]
+        seed_ms += st["seed_ms"]
         launches += st["fill_launches"]
         rays += float(r.sum())
         flags_all |= int(np.bitwise_or.reduce(f))
@@ -196,6 +213,9 @@ def main(argv=None):
             "cus": cus,
             "rays_per_step": rays / args.steps,
             "status_flags": flags_all,
+            "seed_kernel_ms_per_step": seed_ms / args.steps,
+            "fill_kernel_ms_per_step": fill_ms / args.steps,
+            "fill_launches_per_step": launches / args.steps,
         },
         "roofline": {
             "bound": "mfma",
@@ -203,7 +223,7 @@ def main(argv=None):
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": measured_traffic(),
             "kernel": "fill_kernel",
             "kernel_ms": fill_launch_ms,
             "flop_per_ray": FLOP_PER_RAY,
