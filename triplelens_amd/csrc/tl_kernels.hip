@@ -213,10 +213,29 @@ __device__ __forceinline__ cx wl_poly(const Lens& L, const dcx& Dv, const dcx& S
   return k <= 10 ? dc_round(dc_sub(right, left)) : mk(0.0, 0.0);
 }
 
+// One part of the quotient n / d as cdiv_sel forms it (part 0: real, 1:
+// imaginary), for this lane's numerator and part: lanes that need the
+// quotients of several numerators over one denominator share one division
+// sequence.
+__device__ __forceinline__ double smith_part(cx n, cx d, int part) {
+  const double a = n.re, b = n.im, c = d.re, e = d.im;
+  const bool ce = fabs(c) < fabs(e);
+  const double num = ce ? c : e, den = ce ? e : c;
+  const double ratio = num / den;
+  const double denom = (num * ratio) + den;
+  const double ar = a * ratio, br = b * ratio;
+  const double xn = ce ? ar + b : br + a;
+  const double yn = ce ? br - a : b - ar;
+  return (part ? yn : xn) / denom;
+}
+
 // Laguerre::laguerre (laguerre.cc:18-99) for an x held by every lane; the
 // polynomial's coefficient j is lane j of pv.  The Horner recurrence runs on
-// all lanes, lane j keeps b_j, the moduli |b_j| are taken in parallel and the
-// error bound is accumulated in the reference's order.
+// all lanes and lane j keeps b_j; independent scalar steps of the reference
+// run on separate lanes in one instruction sequence: the moduli |b_j|,
+// |p_m| and |x|, the four parts of f/b and 2s/b, the moduli of the two
+// candidate denominators and the two parts of m/g.  Every value is the one
+// the serial code computes.
 __device__ __forceinline__ bool wl_lag(const cx& pv, int m, cx* xp, const TrigTable* trig) {
   const int ln = __lane_id();
   cx x = *xp;
@@ -231,31 +250,46 @@ __device__ __forceinline__ bool wl_lag(const cx& pv, int m, cx* xp, const TrigTa
     return false;
   }
   for (int i = 1; i <= kLagMaxIt; i++) {
-    cx b = rlc(pv, m);
-    const double err0 = cabs_sel(b);
-    const double ax = cabs_sel(x);
-    cx s = mk(0.0, 0.0), f = mk(0.0, 0.0), mine = mk(0.0, 0.0);
+    const cx pm = rlc(pv, m);
+    cx b = pm;
+    cx s = mk(0.0, 0.0), f = mk(0.0, 0.0);
+    cx mine = ln == m ? pm : (ln == m + 1 ? x : mk(0.0, 0.0));
     for (int j = m - 1; j >= 0; j--) {
       s = cadd(cmul(x, s), f);
       f = cadd(cmul(x, f), b);
       b = cadd(cmul(x, b), rlc(pv, j));
       if (ln == j) mine = b;
     }
+    // lane j < m: |b_j|; lane m: |p_m| (the error bound's start); lane m+1: |x|
     const double mabs = cabs_sel(mine);
-    double err = err0;
+    const double ax = rl(mabs, m + 1);
+    double err = rl(mabs, m);
     for (int j = m - 1; j >= 0; j--) err = rl(mabs, j) + ax * err;
     err *= kLagEps;
     if (rl(mabs, 0) <= err) { *xp = x; return true; }
-    cx g = cdiv_sel(f, b);
-    cx h = csub(cmul(g, g), cdiv_sel(cscale(s, 2.0), b));
-    cx rad = cscale(csub(cscale(h, md), cmul(g, g)), md - 1.0);
-    cx sq = csqrt_sel(rad);
+    // g = f / b (lanes 0, 1) and 2s / b (lanes 2, 3)
+    const double qv = smith_part(ln < 2 ? f : cscale(s, 2.0), b, ln & 1);
+    const cx g = mk(rl(qv, 0), rl(qv, 1));
+    const cx h = csub(cmul(g, g), mk(rl(qv, 2), rl(qv, 3)));
+    const cx rad = cscale(csub(cscale(h, md), cmul(g, g)), md - 1.0);
+    const cx sq = csqrt_sel(rad);
     cx gp = cadd(g, sq);
-    cx gm = csub(g, sq);
-    if (cabs_sel(gp) < cabs_sel(gm)) gp = gm;
+    const cx gm = csub(g, sq);
+    // |gp| (lane 0) and |gm| (lane 1)
+    const double av = cabs_sel(ln == 0 ? gp : gm);
+    double agp = rl(av, 0);
+    const double agm = rl(av, 1);
+    if (agp < agm) {
+      gp = gm;
+      agp = agm;
+    }
     cx dx;
-    if (cabs_sel(gp) > 0.0) dx = cdiv_rc_sel(md, gp);
-    else dx = cscale(mk(trig->c[i], trig->s[i]), 1.0 + cabs_sel(x));
+    if (agp > 0.0) {
+      const double dv = smith_part(mk(md, 0.0), gp, ln & 1);
+      dx = mk(rl(dv, 0), rl(dv, 1));
+    } else {
+      dx = cscale(mk(trig->c[i], trig->s[i]), 1.0 + ax);
+    }
     cx x1 = csub(x, dx);
     if (x.re == x1.re && x.im == x1.im) { *xp = x; return true; }
     if (i % kLagMT) x = x1;
