@@ -53,6 +53,18 @@ void set_limb_darkening(void* lc, const char* model, double v);
  * (lcirs.cc:479-512). */
 void set_amoeba_printout(void* lc, const char* amoebaPrefix, const char* parFile);
 
+/* ---- reference ABI of LightCurveCUDA (src/Cuda/cudalc.cc:486-542) -------- */
+
+/* lccuda_new (cudalc.cc:488-507): the same object as lcirs_new. */
+void* lccuda_new(double a, double b, double th, double m2, double m3, double sourceSize,
+                 unsigned lcLength, long pointsPerRadius);
+/* get_lc_cuda (cudalc.cc:510-520): the extended-source curve.  The
+ * reference's CUDA path is an FP32 approximation; here it is the exact FP64
+ * curve of get_lc_irs (lcLength+1 positions, copy_lc copies lcLength). */
+void get_lc_cuda(void* lc, double iniX, double iniY, double finX, double finY);
+/* set_limb_darkening_cuda (cudalc.cc:523-540): as set_limb_darkening. */
+void set_limb_darkening_cuda(void* lc, const char* model, double v);
+
 /* ---- additions (no counterpart in the reference) ------------------------ */
 
 /* Frees an object (the reference never frees them). */

@@ -819,6 +819,22 @@ void set_amoeba_printout(void* lc, const char* amoebaPrefix, const char* parFile
   });
 }
 
+// LightCurveCUDA's C ABI (src/Cuda/cudalc.cc:486-542), on this engine: the
+// same objects, and get_lc_cuda computes the exact FP64 extended-source curve
+// that get_lc_irs computes.
+void* lccuda_new(double a, double b, double th, double m2, double m3, double sourceSize,
+                 unsigned lcLength, long pointsPerRadius) {
+  return lcirs_new(a, b, th, m2, m3, sourceSize, lcLength, pointsPerRadius);
+}
+
+void get_lc_cuda(void* lc, double iniX, double iniY, double finX, double finY) {
+  get_lc_irs(lc, iniX, iniY, finX, finY);
+}
+
+void set_limb_darkening_cuda(void* lc, const char* model, double v) {
+  set_limb_darkening(lc, model, v);
+}
+
 int lcirs_copy_lc_full(void* lc, double* out) {
   const Curve* c = static_cast<Curve*>(lc);
   if (!c || !c->hasIRS) return -1;
