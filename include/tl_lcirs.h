@@ -75,6 +75,11 @@ void lcirs_delete(void* lc);
  * status flags (bit values in TL_FLAG_*).  Return 0, or -1 if there is no
  * light curve yet. */
 int lcirs_copy_lc_full(void* lc, double* out);
+/* Point-source magnification (LightCurveBase::getPointAmp, lcbase.cc:17-38)
+ * at each of the lcLength+1 positions of the last get_lc_irs, from the images
+ * its seed search found, and the magnification-weighted centroid of those
+ * images (x, y per position; 2*(lcLength+1) values).  Either may be NULL. */
+int lcirs_copy_point_amp(void* lc, double* amp, double* centroid);
 int lcirs_copy_ray_counts(void* lc, uint64_t* out);
 int lcirs_copy_flags(void* lc, uint32_t* out);
 
@@ -112,6 +117,14 @@ int lcirs_fill_seeds(void* lc, int n, const double* pos, const int* nSeeds, cons
  * inserts (ms), out[10] idle polls, out[11] wavefronts of the last launch.
  * out must hold 12 values. */
 int tl_last_stats(double* out);
+
+/* Critical curves and caustics of n geometries on the GPU (one lane each):
+ * params[5*g..5*g+4] = a, b, th, m2, m3; cc and ca receive 6*500 complex
+ * values per geometry (branch-major, as lcirs_copy_cc_ca), equal to the host
+ * set-up's (CriticalCurveCaustic::getCC/getCa, ccc.cc:42-98,160-203) to the
+ * bit.  For callers that need many geometries at once (a Metropolis chain
+ * makes a new one per proposal).  Returns 0, or -1 with tl_last_error(). */
+int tl_ccc_batch(int n, const double* params, double* cc, double* ca);
 
 /* Host self-test of the branch-free complex routines the kernels use (modulus,
  * division, square root) against the general ones on n random operand pairs,

@@ -262,3 +262,48 @@ def test_lc_cuda_entry_points_give_the_irs_curve(E):
         out.append(v)
     assert np.array_equal(out[0], out[1])
     assert _rel(out[1], g["amp"][:c.lc_len]) < 1e-6
+
+
+def test_critical_curves_batch_equal_host(E):
+    """tl_ccc_batch: the critical curves and caustics of 96 geometries on the
+    GPU equal the host set-up's (which equals the reference's) to the bit."""
+    from triplelens_amd import workloads as W
+    rng = np.random.default_rng(11)
+    rows = [list(c.lens()) for c in W.c3()]
+    for _ in range(32):
+        rows.append([float(np.exp(rng.uniform(np.log(0.3), np.log(3)))),
+                     float(np.exp(rng.uniform(np.log(0.3), np.log(3)))),
+                     float(rng.uniform(0, 2 * np.pi)),
+                     float(np.exp(rng.uniform(np.log(1e-4), np.log(0.3)))),
+                     float(np.exp(rng.uniform(np.log(1e-4), np.log(0.3))))])
+    cc, ca = E.ccc_batch(np.array(rows))
+    for g, r in enumerate(rows):
+        hc, ha = E.LightCurve(*r, 1e-3, 10, 10).cc_ca()
+        assert np.array_equal(cc[g], hc) and np.array_equal(ca[g], ha), g
+
+
+def test_point_magnification_and_centroid_at_irs_positions(E, oracle):
+    """lcirs_copy_point_amp: A_point at the IRS positions i/lcLength equals the
+    oracle's getPointAmp chain (to 1e-10; the two chains differ in the trial
+    solves in between) and the centroid the weighted mean of the oracle's
+    images."""
+    from triplelens_amd import workloads as W
+    c = W.c1()
+    lc = _new(E, c)
+    lc.lc_irs(*c.traj())
+    amp, cen = lc.point_amp()
+    t = np.arange(c.n_points) / c.lc_len
+    pos = (c.sx + (c.ex - c.sx) * t) + 1j * (c.sy + (c.ey - c.sy) * t)
+    ref = oracle.point_amp(*c.lens(), pos)
+    assert _rel(amp, ref) < 1e-10
+    roots, is_img, _ = oracle.images(*c.lens(), pos)
+    L = c.lens()
+    z1, z2 = 0.0, L[0]
+    z3 = L[1] * np.exp(1j * L[2])
+    m1, m2, m3 = 1 - L[3] - L[4], L[3], L[4]
+    for i in range(0, c.n_points, 7):
+        z = roots[i][is_img[i]]
+        s = m1 / (z - z1) ** 2 + m2 / (z - z2) ** 2 + m3 / (z - z3) ** 2
+        w = 1 / np.abs(1 - np.abs(s) ** 2)
+        want = np.sum(w * z) / np.sum(w)
+        assert abs(complex(*cen[i]) - want) < 1e-9 * max(1.0, abs(want)), i

@@ -71,6 +71,8 @@ struct Curve {
   std::vector<uint64_t> rays;
   std::vector<uint32_t> flags;
   std::vector<int> nSeeds, seedPix;
+  std::vector<double> pointAmp;   // point-source magnification of the last IRS curve
+  std::vector<double> centroid;   // x, y of the weighted image centroid per position
   bool hasIRS = false;
   bool amoebaOut = false;
   std::string amoebaPrefix, parFile;
@@ -86,55 +88,24 @@ void minmax_(const cx* v, int n, cx* mn, cx* mx) {
   }
 }
 
-// CriticalCurveCaustic::getCC + getCa (ccc.cc:42-98,160-203): the degree-6
-// critical-curve polynomial in the reference's operation order, solved at
-// phi = 2*3.14159265359*j/len with root polishing continued from step to step.
+// exp(i phi_j), phi_j = 2*3.14159265359*j/kCccLen, with the host's sincos
+// (the values the reference's critical-curve loop uses, ccc.cc:57).
+const CccTable& ccc_table() {
+  static const CccTable t = [] {
+    CccTable r;
+    for (int j = 0; j < kCccLen; j++) {
+      const double phi = 2.0 * 3.14159265359 * j / kCccLen;
+      host_sincos(phi, &r.e[j].im, &r.e[j].re);
+    }
+    return r;
+  }();
+  return t;
+}
+
 void critical_curves(const Lens& L, std::vector<cx>& cc, std::vector<cx>& ca) {
-  const int len = kCccLen;
-  cc.assign(6 * len, mk(0, 0));
-  ca.assign(6 * len, mk(0, 0));
-  const cx z2 = L.z2, z3 = L.z3;
-  const double m2 = L.m2, m3 = L.m3;
-  cx p[7];
-  const cx z2pz3 = cadd(z2, z3);
-  p[1] = cscale(cmul(cmul(cscale(z2, 2.0), z3), z2pz3), m2 + m3 - 1.0);
-  p[0] = cscale(cmul(cmul(cmul(cneg(z2), z2), z3), z3), m2 + m3 - 1.0);
-  cx roots[6];
-  bool have = false;
-  for (int j = 0; j < len; j++) {
-    const double phi = 2.0 * 3.14159265359 * j / len;
-    double sn, cs;
-    host_sincos(phi, &sn, &cs);
-    const cx e = mk(cs, sn);
-    p[6] = cneg(e);
-    p[5] = cmul(cscale(e, 2.0), z2pz3);
-    const cx q4 = cmul(e, cadd(cadd(cmul(z2, z2), cmul(z3, z3)), cmul(cscale(z2, 4.0), z3)));
-    p[4] = mk(1.0 - q4.re, -q4.im);
-    p[3] = cadd(cadd(cmul(cmul(cmul(cscale(e, 2.0), z2), z3), z2pz3),
-                     cscale(cscale(z2, 2.0), m2 - 1.0)),
-                cscale(cscale(z3, 2.0), m3 - 1.0));
-    const cx t1 = cmul(cmul(cmul(cmul(cneg(e), z2), z2), z3), z3);
-    const cx t2 = cmul(cscale(z2, 4.0 * (1.0 - m3 - m2)), z3);
-    const cx t3 = cscale(cmul(z2, z2), m2 - 1.0);
-    const cx t4 = cscale(cmul(z3, z3), m3 - 1.0);
-    p[2] = csub(csub(cadd(t1, t2), t3), t4);
-    if (!have) {
-      lag_solve<6>(p, nullptr, roots, &trig());
-      have = true;
-    } else {
-      lag_polish<6>(p, roots, &trig());
-      if (!lag_check<6>(p, roots)) lag_solve<6>(p, nullptr, roots, &trig());
-    }
-    for (int k = 0; k < 6; k++) cc[k * len + j] = roots[k];
-  }
-  for (int k = 0; k < 6; k++)
-    for (int j = 0; j < len; j++) {
-      const cx s = cc[k * len + j];
-      cx v = csub(s, cdiv_rc(L.m1, cconj(csub(s, L.z1))));
-      v = csub(v, cdiv_rc(L.m2, cconj(csub(s, L.z2))));
-      v = csub(v, cdiv_rc(L.m3, cconj(csub(s, L.z3))));
-      ca[k * len + j] = v;
-    }
+  cc.assign(6 * kCccLen, mk(0, 0));
+  ca.assign(6 * kCccLen, mk(0, 0));
+  critical_curves_into(L, ccc_table(), &trig(), cc.data(), ca.data());
 }
 
 void ld_default(Curve& c) {
@@ -272,6 +243,8 @@ struct Engine {
   DBuf<int> dNSeeds, dPtCurve, dBatchPts, dLParent, dLPoint;
   DBuf<double2> dPtPos;
   DBuf<float> dCost;
+  DBuf<double> dPtAmp;
+  DBuf<double2> dPtCen;
   DBuf<uint32_t> dFlags;
   DBuf<int2> dSeedTasks;
   DBuf<unsigned long long> dHtab, dLCount, dQSeq, dSync, dPtCount;
@@ -279,6 +252,9 @@ struct Engine {
   DBuf<Task> dQBuf;
   DBuf<RootChain> dChain;
   DBuf<double> dPointLC;
+  DBuf<Lens> dLenses;
+  DBuf<CccTable> dCccTab;
+  DBuf<cx> dCcOut, dCaOut;
   Options opt;
   Stats stats;
 
@@ -409,6 +385,8 @@ void Engine::run(Job& job) {
   dNSeeds.ensure(nPts);
   dPtPos.ensure(nPts);
   dCost.ensure(nPts);
+  dPtAmp.ensure(nPts);
+  dPtCen.ensure(nPts);
   dFlags.ensure(nPts);
   dPtCurve.ensure(nPts);
   HIPCHK(hipMemcpyAsync(dCurves.p, hc.data(), nc * sizeof(CurveDev), hipMemcpyHostToDevice, stream));
@@ -445,6 +423,8 @@ void Engine::run(Job& job) {
     sa.nSeeds = dNSeeds.p;
     sa.ptPos = dPtPos.p;
     sa.cost = dCost.p;
+    sa.ptAmp = dPtAmp.p;
+    sa.ptCen = dPtCen.p;
     sa.flags = dFlags.p;
     sa.chainOut = dChain.p;
     // the seed stream starts after the set-up copies on the engine stream
@@ -644,9 +624,15 @@ void Engine::run(Job& job) {
 
   // results back into the objects
   std::vector<int2> seedsHost;
+  std::vector<double> ampHost;
+  std::vector<double2> cenHost;
   if (job.runSeeds) {
     seedsHost.resize((size_t)nPts * kMaxSeeds);
     HIPCHK(hipMemcpy(seedsHost.data(), dSeeds.p, seedsHost.size() * sizeof(int2), hipMemcpyDeviceToHost));
+    ampHost.resize(nPts);
+    cenHost.resize(nPts);
+    HIPCHK(hipMemcpy(ampHost.data(), dPtAmp.p, nPts * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(cenHost.data(), dPtCen.p, nPts * sizeof(double2), hipMemcpyDeviceToHost));
   }
   for (int k = 0; k < nc; k++) {
     Curve& c = *job.curves[k];
@@ -667,6 +653,12 @@ void Engine::run(Job& job) {
                   (size_t)n * kMaxSeeds * sizeof(int2));
       for (unsigned i = 0; i < c.lcLength && i < (unsigned)n; i++) c.lcVec[i] = c.full[i];
       c.hasIRS = true;
+      c.pointAmp.assign(ampHost.begin() + pointBase[k], ampHost.begin() + pointBase[k + 1]);
+      c.centroid.resize(2 * (size_t)n);
+      for (int i = 0; i < n; i++) {
+        c.centroid[2 * i] = cenHost[pointBase[k] + i].x;
+        c.centroid[2 * i + 1] = cenHost[pointBase[k] + i].y;
+      }
       if (wantAmoebae && c.amoebaOut)
         for (int i = 0; i < n; i++) write_amoeba_file(c, i, amoebae[pointBase[k] + i]);
     }
@@ -835,6 +827,17 @@ void set_limb_darkening_cuda(void* lc, const char* model, double v) {
   set_limb_darkening(lc, model, v);
 }
 
+int lcirs_copy_point_amp(void* lc, double* amp, double* centroid) {
+  return guarded([&] {
+    if (!lc) throw std::invalid_argument("null light-curve object");
+    Curve* c = static_cast<Curve*>(lc);
+    if (!c->hasIRS) throw std::runtime_error("no extended-source curve computed yet");
+    if (amp) std::copy(c->pointAmp.begin(), c->pointAmp.end(), amp);
+    if (centroid) std::copy(c->centroid.begin(), c->centroid.end(), centroid);
+    return 0;
+  });
+}
+
 int lcirs_copy_lc_full(void* lc, double* out) {
   const Curve* c = static_cast<Curve*>(lc);
   if (!c || !c->hasIRS) return -1;
@@ -935,6 +938,37 @@ int lcirs_fill_seeds(void* lc, int n, const double* pos, const int* nSeeds, cons
       if (amp) amp[i] = c->full[i];
       if (flags) flags[i] = c->flags[i];
     }
+    return 0;
+  });
+}
+
+int tl_ccc_batch(int n, const double* params, double* cc, double* ca) {
+  return guarded([&] {
+    if (n < 0 || (n > 0 && (!params || !cc || !ca))) throw std::invalid_argument("bad arguments");
+    if (n == 0) return 0;
+    std::vector<Lens> lenses(n);
+    for (int g = 0; g < n; g++) {
+      const double* q = params + 5 * (size_t)g;
+      for (int k = 0; k < 5; k++)
+        if (!std::isfinite(q[k])) throw std::invalid_argument("lens parameters must be finite");
+      lenses[g] = make_lens(q[0], q[1], q[2], q[3], q[4]);
+    }
+    Engine& E = engine();
+    std::lock_guard<std::mutex> lock(E.mu);
+    E.init();
+    E.dLenses.ensure(n);
+    E.dCccTab.ensure(1);
+    E.dCcOut.ensure((size_t)n * 6 * kCccLen);
+    E.dCaOut.ensure((size_t)n * 6 * kCccLen);
+    HIPCHK(hipMemcpyAsync(E.dLenses.p, lenses.data(), n * sizeof(Lens), hipMemcpyHostToDevice,
+                          E.stream));
+    HIPCHK(hipMemcpyAsync(E.dCccTab.p, &ccc_table(), sizeof(CccTable), hipMemcpyHostToDevice,
+                          E.stream));
+    HIPCHK((hipError_t)launch_ccc(E.dLenses.p, n, E.dCccTab.p, E.dTrig.p, E.dCcOut.p, E.dCaOut.p,
+                                  E.stream));
+    HIPCHK(hipStreamSynchronize(E.stream));
+    HIPCHK(hipMemcpy(cc, E.dCcOut.p, (size_t)n * 6 * kCccLen * sizeof(cx), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ca, E.dCaOut.p, (size_t)n * 6 * kCccLen * sizeof(cx), hipMemcpyDeviceToHost));
     return 0;
   });
 }

@@ -64,6 +64,8 @@ struct SeedArgs {
   int* nSeeds;
   double2* ptPos;     // source position of every point
   float* cost;        // predicted rays (point-source magnification x pi ppr^2)
+  double* ptAmp;      // point-source magnification at every position
+  double2* ptCen;     // magnification-weighted centroid of the point images
   uint32_t* flags;
   RootChain* chainOut; // per curve: root-reuse state after the last position
   const RootChain* chainIn;  // per curve: state to start from (nullptr: the curve's own)
@@ -112,6 +114,8 @@ struct FillArgs {
 
 // Host-side launchers (tl_kernels.hip).  All return hipError_t as int.
 int launch_seed(const SeedArgs& a, hipStream_t s);
+int launch_ccc(const Lens* lenses, int n, const CccTable* tab, const TrigTable* trig, cx* cc,
+               cx* ca, hipStream_t s);
 int launch_fill(const FillArgs& a, int nWorkgroups, hipStream_t s);
 int fill_resident_workgroups();
 int launch_clear_flag_bits(uint32_t* flags, const int* pts, int n, uint32_t bits, hipStream_t s);

@@ -505,8 +505,25 @@ __global__ void __launch_bounds__(64) seed_kernel(SeedArgs a) {
       }
       const unsigned long long m = wl_get_images(L, Dv, Sv, chainv, have, zeta, a.trig);
       emit_seeds(C, chainv, m, out, cnt, fl);
-      // predicted image area (point magnification) for the fill's ordering
-      if (centre) amp = wave_sum_d(((m >> ln) & 1ull) ? point_amp_term(L, chainv) : 0.0);
+      if (centre) {
+        // point-source magnification (getPointAmp, lcbase.cc:17-38) and the
+        // magnification-weighted image centroid, summed in root order
+        const double term = ((m >> ln) & 1ull) ? point_amp_term(L, chainv) : 0.0;
+        const double wx = term * chainv.re, wy = term * chainv.im;
+        double sx = 0.0, sy = 0.0;
+        unsigned long long mm = m;
+        while (mm) {
+          const int k = __builtin_ctzll(mm);
+          mm &= mm - 1;
+          amp += rl(term, k);
+          sx += rl(wx, k);
+          sy += rl(wy, k);
+        }
+        if (ln == 0) {
+          a.ptAmp[p] = amp;
+          a.ptCen[p] = make_double2(sx / amp, sy / amp);
+        }
+      }
     }
     if (ln == 0) {
       a.nSeeds[p] = cnt;
@@ -568,6 +585,25 @@ int launch_clear_flag_bits(uint32_t* flags, const int* pts, int n, uint32_t bits
   if (n <= 0) return 0;
   hipLaunchKernelGGL(clear_flag_bits_kernel, dim3((n + 255) / 256), dim3(256), 0, s, flags, pts, n,
                      bits);
+  return (int)hipGetLastError();
+}
+
+// Critical curves and caustics of many geometries, one per lane: the host
+// routine (critical_curves_into) with the host's step table, so every value
+// is the host's, and the reference's, to the bit.
+__global__ void __launch_bounds__(64) ccc_kernel(const Lens* lenses, int n, const CccTable* tab,
+                                                 const TrigTable* trig, cx* cc, cx* ca) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const size_t off = (size_t)g * 6 * kCccSteps;
+  critical_curves_into(lenses[g], *tab, trig, cc + off, ca + off);
+}
+
+int launch_ccc(const Lens* lenses, int n, const CccTable* tab, const TrigTable* trig, cx* cc,
+               cx* ca, hipStream_t s) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(ccc_kernel, dim3((n + 63) / 64), dim3(64), 0, s, lenses, n, tab, trig, cc,
+                     ca);
   return (int)hipGetLastError();
 }
 

@@ -330,6 +330,62 @@ inline Lens make_lens(double a, double b, double th, double m2, double m3) {
   return L;
 }
 
+// ---- critical curves and caustics -----------------------------------------
+constexpr int kCccSteps = 500;  // critical-curve samples per branch (lcirs.cc:16)
+
+// exp(i phi_j) for the critical-curve steps, filled on the host with the
+// system sincos (the reference's values).
+struct CccTable { cx e[kCccSteps]; };
+
+// CriticalCurveCaustic::getCC + getCa (ccc.cc:42-98,160-203): the degree-6
+// critical-curve polynomial in the reference's operation order, solved at
+// phi_j = 2*3.14159265359*j/kCccSteps with root polishing continued from step
+// to step, and the caustic as the lens map of every critical point.  cc and
+// ca receive 6 x kCccSteps values, branch-major.
+TL_HD void critical_curves_into(const Lens& L, const CccTable& tab, const TrigTable* trig,
+                                cx* cc, cx* ca) {
+  const int len = kCccSteps;
+  const cx z2 = L.z2, z3 = L.z3;
+  const double m2 = L.m2, m3 = L.m3;
+  cx p[7];
+  const cx z2pz3 = cadd(z2, z3);
+  p[1] = cscale(cmul(cmul(cscale(z2, 2.0), z3), z2pz3), m2 + m3 - 1.0);
+  p[0] = cscale(cmul(cmul(cmul(cneg(z2), z2), z3), z3), m2 + m3 - 1.0);
+  cx roots[6];
+  bool have = false;
+  for (int j = 0; j < len; j++) {
+    const cx e = tab.e[j];
+    p[6] = cneg(e);
+    p[5] = cmul(cscale(e, 2.0), z2pz3);
+    const cx q4 = cmul(e, cadd(cadd(cmul(z2, z2), cmul(z3, z3)), cmul(cscale(z2, 4.0), z3)));
+    p[4] = mk(1.0 - q4.re, -q4.im);
+    p[3] = cadd(cadd(cmul(cmul(cmul(cscale(e, 2.0), z2), z3), z2pz3),
+                     cscale(cscale(z2, 2.0), m2 - 1.0)),
+                cscale(cscale(z3, 2.0), m3 - 1.0));
+    const cx t1 = cmul(cmul(cmul(cmul(cneg(e), z2), z2), z3), z3);
+    const cx t2 = cmul(cscale(z2, 4.0 * (1.0 - m3 - m2)), z3);
+    const cx t3 = cscale(cmul(z2, z2), m2 - 1.0);
+    const cx t4 = cscale(cmul(z3, z3), m3 - 1.0);
+    p[2] = csub(csub(cadd(t1, t2), t3), t4);
+    if (!have) {
+      lag_solve<6>(p, nullptr, roots, trig);
+      have = true;
+    } else {
+      lag_polish<6>(p, roots, trig);
+      if (!lag_check<6>(p, roots)) lag_solve<6>(p, nullptr, roots, trig);
+    }
+    for (int k = 0; k < 6; k++) cc[k * len + j] = roots[k];
+  }
+  for (int k = 0; k < 6; k++)
+    for (int j = 0; j < len; j++) {
+      const cx s = cc[k * len + j];
+      cx v = csub(s, cdiv_rc(L.m1, cconj(csub(s, L.z1))));
+      v = csub(v, cdiv_rc(L.m2, cconj(csub(s, L.z2))));
+      v = csub(v, cdiv_rc(L.m3, cconj(csub(s, L.z3))));
+      ca[k * len + j] = v;
+    }
+}
+
 // ---- double-double polynomial arithmetic for the image polynomial ----------
 struct dd { double hi, lo; };
 struct dcx { dd re, im; };

@@ -133,6 +133,16 @@ class LightCurve:
         _check(self._lib.lcirs_copy_flags(self.h, flags.ctypes.data))
         return amp, rays, flags
 
+    def point_amp(self):
+        """Point-source magnification and weighted image centroid at the
+        positions of the last lc_irs: (lcLength+1,), (lcLength+1, 2)."""
+        n = self.lc_len + 1
+        amp = np.zeros(n)
+        cen = np.zeros((n, 2))
+        self._lib.lcirs_copy_point_amp.argtypes = [_P, _P, _P]
+        _check(self._lib.lcirs_copy_point_amp(self.h, amp.ctypes.data, cen.ctypes.data))
+        return amp, cen
+
     def point_lc(self, sx, sy, ex, ey):
         self._lib.get_lc(self.h, sx, sy, ex, ey)
         _check()
@@ -176,6 +186,19 @@ class LightCurve:
                                           pix.ctypes.data, rays.ctypes.data, amp.ctypes.data,
                                           flags.ctypes.data))
         return amp, rays, flags
+
+
+def ccc_batch(params):
+    """Critical curves and caustics of many geometries on the GPU.
+    params: (n, 5) rows of a, b, th, m2, m3.  Returns cc, ca as (n, 6, 500)."""
+    lib = load()
+    p = np.ascontiguousarray(params, np.float64).reshape(-1, 5)
+    n = len(p)
+    cc = np.zeros((n, 6, 500), np.complex128)
+    ca = np.zeros((n, 6, 500), np.complex128)
+    lib.tl_ccc_batch.argtypes = [ctypes.c_int, _P, _P, _P]
+    _check(lib.tl_ccc_batch(n, p.ctypes.data, cc.ctypes.data, ca.ctypes.data))
+    return cc, ca
 
 
 def batch_irs(curves, trajs):
