@@ -76,10 +76,34 @@ void lcirs_delete(void* lc);
  * light curve yet. */
 int lcirs_copy_lc_full(void* lc, double* out);
 /* Point-source magnification (LightCurveBase::getPointAmp, lcbase.cc:17-38)
- * at each of the lcLength+1 positions of the last get_lc_irs, from the images
- * its seed search found, and the magnification-weighted centroid of those
- * images (x, y per position; 2*(lcLength+1) values).  Either may be NULL. */
-int lcirs_copy_point_amp(void* lc, double* amp, double* centroid);
+ * at each of the lcLength+1 positions of the last seed search (get_lc_irs or
+ * lcirs_find_seeds), from the images it found; the magnification-weighted
+ * centroid of those images (x, y per position; 2*(lcLength+1) values); and
+ * the distance from the position to the nearest caustic sample, negative
+ * where the position has more than 4 images; and the share of the point
+ * magnification carried by its two brightest images.  Any pointer may be
+ * NULL. */
+int lcirs_copy_point_amp(void* lc, double* amp, double* centroid, double* causticDist,
+                         double* pairShare);
+
+/* Disk samples.  With `on`, every later seed search of the object (get_lc_irs,
+ * lcirs_find_seeds, lcirs_batch_irs) also computes the point magnification
+ * at 12 points around each position, solved from that position's own image
+ * roots: offsets rho/2 on the axes (+x, +y, -x, -y), rho on the axes, and
+ * rho on the diagonals (45, 135, 225, 315 degrees).  These are the samples
+ * of Gould's (2008) hexadecapole approximation of the finite-source
+ * magnification.  No counterpart in the reference. */
+int lcirs_set_ring(void* lc, int on);
+/* The disk-sample magnifications of the last seed search, (lcLength+1) x 12,
+ * and (may be NULL) the 10 roots of each position's own solve, (lcLength+1)
+ * x 10 complex as re, im pairs. */
+int lcirs_copy_ring_amp(void* lc, double* ring, double* centreRoots);
+/* The seed search of get_lc_irs alone (lcirs.cc:116-165, no fill): seeds
+ * (lcirs_copy_seeds), point magnifications, centroids and caustic distances
+ * (lcirs_copy_point_amp), with the root chain advanced as get_lc_irs
+ * advances it.  For callers that fill only some positions
+ * (lcirs_fill_seeds). */
+int lcirs_find_seeds(void* lc, double iniX, double iniY, double finX, double finY);
 int lcirs_copy_ray_counts(void* lc, uint64_t* out);
 int lcirs_copy_flags(void* lc, uint32_t* out);
 

@@ -604,6 +604,30 @@ int tlo_point_amp(double a, double b, double th, double m2, double m3, int n,
   return 0;
 }
 
+/* Point magnification at the 12 disk samples around each position (the
+ * samples of Gould's 2008 hexadecapole approximation: rho/2 on the axes,
+ * rho on the axes, rho on the diagonals), each solved from the given roots
+ * of the position's own solve: the root-reuse solve of getRootsPrecalculated
+ * (imgpoint.cc:160-205) and the sum of getPointAmp (lcbase.cc:17-38).  No
+ * counterpart in the reference; restates the engine's ring_amp_kernel. */
+int tlo_ring_amp(double a, double b, double th, double m2, double m3, double rho, int n,
+                 const double* pos, const double* roots, double* out) {
+  static const double c = 0.70710678118654752;
+  static const double off[12][2] = {{0.5, 0.0}, {0.0, 0.5}, {-0.5, 0.0}, {0.0, -0.5},
+                                    {1.0, 0.0}, {0.0, 1.0}, {-1.0, 0.0}, {0.0, -1.0},
+                                    {c, c}, {-c, c}, {-c, -c}, {c, -c}};
+  imgpoint_t ip;
+  imgpoint_init(&ip, a, b, th, m2, m3);
+  for (int i = 0; i < n; i++)
+    for (int s = 0; s < 12; s++) {
+      for (int k = 0; k < 10; k++) ip.tempRoots[k] = mk(roots[20 * i + 2 * k], roots[20 * i + 2 * k + 1]);
+      ip.tempRoots[10] = mk(0.0, 0.0);
+      ip.haveRoots = 1;
+      out[12 * i + s] = point_amp(&ip, mk(pos[2 * i] + rho * off[s][0], pos[2 * i + 1] + rho * off[s][1]));
+    }
+  return 0;
+}
+
 /* ---- critical curves and caustics (ccc.cc) ------------------------------ */
 
 #define CCC_MAXLEN 100000

@@ -105,6 +105,10 @@ int tlo_hit_mask(const tlo_lc* lc, long x0, long x1, long y0, long y1, double sx
  * along n positions with one root chain. */
 int tlo_point_amp(double a, double b, double th, double m2, double m3, int n,
                   const double* pos, double* out);
+/* Point magnification at the 12 disk samples around each position, each
+ * solved from the position's given 10 roots (re, im pairs): out[12 i + s]. */
+int tlo_ring_amp(double a, double b, double th, double m2, double m3, double rho, int n,
+                 const double* pos, const double* roots, double* out);
 
 /* The complex-arithmetic helpers, exposed for tests against libm/libgcc, and
  * the system routines they restate. */

@@ -291,7 +291,7 @@ def test_point_magnification_and_centroid_at_irs_positions(E, oracle):
     c = W.c1()
     lc = _new(E, c)
     lc.lc_irs(*c.traj())
-    amp, cen = lc.point_amp()
+    amp, cen, caus = lc.point_amp()
     t = np.arange(c.n_points) / c.lc_len
     pos = (c.sx + (c.ex - c.sx) * t) + 1j * (c.sy + (c.ey - c.sy) * t)
     ref = oracle.point_amp(*c.lens(), pos)
@@ -307,3 +307,41 @@ def test_point_magnification_and_centroid_at_irs_positions(E, oracle):
         w = 1 / np.abs(1 - np.abs(s) ** 2)
         want = np.sum(w * z) / np.sum(w)
         assert abs(complex(*cen[i]) - want) < 1e-9 * max(1.0, abs(want)), i
+    # caustic distance: the nearest of the curve's caustic samples, signed by
+    # the image count
+    _, ca = lc.cc_ca()
+    for i in range(0, c.n_points, 13):
+        d = np.min(np.abs(ca.ravel() - pos[i]))
+        want = -d if is_img[i].sum() > 4 else d
+        assert abs(caus[i] - want) <= 1e-12 * max(1.0, d), i
+
+
+def test_disk_samples_match_oracle(E, oracle):
+    """lcirs_set_ring: the 12 disk-sample magnifications around each IRS
+    position equal the oracle's (tlo_ring_amp) from the same centre roots,
+    and those roots give the position's point magnification."""
+    from triplelens_amd import workloads as W
+    c = W.c1()
+    lc = _new(E, c)
+    lc.set_ring(True)
+    lc.find_seeds(*c.traj())
+    amp, _, _ = lc.point_amp()
+    ring, roots = lc.ring_amp()
+    t = np.arange(c.n_points) / c.lc_len
+    pos = (c.sx + (c.ex - c.sx) * t) + 1j * (c.sy + (c.ey - c.sy) * t)
+    want = oracle.ring_amp(*c.lens(), c.rho, pos, roots)
+    assert _rel(ring, want) < 1e-13
+    L = c.lens()
+    zs = np.array([0.0, L[0], L[1] * np.exp(1j * L[2])])
+    ms = np.array([1 - L[3] - L[4], L[3], L[4]])
+    for i in range(0, c.n_points, 11):
+        z = roots[i][:, None]
+        res = np.abs(z[:, 0] - np.sum(ms / np.conj(z - zs), axis=1) - pos[i])
+        s = np.sum(ms / (z - zs) ** 2, axis=1)
+        a = np.sum(1 / np.abs(1 - np.abs(s[res < 1e-6]) ** 2))
+        assert abs(a - amp[i]) <= 1e-12 * amp[i], i
+    # without set_ring the copy is refused
+    lc2 = _new(E, c)
+    lc2.find_seeds(*c.traj())
+    with pytest.raises(E.EngineError):
+        lc2.ring_amp()

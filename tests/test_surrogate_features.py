@@ -1,0 +1,94 @@
+"""The surrogate's physics features on the CPU: the disk moments of the
+limb-darkening profile against quadrature, the hexadecapole assembly against
+the exact disk mean of fields it must reproduce, and the oracle's disk-sample
+magnifications (tlo_ring_amp) against images found independently with numpy's
+polynomial roots."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate
+
+from triplelens_amd import surrogate as S
+
+
+def _profile_mean(f, u):
+    """Mean of f(r, t) over the unit disk weighted by 1 - u (1 - sqrt(1 - r^2))."""
+    w = lambda r: (1.0 - u * (1.0 - math.sqrt(max(0.0, 1.0 - r * r)))) * r
+    num = integrate.dblquad(lambda t, r: f(r, t) * w(r), 0.0, 1.0, 0.0, 2 * math.pi,
+                            epsabs=1e-13, epsrel=1e-12)[0]
+    den = integrate.quad(lambda r: 2 * math.pi * w(r), 0.0, 1.0, epsabs=1e-14)[0]
+    return num / den
+
+
+@pytest.mark.parametrize("u", [0.0, 0.3, 0.5, 0.8])
+def test_disk_moments_match_quadrature(u):
+    x2, x4 = S.disk_moments(u)
+    assert x2 == pytest.approx(_profile_mean(lambda r, t: r * r, u), rel=1e-10)
+    assert x4 == pytest.approx(_profile_mean(lambda r, t: r ** 4, u), rel=1e-10)
+    if u == 0.0:
+        assert (x2, x4) == (0.5, pytest.approx(1.0 / 3.0))
+
+
+@pytest.mark.parametrize("u", [0.0, 0.5])
+def test_hexadecapole_exact_for_fields_up_to_fourth_order(u):
+    """For A = c0 + c2 r^2 + c4 r^4 plus angular modes m = 1, 2, 3 (which
+    the four-point means cancel), the expansion is the exact disk mean; a
+    dropped term, a wrong weight or a wrong sample radius breaks this."""
+    rho = 0.37
+    c0, c2, c4, d1, d2, d3 = 2.0, 0.8, -3.1, 0.6, 1.7, -0.9
+
+    def A(r, t):  # r in units of rho
+        rr = r * rho
+        return (c0 + c2 * rr ** 2 + c4 * rr ** 4 + d1 * rr * math.cos(t)
+                + d2 * rr ** 2 * math.cos(2 * t) + d3 * rr ** 3 * math.sin(3 * t))
+
+    c = math.sqrt(0.5)
+    off = [(0.5, 0), (0, 0.5), (-0.5, 0), (0, -0.5), (1, 0), (0, 1), (-1, 0), (0, -1),
+           (c, c), (-c, c), (-c, -c), (c, -c)]
+    ring = np.array([[A(math.hypot(x, y), math.atan2(y, x)) for x, y in off]])
+    A_hex, t4 = S.hexadecapole(np.array([c0]), ring, u)
+    want = _profile_mean(A, u)
+    assert A_hex[0] == pytest.approx(want, rel=1e-11)
+    x2, x4 = S.disk_moments(u)
+    assert t4[0] == pytest.approx(c4 * rho ** 4 * x4, rel=1e-9)
+
+
+def _numpy_point_amp(lens, zeta):
+    """Images from numpy's roots of the oracle's polynomial (numpy's
+    companion-matrix eigenvalues, not Laguerre), kept when they satisfy the
+    lens equation, and the sum of 1/|det J|."""
+    from oracle import oracle as O
+    a, b, th, m2, m3 = lens
+    R = O.Oracle()
+    zs = np.array([0.0, a, b * np.exp(1j * th)])
+    ms = np.array([1 - m2 - m3, m2, m3])
+    c = R.coeffs(a, b, th, m2, m3, zeta)
+    roots = np.roots(c[::-1])
+    amp = 0.0
+    for z in roots:
+        # numpy's roots of these polynomials satisfy the lens equation to
+        # ~1e-11 where they are images and miss it by O(0.1) where not
+        if abs(z - np.sum(ms / np.conj(z - zs)) - zeta) < 1e-8:
+            s = np.sum(ms / (z - zs) ** 2)
+            amp += 1.0 / abs(1.0 - abs(s) ** 2)
+    return amp
+
+
+def test_oracle_ring_amp_matches_independent_images(oracle):
+    from triplelens_amd import workloads as W
+    c = W.c1()
+    lens = c.lens()
+    idx = [0, 40, 120, 199]
+    t = np.array(idx) / c.lc_len
+    pos = (c.sx + (c.ex - c.sx) * t) + 1j * (c.sy + (c.ey - c.sy) * t)
+    roots, _, _ = oracle.images(*lens, pos)
+    ring = oracle.ring_amp(*lens, c.rho, pos, roots)
+    cc = math.sqrt(0.5)
+    off = np.array([0.5, 0.5j, -0.5, -0.5j, 1, 1j, -1, -1j,
+                    cc + cc * 1j, -cc + cc * 1j, -cc - cc * 1j, cc - cc * 1j])
+    for i, p in enumerate(pos):
+        want = np.array([_numpy_point_amp(lens, p + c.rho * o) for o in off])
+        np.testing.assert_allclose(ring[i], want, rtol=1e-9)

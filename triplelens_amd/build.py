@@ -17,7 +17,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "liblcirs.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["tl_kernels.hip", "tl_api.cpp"]
+SOURCES = ["tl_kernels.hip", "tl_api.cpp", "tl_mlp.hip", "tl_surrogate.cpp"]
 FLAGS = [
     "--offload-arch=gfx950",
     "-O3",

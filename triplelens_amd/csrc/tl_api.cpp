@@ -73,6 +73,12 @@ struct Curve {
   std::vector<int> nSeeds, seedPix;
   std::vector<double> pointAmp;   // point-source magnification of the last IRS curve
   std::vector<double> centroid;   // x, y of the weighted image centroid per position
+  std::vector<double> causDist;   // signed distance to the nearest caustic sample
+  std::vector<double> pairShare;  // share of the point magnification in the two brightest images
+  bool ring = false;              // seed searches also compute the disk-sample magnifications
+  std::vector<double> ringAmp;    // kRingSamples per position
+  std::vector<cx> centreRoots;    // 10 per position: the roots of the position's own solve
+  bool hasSeeds = false;          // a seed search (with or without fills) has run
   bool hasIRS = false;
   bool amoebaOut = false;
   std::string amoebaPrefix, parFile;
@@ -221,6 +227,7 @@ struct Job {          // one engine run over a set of curves
   std::vector<Curve*> curves;
   std::vector<cx> starts, ends;
   bool runSeeds = true;
+  bool runFill = true;   // false: the seed search only
   // when runSeeds is false (lcirs_fill_seeds): explicit positions and seeds
   std::vector<double2> pos;
   std::vector<int> nSeeds;
@@ -243,7 +250,8 @@ struct Engine {
   DBuf<int> dNSeeds, dPtCurve, dBatchPts, dLParent, dLPoint;
   DBuf<double2> dPtPos;
   DBuf<float> dCost;
-  DBuf<double> dPtAmp;
+  DBuf<double> dPtAmp, dPtCaus, dPtPair, dRing;
+  DBuf<cx> dPtRoots;
   DBuf<double2> dPtCen;
   DBuf<uint32_t> dFlags;
   DBuf<int2> dSeedTasks;
@@ -350,6 +358,7 @@ CurveDev to_dev(const Curve& c, cx s, cx e, int pointBase, int causticBase) {
   d.pointBase = pointBase;
   d.causticBase = causticBase;
   d.chainHave = c.chain.have;
+  d.ring = c.ring ? 1 : 0;
   for (int k = 0; k < 11; k++) d.chain[k] = c.chain.tempRoots[k];
   for (int k = 0; k < 6; k++) {
     d.boxMin[k] = c.boxMin[k];
@@ -387,6 +396,12 @@ void Engine::run(Job& job) {
   dCost.ensure(nPts);
   dPtAmp.ensure(nPts);
   dPtCen.ensure(nPts);
+  dPtCaus.ensure(nPts);
+  dPtPair.ensure(nPts);
+  dPtRoots.ensure((size_t)nPts * 10);
+  bool anyRing = false;
+  for (auto* c : job.curves) anyRing |= c->ring && job.runSeeds;
+  if (anyRing) dRing.ensure((size_t)nPts * kRingSamples);
   dFlags.ensure(nPts);
   dPtCurve.ensure(nPts);
   HIPCHK(hipMemcpyAsync(dCurves.p, hc.data(), nc * sizeof(CurveDev), hipMemcpyHostToDevice, stream));
@@ -425,6 +440,9 @@ void Engine::run(Job& job) {
     sa.cost = dCost.p;
     sa.ptAmp = dPtAmp.p;
     sa.ptCen = dPtCen.p;
+    sa.ptCaus = dPtCaus.p;
+    sa.ptPair = dPtPair.p;
+    sa.ptRoots = dPtRoots.p;
     sa.flags = dFlags.p;
     sa.chainOut = dChain.p;
     // the seed stream starts after the set-up copies on the engine stream
@@ -438,6 +456,11 @@ void Engine::run(Job& job) {
       HIPCHK((hipError_t)launch_seed(sa, sstream));
       HIPCHK(hipEventRecord(evSeed[ch], sstream));
     }
+    // disk samples of the curves that asked for them, behind the seed search
+    // (and beside the fills)
+    if (anyRing)
+      HIPCHK((hipError_t)launch_ring_amp(dCurves.p, dPtCurve.p, dPtPos.p, dPtRoots.p, nPts,
+                                         dTrig.p, dRing.p, sstream));
   } else {
     for (int p = 0; p < nPts; p++) {
       nSeeds[p] = std::min(job.nSeeds[p], kMaxSeeds);
@@ -487,19 +510,30 @@ void Engine::run(Job& job) {
   for (int k = 0; k < nc; k++) {
     const long long len = pointBase[k + 1] - pointBase[k];
     const int i0 = (int)(len * ch / nChunks), i1 = (int)(len * (ch + 1) / nChunks);
-    for (int i = i0; i < i1; i++)
+    for (int i = i0; i < i1 && job.runFill; i++)
       if (nSeeds[pointBase[k] + i] > 0) order.push_back(pointBase[k] + i);
   }
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
-  for (size_t b0 = 0; b0 < order.size(); b0 += kMaxBatch) {
-    const int nb = (int)std::min<size_t>(kMaxBatch, order.size() - b0);
-    std::vector<int> batchPts(order.begin() + b0, order.begin() + b0 + nb);
-    std::vector<int2> seedTasks;
+  // A batch holds at most kMaxBatch positions and about kRunBudget predicted
+  // runs (the run table is sized from the prediction).
+  constexpr double kRunBudget = 16.0 * 1024 * 1024;
+  size_t stepPts = 0;
+  for (size_t b0 = 0; b0 < order.size(); b0 += stepPts) {
     double predicted = 0;
+    size_t b1 = b0;
+    while (b1 < order.size() && b1 - b0 < (size_t)kMaxBatch) {
+      const double pc = std::max(256.0, (double)cost[order[b1]] / 64.0);
+      if (b1 > b0 && predicted + pc > kRunBudget) break;
+      predicted += pc;
+      b1++;
+    }
+    stepPts = b1 - b0;
+    const int nb = (int)stepPts;
+    std::vector<int> batchPts(order.begin() + b0, order.begin() + b1);
+    std::vector<int2> seedTasks;
     for (int b = 0; b < nb; b++) {
       const int gp = batchPts[b];
       for (int k = 0; k < nSeeds[gp]; k++) seedTasks.push_back(make_int2(b, k));
-      predicted += std::max(256.0, (double)cost[gp] / 64.0);
     }
     dBatchPts.ensure(nb);
     dSeedTasks.ensure(seedTasks.size());
@@ -624,7 +658,8 @@ void Engine::run(Job& job) {
 
   // results back into the objects
   std::vector<int2> seedsHost;
-  std::vector<double> ampHost;
+  std::vector<double> ampHost, causHost, pairHost, ringHost;
+  std::vector<cx> rootsHost;
   std::vector<double2> cenHost;
   if (job.runSeeds) {
     seedsHost.resize((size_t)nPts * kMaxSeeds);
@@ -633,6 +668,18 @@ void Engine::run(Job& job) {
     cenHost.resize(nPts);
     HIPCHK(hipMemcpy(ampHost.data(), dPtAmp.p, nPts * sizeof(double), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(cenHost.data(), dPtCen.p, nPts * sizeof(double2), hipMemcpyDeviceToHost));
+    causHost.resize(nPts);
+    HIPCHK(hipMemcpy(causHost.data(), dPtCaus.p, nPts * sizeof(double), hipMemcpyDeviceToHost));
+    pairHost.resize(nPts);
+    HIPCHK(hipMemcpy(pairHost.data(), dPtPair.p, nPts * sizeof(double), hipMemcpyDeviceToHost));
+    if (anyRing) {
+      ringHost.resize((size_t)nPts * kRingSamples);
+      HIPCHK(hipMemcpy(ringHost.data(), dRing.p, ringHost.size() * sizeof(double),
+                       hipMemcpyDeviceToHost));
+      rootsHost.resize((size_t)nPts * 10);
+      HIPCHK(hipMemcpy(rootsHost.data(), dPtRoots.p, rootsHost.size() * sizeof(cx),
+                       hipMemcpyDeviceToHost));
+    }
   }
   for (int k = 0; k < nc; k++) {
     Curve& c = *job.curves[k];
@@ -651,8 +698,22 @@ void Engine::run(Job& job) {
       c.seedPix.assign((size_t)n * kMaxSeeds * 2, 0);
       std::memcpy(c.seedPix.data(), seedsHost.data() + (size_t)pointBase[k] * kMaxSeeds,
                   (size_t)n * kMaxSeeds * sizeof(int2));
-      for (unsigned i = 0; i < c.lcLength && i < (unsigned)n; i++) c.lcVec[i] = c.full[i];
-      c.hasIRS = true;
+      if (job.runFill) {
+        for (unsigned i = 0; i < c.lcLength && i < (unsigned)n; i++) c.lcVec[i] = c.full[i];
+        c.hasIRS = true;
+      }
+      c.hasSeeds = true;
+      c.causDist.assign(causHost.begin() + pointBase[k], causHost.begin() + pointBase[k + 1]);
+      c.pairShare.assign(pairHost.begin() + pointBase[k], pairHost.begin() + pointBase[k + 1]);
+      if (c.ring) {
+        c.ringAmp.assign(ringHost.begin() + (size_t)pointBase[k] * kRingSamples,
+                         ringHost.begin() + (size_t)pointBase[k + 1] * kRingSamples);
+        c.centreRoots.assign(rootsHost.begin() + (size_t)pointBase[k] * 10,
+                             rootsHost.begin() + (size_t)pointBase[k + 1] * 10);
+      } else {
+        c.ringAmp.clear();
+        c.centreRoots.clear();
+      }
       c.pointAmp.assign(ampHost.begin() + pointBase[k], ampHost.begin() + pointBase[k + 1]);
       c.centroid.resize(2 * (size_t)n);
       for (int i = 0; i < n; i++) {
@@ -827,13 +888,56 @@ void set_limb_darkening_cuda(void* lc, const char* model, double v) {
   set_limb_darkening(lc, model, v);
 }
 
-int lcirs_copy_point_amp(void* lc, double* amp, double* centroid) {
+int lcirs_copy_point_amp(void* lc, double* amp, double* centroid, double* causticDist,
+                         double* pairShare) {
   return guarded([&] {
     if (!lc) throw std::invalid_argument("null light-curve object");
     Curve* c = static_cast<Curve*>(lc);
-    if (!c->hasIRS) throw std::runtime_error("no extended-source curve computed yet");
+    if (!c->hasSeeds) throw std::runtime_error("no seed search has run on this object");
     if (amp) std::copy(c->pointAmp.begin(), c->pointAmp.end(), amp);
     if (centroid) std::copy(c->centroid.begin(), c->centroid.end(), centroid);
+    if (causticDist) std::copy(c->causDist.begin(), c->causDist.end(), causticDist);
+    if (pairShare) std::copy(c->pairShare.begin(), c->pairShare.end(), pairShare);
+    return 0;
+  });
+}
+
+int lcirs_set_ring(void* lc, int on) {
+  return guarded([&] {
+    if (!lc) throw std::invalid_argument("null light-curve object");
+    static_cast<Curve*>(lc)->ring = on != 0;
+    return 0;
+  });
+}
+
+int lcirs_copy_ring_amp(void* lc, double* ring, double* centreRoots) {
+  return guarded([&] {
+    if (!lc) throw std::invalid_argument("null light-curve object");
+    Curve* c = static_cast<Curve*>(lc);
+    if (!c->hasSeeds || c->ringAmp.empty())
+      throw std::runtime_error("no seed search with disk samples has run on this object");
+    if (ring) std::copy(c->ringAmp.begin(), c->ringAmp.end(), ring);
+    if (centreRoots)
+      for (size_t i = 0; i < c->centreRoots.size(); i++) {
+        centreRoots[2 * i] = c->centreRoots[i].re;
+        centreRoots[2 * i + 1] = c->centreRoots[i].im;
+      }
+    return 0;
+  });
+}
+
+int lcirs_find_seeds(void* lc, double iniX, double iniY, double finX, double finY) {
+  return guarded([&] {
+    if (!lc) throw std::invalid_argument("null light-curve object");
+    Curve* c = static_cast<Curve*>(lc);
+    Engine& E = engine();
+    std::lock_guard<std::mutex> lock(E.mu);
+    Job job;
+    job.curves = {c};
+    job.starts = {mk(iniX, iniY)};
+    job.ends = {mk(finX, finY)};
+    job.runFill = false;
+    E.run(job);
     return 0;
   });
 }
@@ -854,14 +958,14 @@ int lcirs_copy_ray_counts(void* lc, uint64_t* out) {
 
 int lcirs_copy_flags(void* lc, uint32_t* out) {
   const Curve* c = static_cast<Curve*>(lc);
-  if (!c || !c->hasIRS) return -1;
+  if (!c || !c->hasSeeds) return -1;
   std::copy(c->flags.begin(), c->flags.end(), out);
   return 0;
 }
 
 int lcirs_copy_seeds(void* lc, int* nSeeds, int* pix) {
   const Curve* c = static_cast<Curve*>(lc);
-  if (!c || !c->hasIRS) return -1;
+  if (!c || !c->hasSeeds) return -1;
   std::copy(c->nSeeds.begin(), c->nSeeds.end(), nSeeds);
   std::copy(c->seedPix.begin(), c->seedPix.end(), pix);
   return 0;

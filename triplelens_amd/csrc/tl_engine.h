@@ -41,6 +41,7 @@ struct CurveDev {
   int pointBase;               // first global point index
   int causticBase;             // first caustic sample (6*kCccLen per curve)
   int chainHave;               // root-reuse state at the start of the run
+  int ring;                    // compute the disk-sample magnifications (ring_amp_kernel)
   cx chain[11];
   cx boxMin[6], boxMax[6];
 };
@@ -66,6 +67,10 @@ struct SeedArgs {
   float* cost;        // predicted rays (point-source magnification x pi ppr^2)
   double* ptAmp;      // point-source magnification at every position
   double2* ptCen;     // magnification-weighted centroid of the point images
+  double* ptCaus;     // distance to the nearest caustic sample, negative when the
+                      // position has more than 4 images (inside a caustic)
+  double* ptPair;     // share of the point magnification in its two brightest images
+  cx* ptRoots;        // the 10 roots of each position's own image solve
   uint32_t* flags;
   RootChain* chainOut; // per curve: root-reuse state after the last position
   const RootChain* chainIn;  // per curve: state to start from (nullptr: the curve's own)
@@ -114,6 +119,12 @@ struct FillArgs {
 
 // Host-side launchers (tl_kernels.hip).  All return hipError_t as int.
 int launch_seed(const SeedArgs& a, hipStream_t s);
+// Point magnification at the kRingSamples disk samples around each position
+// of the curves with `ring` set: ring[kRingSamples p + s].
+constexpr int kRingSamples = 12;
+int launch_ring_amp(const CurveDev* curves, const int* ptCurve, const double2* ptPos,
+                    const cx* ptRoots, int nPts, const TrigTable* trig, double* ring,
+                    hipStream_t s);
 int launch_ccc(const Lens* lenses, int n, const CccTable* tab, const TrigTable* trig, cx* cc,
                cx* ca, hipStream_t s);
 int launch_fill(const FillArgs& a, int nWorkgroups, hipStream_t s);

@@ -519,9 +519,37 @@ __global__ void __launch_bounds__(64) seed_kernel(SeedArgs a) {
           sx += rl(wx, k);
           sy += rl(wy, k);
         }
+        // distance to the nearest of the curve's caustic samples
+        double d2 = 1e300;
+        for (int q = ln; q < 6 * kCccLen; q += 64) {
+          const cx dq = csub(caus[q], pos);
+          const double v = cnorm(dq);
+          d2 = v < d2 ? v : d2;
+        }
+        for (int o = 32; o >= 1; o >>= 1) {
+          const double w2 = __shfl_xor(d2, o, 64);
+          d2 = w2 < d2 ? w2 : d2;
+        }
+        // the two largest image magnifications
+        double t1 = term;
+        for (int o = 32; o >= 1; o >>= 1) {
+          const double w2 = __shfl_xor(t1, o, 64);
+          t1 = w2 > t1 ? w2 : t1;
+        }
+        const unsigned long long top = __ballot(term == t1 && ((m >> ln) & 1ull));
+        const int topLane = top ? __builtin_ctzll(top) : -1;
+        double t2 = (ln == topLane) ? 0.0 : term;
+        for (int o = 32; o >= 1; o >>= 1) {
+          const double w2 = __shfl_xor(t2, o, 64);
+          t2 = w2 > t2 ? w2 : t2;
+        }
+        if (ln < 10) a.ptRoots[(size_t)p * 10 + ln] = chainv;
         if (ln == 0) {
           a.ptAmp[p] = amp;
           a.ptCen[p] = make_double2(sx / amp, sy / amp);
+          const double dist = sqrt(d2);
+          a.ptCaus[p] = __builtin_popcountll(m) > 4 ? -dist : dist;
+          a.ptPair[p] = amp > 0.0 ? (t1 + t2) / amp : 0.0;
         }
       }
     }
@@ -573,6 +601,59 @@ int launch_point_lc(const CurveDev* curves, int nCurves, const TrigTable* trig, 
   if (nCurves <= 0) return 0;
   hipLaunchKernelGGL(point_lc_kernel, dim3(nCurves), dim3(64), 0, s, curves, nCurves, trig, out,
                      chainOut);
+  return (int)hipGetLastError();
+}
+
+// Offsets, in source radii, of the disk samples of Gould's (2008, ApJ 681,
+// 1593) hexadecapole approximation: radius 1/2 on the axes, radius 1 on the
+// axes, radius 1 on the diagonals.
+__constant__ double kRingOff[kRingSamples][2] = {
+    {0.5, 0.0}, {0.0, 0.5}, {-0.5, 0.0}, {0.0, -0.5},
+    {1.0, 0.0}, {0.0, 1.0}, {-1.0, 0.0}, {0.0, -1.0},
+    {0.70710678118654752, 0.70710678118654752}, {-0.70710678118654752, 0.70710678118654752},
+    {-0.70710678118654752, -0.70710678118654752}, {0.70710678118654752, -0.70710678118654752}};
+
+// Point magnification (getPointAmp's sum, lcbase.cc:17-38) at the disk
+// samples around each position, one wavefront per position.  Every sample
+// is solved from the position's own centre roots (the reference's
+// root-reuse solve, imgpoint.cc:160-274), so the samples are independent of
+// each other and of the order they are visited in.
+__global__ void __launch_bounds__(64) ring_amp_kernel(const CurveDev* curves, const int* ptCurve,
+                                                      const double2* ptPos, const cx* ptRoots,
+                                                      int nPts, const TrigTable* trig,
+                                                      double* ring) {
+  const int p = blockIdx.x;
+  if (p >= nPts) return;
+  const CurveDev& C = curves[ptCurve[p]];
+  if (!C.ring) return;
+  const int ln = __lane_id();
+  const Lens L = C.L;
+  const dcx Dv = ln < 4 ? C.pb.D[ln < 4 ? ln : 0] : dc_zero();
+  const dcx Sv = ln < 3 ? C.pb.S[ln < 3 ? ln : 0] : dc_zero();
+  const cx centre = ln < 10 ? ptRoots[(size_t)p * 10 + (ln < 10 ? ln : 0)] : mk(0.0, 0.0);
+  const double2 pp = ptPos[p];
+  for (int s = 0; s < kRingSamples; s++) {
+    cx chainv = centre;
+    int have = 1;
+    const cx zeta = mk(pp.x + C.rho * kRingOff[s][0], pp.y + C.rho * kRingOff[s][1]);
+    unsigned long long m = wl_get_images(L, Dv, Sv, chainv, have, zeta, trig);
+    const double term = ((m >> ln) & 1ull) ? point_amp_term(L, chainv) : 0.0;
+    double amp = 0.0;
+    while (m) {
+      const int k = __builtin_ctzll(m);
+      m &= m - 1;
+      amp += rl(term, k);
+    }
+    if (ln == 0) ring[(size_t)p * kRingSamples + s] = amp;
+  }
+}
+
+int launch_ring_amp(const CurveDev* curves, const int* ptCurve, const double2* ptPos,
+                    const cx* ptRoots, int nPts, const TrigTable* trig, double* ring,
+                    hipStream_t s) {
+  if (nPts <= 0) return 0;
+  hipLaunchKernelGGL(ring_amp_kernel, dim3(nPts), dim3(64), 0, s, curves, ptCurve, ptPos, ptRoots,
+                     nPts, trig, ring);
   return (int)hipGetLastError();
 }
 

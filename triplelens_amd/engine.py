@@ -134,14 +134,46 @@ class LightCurve:
         return amp, rays, flags
 
     def point_amp(self):
-        """Point-source magnification and weighted image centroid at the
-        positions of the last lc_irs: (lcLength+1,), (lcLength+1, 2)."""
+        """Point-source magnification, weighted image centroid and signed
+        caustic distance at the positions of the last seed search:
+        (n,), (n, 2), (n,) with n = lcLength+1."""
         n = self.lc_len + 1
         amp = np.zeros(n)
         cen = np.zeros((n, 2))
-        self._lib.lcirs_copy_point_amp.argtypes = [_P, _P, _P]
-        _check(self._lib.lcirs_copy_point_amp(self.h, amp.ctypes.data, cen.ctypes.data))
-        return amp, cen
+        caus = np.zeros(n)
+        self._lib.lcirs_copy_point_amp.argtypes = [_P, _P, _P, _P, _P]
+        _check(self._lib.lcirs_copy_point_amp(self.h, amp.ctypes.data, cen.ctypes.data,
+                                              caus.ctypes.data, None))
+        return amp, cen, caus
+
+    def pair_share(self):
+        """Share of the point magnification in the two brightest images, at
+        the positions of the last seed search."""
+        out = np.zeros(self.lc_len + 1)
+        self._lib.lcirs_copy_point_amp.argtypes = [_P, _P, _P, _P, _P]
+        _check(self._lib.lcirs_copy_point_amp(self.h, None, None, None, out.ctypes.data))
+        return out
+
+    def set_ring(self, on: bool = True):
+        """With on, later seed searches also compute the point magnification
+        at 12 disk samples around each position (lcirs_set_ring)."""
+        self._lib.lcirs_set_ring.argtypes = [_P, ctypes.c_int]
+        _check(self._lib.lcirs_set_ring(self.h, 1 if on else 0))
+
+    def ring_amp(self):
+        """Disk-sample magnifications (n, 12) and each position's own roots
+        (n, 10) of the last seed search, n = lcLength+1."""
+        n = self.lc_len + 1
+        ring = np.zeros((n, 12))
+        roots = np.zeros((n, 10), np.complex128)
+        self._lib.lcirs_copy_ring_amp.argtypes = [_P, _P, _P]
+        _check(self._lib.lcirs_copy_ring_amp(self.h, ring.ctypes.data, roots.ctypes.data))
+        return ring, roots
+
+    def find_seeds(self, sx, sy, ex, ey):
+        """The seed search alone (no fill); see point_amp and seeds."""
+        self._lib.lcirs_find_seeds.argtypes = [_P, _D, _D, _D, _D]
+        _check(self._lib.lcirs_find_seeds(self.h, sx, sy, ex, ey))
 
     def point_lc(self, sx, sy, ex, ey):
         self._lib.get_lc(self.h, sx, sy, ex, ey)

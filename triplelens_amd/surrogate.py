@@ -1,0 +1,355 @@
+"""Surrogate magnification with ray-shooting fallback.
+
+A small network (include/tl_surrogate.h, kernels in csrc/tl_mlp.hip) predicts,
+for a source position, mu = ln(A_finite / A_point) and s = ln(sigma), from
+fourteen features:
+
+    a, b, th                      lens separations and angle
+    log10(m2/m1), log10(m3/m1)    mass ratios
+    log10(rho), u                 source radius and linear limb darkening
+    dx, dy                        offset of the magnification-weighted image
+                                  centroid from the source position
+    d / rho                       signed distance to the nearest caustic
+                                  sample (negative inside a caustic), in
+                                  source radii, clipped to [-1000, 1000]
+    ln A_point                    point-source magnification
+    pair share                    share of A_point in the two brightest
+                                  images (near a fold: the merging pair)
+    h = ln(A_hex / A_point)       Gould's (2008, ApJ 681, 1593) hexadecapole
+                                  estimate of the finite-source
+                                  magnification, from 12 point
+                                  magnifications on the source disk
+                                  (lcirs_set_ring) and the exact disk
+                                  moments of the limb-darkening profile
+    t4                            the hexadecapole term relative to A_point,
+                                  log-compressed: large where the expansion
+                                  in rho breaks down
+
+A_point, the centroid, d, the pair share and the disk samples come from
+the engine's seed search (lcirs_find_seeds), which every position needs
+anyway.  The network's mean output is a correction to h (a skip
+connection): mu = h + the network's first output, so it learns where and by
+how much the expansion in rho fails rather than the expansion itself.  A position is
+computed by inverse ray shooting instead when it lies within `gate_radii`
+source radii of a caustic, when sigma exceeds `sigma_max`, or when the
+prediction is not finite; a random `audit` fraction of the accepted
+positions is ray-shot as well and compared.  The network is trained on
+ray-shooting output the engine generates itself (training_set).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import engine as _E
+
+N_FEATURES = 14
+F_DIST, F_HEX = 9, 12
+_P = ctypes.c_void_p
+
+
+class SurrogateError(RuntimeError):
+    pass
+
+
+def _lib():
+    lib = _E.load()
+    if getattr(lib, "_tl_mlp_declared", False):
+        return lib
+    lib.tl_mlp_new.argtypes = [ctypes.c_ulonglong]
+    lib.tl_mlp_new.restype = _P
+    lib.tl_mlp_delete.argtypes = [_P]
+    lib.tl_mlp_param_count.restype = ctypes.c_long
+    lib.tl_mlp_get_params.argtypes = [_P, _P]
+    lib.tl_mlp_set_params.argtypes = [_P, _P]
+    lib.tl_mlp_set_scaling.argtypes = [_P, _P, _P]
+    lib.tl_mlp_predict.argtypes = [_P, ctypes.c_int, _P, _P]
+    lib.tl_mlp_grads.argtypes = [_P, ctypes.c_int, _P, _P, _P, _P]
+    lib.tl_mlp_train.argtypes = [_P, ctypes.c_int, _P, _P, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_float, ctypes.c_ulonglong, _P]
+    lib.tl_mlp_last_error.restype = ctypes.c_char_p
+    lib._tl_mlp_declared = True
+    return lib
+
+
+def _ok(rc):
+    if rc != 0:
+        raise SurrogateError(_lib().tl_mlp_last_error().decode() or f"surrogate call failed ({rc})")
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, np.float32)
+    return a if shape is None else a.reshape(shape)
+
+
+class Net:
+    """The network on the GPU."""
+
+    def __init__(self, seed: int = 1):
+        lib = _lib()
+        self._lib = lib
+        self.h = lib.tl_mlp_new(seed)
+        if not self.h:
+            raise SurrogateError(lib.tl_mlp_last_error().decode())
+        self.mean = np.zeros(N_FEATURES, np.float32)
+        self.scale = np.ones(N_FEATURES, np.float32)
+        # the network works on standardised targets (y - y_mean) / y_scale, so
+        # that its BF16 activations carry O(1) values; predict() undoes it
+        self.y_mean, self.y_scale = 0.0, 1.0
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            self._lib.tl_mlp_delete(h)
+            self.h = None
+
+    @staticmethod
+    def param_count() -> int:
+        return int(_lib().tl_mlp_param_count())
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.param_count(), np.float32)
+        _ok(self._lib.tl_mlp_get_params(self.h, out.ctypes.data))
+        return out
+
+    def set_params(self, theta):
+        t = _f32(theta)
+        if t.size != self.param_count():
+            raise ValueError("wrong parameter count")
+        _ok(self._lib.tl_mlp_set_params(self.h, t.ctypes.data))
+
+    def set_scaling(self, mean, scale):
+        self.mean, self.scale = _f32(mean).copy(), _f32(scale).copy()
+        _ok(self._lib.tl_mlp_set_scaling(self.h, self.mean.ctypes.data, self.scale.ctypes.data))
+
+    def fit_scaling(self, X, y=None):
+        """Feature standardisation (on the GPU) and, given targets, a robust
+        target standardisation (median and scaled median absolute
+        deviation), so that the rare large targets do not set the scale."""
+        X = np.asarray(X, np.float64)
+        sd = X.std(axis=0)
+        self.set_scaling(X.mean(axis=0), np.where(sd > 0, sd, 1.0))
+        if y is not None:
+            y = np.asarray(y, np.float64)
+            self.y_mean = float(np.median(y))
+            mad = float(np.median(np.abs(y - self.y_mean)))
+            self.y_scale = max(1.4826 * mad, 1e-7)
+
+    def predict_raw(self, X) -> np.ndarray:
+        """The network's outputs for standardised targets."""
+        X = _f32(X, (-1, N_FEATURES))
+        out = np.zeros((len(X), 2), np.float32)
+        _ok(self._lib.tl_mlp_predict(self.h, len(X), X.ctypes.data, out.ctypes.data))
+        return out
+
+    def predict(self, X) -> np.ndarray:
+        """(mu, ln sigma) per row, in the units of the targets."""
+        o = self.predict_raw(X).astype(np.float64)
+        return np.stack([self.y_mean + self.y_scale * o[:, 0],
+                         o[:, 1] + math.log(self.y_scale)], axis=1)
+
+    def grads(self, X, y):
+        """Loss and gradient of the network for raw outputs vs targets y."""
+        X = _f32(X, (-1, N_FEATURES))
+        y = _f32(y)
+        g = np.zeros(self.param_count(), np.float32)
+        loss = ctypes.c_float(0.0)
+        _ok(self._lib.tl_mlp_grads(self.h, len(X), X.ctypes.data, y.ctypes.data, g.ctypes.data,
+                                   ctypes.byref(loss)))
+        return float(loss.value), g
+
+    def train(self, X, y, steps: int, batch: int = 512, lr: float = 1e-3, seed: int = 1):
+        """Adam on the standardised targets; returns the per-step losses."""
+        X = _f32(X, (-1, N_FEATURES))
+        y = _f32((np.asarray(y, np.float64) - self.y_mean) / self.y_scale)
+        losses = np.zeros(steps, np.float32)
+        _ok(self._lib.tl_mlp_train(self.h, len(X), X.ctypes.data, y.ctypes.data, steps, batch,
+                                   lr, seed, losses.ctypes.data))
+        return losses
+
+
+# ---------------------------------------------------------------- features
+def disk_moments(u: float):
+    """<r^2> and <r^4> (in units of rho) over the source disk weighted by the
+    linear limb-darkened profile 1 - u (1 - sqrt(1 - r^2))."""
+    norm = (1.0 - u) + 2.0 * u / 3.0
+    return (((1.0 - u) / 2.0 + u * 4.0 / 15.0) / norm,
+            ((1.0 - u) / 3.0 + u * 16.0 / 105.0) / norm)
+
+
+def hexadecapole(amp, ring, u):
+    """Gould's (2008) expansion of the finite-source magnification from the
+    centre value and the disk samples of lcirs_set_ring: the four-point
+    means at rho/2 (axes), rho (axes) and rho (diagonals) give the rho^2 and
+    rho^4 coefficients a2, a4 of the circle mean of A, and the disk mean is
+    A0 + a2 <r^2> + a4 <r^4>.  Returns (A_hex, a4 <r^4>)."""
+    A0 = np.asarray(amp, np.float64)
+    ring = np.asarray(ring, np.float64)
+    half = ring[:, 0:4].mean(axis=1) - A0
+    full = ring[:, 4:8].mean(axis=1) - A0
+    diag = ring[:, 8:12].mean(axis=1) - A0
+    a2 = (16.0 * half - full) / 3.0
+    a4 = 0.5 * (full + diag) - a2
+    x2, x4 = disk_moments(u)
+    return A0 + a2 * x2 + a4 * x4, a4 * x4
+
+
+def features(lens, rho, u, pos, centroid, caus, amp, pair, ring) -> np.ndarray:
+    """Feature rows for positions pos (n, 2) of one lens (a, b, th, m2, m3)."""
+    a, b, th, m2, m3 = lens
+    m1 = 1.0 - m2 - m3
+    n = len(pos)
+    X = np.empty((n, N_FEATURES))
+    X[:, 0] = a
+    X[:, 1] = b
+    X[:, 2] = th
+    X[:, 3] = math.log10(m2 / m1)
+    X[:, 4] = math.log10(m3 / m1)
+    X[:, 5] = math.log10(rho)
+    X[:, 6] = u
+    X[:, 7] = centroid[:, 0] - pos[:, 0]
+    X[:, 8] = centroid[:, 1] - pos[:, 1]
+    X[:, F_DIST] = np.clip(caus / rho, -1000.0, 1000.0)
+    amp = np.asarray(amp, np.float64)
+    safe = np.maximum(amp, 1e-300)
+    X[:, 10] = np.log(safe)
+    X[:, 11] = pair
+    A_hex, t4 = hexadecapole(amp, ring, u)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        h = np.log(A_hex / safe)
+    X[:, F_HEX] = np.clip(np.nan_to_num(h, nan=5.0, posinf=5.0, neginf=-5.0), -5.0, 5.0)
+    r4 = np.nan_to_num(t4 / safe, nan=1.0, posinf=1.0, neginf=-1.0)
+    X[:, 13] = np.sign(r4) * np.log10(1.0 + np.abs(r4) * 1e6) / 6.0
+    return X
+
+
+def _point_features(lc, c):
+    ap, cen, caus = lc.point_amp()
+    ring, _ = lc.ring_amp()
+    return ap, features(c.lens(), c.rho, c.u, positions(c), cen, caus, ap, lc.pair_share(), ring)
+
+
+def caustic_zoom(c, half_length: float):
+    """Curve c with its trajectory shortened to 2 half_length about its
+    midpoint (for the C2 workload: the central-caustic crossing), so that
+    most positions lie within a few tens of source radii of the caustic."""
+    from .workloads import Curve
+    mx, my = 0.5 * (c.sx + c.ex), 0.5 * (c.sy + c.ey)
+    dx, dy = c.ex - c.sx, c.ey - c.sy
+    nrm = math.hypot(dx, dy)
+    dx, dy = dx / nrm * half_length, dy / nrm * half_length
+    return Curve(c.a, c.b, c.th, c.m2, c.m3, c.rho, c.ppr, c.u, mx - dx, my - dy, mx + dx, my + dy,
+                 c.n_points)
+
+
+def positions(c) -> np.ndarray:
+    t = np.arange(c.n_points) / c.lc_len
+    return np.stack([c.sx + (c.ex - c.sx) * t, c.sy + (c.ey - c.sy) * t], axis=1)
+
+
+def _new_curve(c):
+    lc = _E.LightCurve(*c.lens(), c.rho, c.lc_len, c.ppr)
+    lc.set_limb_darkening("linear", c.u)
+    lc.set_ring(True)
+    return lc
+
+
+def training_set(curves):
+    """Ray-shot curves (one batched engine run) -> X (n, 14), y = ln(A/A_point),
+    and the raw arrays (A_irs, A_point, caustic distance / rho)."""
+    lcs = [_new_curve(c) for c in curves]
+    res = _E.batch_irs(lcs, np.array([c.traj() for c in curves]))
+    Xs, ys, A, Ap, d = [], [], [], [], []
+    for c, lc, (amp, rays, flags) in zip(curves, lcs, res):
+        ap, X = _point_features(lc, c)
+        ok = np.isfinite(ap) & (ap > 0) & (amp > 0)
+        Xs.append(X[ok])
+        ys.append(np.log(amp[ok] / ap[ok]))
+        A.append(amp[ok])
+        Ap.append(ap[ok])
+        d.append(X[ok, F_DIST])
+    return (np.concatenate(Xs), np.concatenate(ys),
+            {"A": np.concatenate(A), "A_point": np.concatenate(Ap), "d": np.concatenate(d)})
+
+
+class Surrogate:
+    """Network plus gate: surrogate where it is trusted, ray shooting
+    elsewhere and on a random audit sample."""
+
+    def __init__(self, net: Net, gate_radii: float = 5.0, sigma_max: float = 3e-4,
+                 audit: float = 0.01):
+        self.net = net
+        self.gate_radii = gate_radii
+        self.sigma_max = sigma_max
+        self.audit = audit
+
+    def fit(self, X, y, steps: int, batch: int = 512, lr: float = 1e-3, seed: int = 1,
+            margin: float = 0.5):
+        """Train the network on the rows the gate can accept (|d| / rho at
+        least gate_radii - margin), for the residual y - h of the
+        hexadecapole estimate.  Returns the per-step losses."""
+        X = np.asarray(X, np.float64)
+        y = np.asarray(y, np.float64)
+        keep = (np.abs(X[:, F_DIST]) >= self.gate_radii - margin) & np.isfinite(y) & \
+            np.isfinite(X).all(axis=1)
+        Xk, rk = X[keep], y[keep] - X[keep, F_HEX]
+        if len(rk) < batch:
+            raise ValueError("fewer trainable rows than one batch")
+        self.net.fit_scaling(Xk, rk)
+        return self.net.train(Xk, rk, steps=steps, batch=batch, lr=lr, seed=seed)
+
+    def calibrate(self, X, y, bound: float = 1e-3, safety: float = 2.0) -> float:
+        """Set sigma_max from calibration rows the network was not trained
+        on: the largest threshold at which every calibration row that the
+        gate accepts (|d| / rho >= gate_radii, sigma <= threshold) is within
+        bound / safety relative error.  Returns sigma_max."""
+        X = np.asarray(X, np.float64)
+        y = np.asarray(y, np.float64)
+        mu, sigma = self.predict(X)
+        err = np.abs(np.expm1(mu - y))
+        cand = (np.abs(X[:, F_DIST]) >= self.gate_radii) & np.isfinite(err) & np.isfinite(sigma)
+        sig, e = sigma[cand], err[cand]
+        order = np.argsort(sig, kind="stable")
+        # the running maximum is non-decreasing, so the rows within the
+        # bound are a prefix of the sigma order
+        worst = np.maximum.accumulate(e[order])
+        k = int(np.count_nonzero(worst <= bound / safety)) - 1
+        self.sigma_max = float(sig[order[k]]) if k >= 0 else 0.0
+        return self.sigma_max
+
+    def predict(self, X):
+        """(mu, sigma) per row: mu = h + the network's correction."""
+        X = np.asarray(X, np.float64)
+        p = self.net.predict(X)
+        return X[:, F_HEX] + p[:, 0], np.exp(np.clip(p[:, 1], -30.0, 30.0))
+
+    def light_curve(self, lc, c, rng=None):
+        """Magnification at the lcLength+1 positions of curve c on engine object
+        lc.  Returns (A, info) with info = fallback, audited masks and the
+        audit's relative errors."""
+        rng = np.random.default_rng(0) if rng is None else rng
+        lc.set_ring(True)
+        lc.find_seeds(*c.traj())
+        ap, X = _point_features(lc, c)
+        pos = positions(c)
+        mu, sigma = self.predict(X)
+        A = ap * np.exp(mu)
+        fallback = (np.abs(X[:, F_DIST]) < self.gate_radii) | (sigma > self.sigma_max) | \
+            ~np.isfinite(A) | ~(ap > 0)
+        audited = ~fallback & (rng.random(len(A)) < self.audit)
+        shoot = fallback | audited
+        errs = np.zeros(0)
+        if shoot.any():
+            ns, pix = lc.seeds()
+            idx = np.nonzero(shoot)[0]
+            A_irs, _, flags = lc.fill_seeds(pos[idx], ns[idx], pix[idx])
+            bad = int(np.bitwise_or.reduce(flags)) & ~_E.FLAG_EDGE
+            if bad:
+                raise SurrogateError(f"ray shooting reported status flags {bad:#x}")
+            full = np.zeros(len(A))
+            full[idx] = A_irs
+            errs = np.abs(A[audited] - full[audited]) / full[audited]
+            A = np.where(fallback, full, A)
+        return A, {"fallback": fallback, "audited": audited, "audit_rel_err": errs}
