@@ -1,20 +1,29 @@
 #!/usr/bin/env python3
-"""Light-curve benchmark of the MI355X IRS engine.
+"""Light-curve benchmark of the MI355X IRS engine: surrogate + IRS against
+plain IRS on the C2 workload.
 
-One step = one extended-source light curve of the C2 workload computed by
-the engine (get_lc_irs through the C ABI: seed search, flood fill and ray
-map on the GPU): a seeded triple lens (separations log-uniform [0.5, 2],
+The C2 workload: a seeded triple lens (separations log-uniform [0.5, 2],
 m2, m3 log-uniform [1e-3, 0.3]), linear limb darkening u = 0.5,
 rho = 5e-4, pixel rho/200, 4096 positions on a straight line of length 2
-through the central caustic.  The printed value is light-curve points per
-second.
+through the central caustic (geometry seed 2, which the surrogate's
+training and calibration curves exclude).
+
+Two timed loops over the same curve, each of --steps light curves:
+  * plain IRS: get_lc_irs through the C ABI (seed search with the
+    reference's single root chain, flood fill and ray map on the GPU);
+  * surrogate + IRS (the default mode): the split seed search with the
+    disk samples, the network on the GPU, and ray shooting of the positions
+    the gate sends back plus a 1% audit (Surrogate.light_curve).
+The printed value is the surrogate loop's light-curve points per second;
+every position of its last curve is compared with the plain-IRS curve and
+the largest relative difference is reported (the bound is 1e-3).  With
+--mode plain only the plain loop runs and its rate is the value.
 
 Roofline figures are those of the fill kernel, where the path's arithmetic
 is (43 FP64 operations per counted ray): its launch durations come from HIP
-events around it on the engine's stream in this run, and its HBM traffic
-per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary under
-profiles/.  The seed search (the serial root chain, one wavefront per curve)
-runs concurrently on a second stream; its time is reported beside it.
+events around it on the engine's stream, over the plain-IRS timed loop, and
+its HBM traffic per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+summary under profiles/.
 
 The CPU baseline is the reference LightCurveIRS compiled from its own
 sources (oracle/_ref, pinned flags -O2 -ffp-contract=off -mavx) run over the
@@ -39,6 +48,7 @@ sys.path.insert(0, HERE)
 FLOP_PER_RAY = 43          # irsSIMD 35 + pixel position 2 + profile and sum 6
 FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 vector/matrix peak
 METRIC = "IRS light-curve points/s at ≤1e-3 rel err: surrogate+IRS vs plain IRS, 1 GPU"
+MODEL = os.path.join(HERE, "triplelens_amd", "data", "surrogate_c2.npz")
 
 
 def c2_curve():
@@ -150,6 +160,8 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--mode", choices=["surrogate", "plain"], default="surrogate")
+    ap.add_argument("--model", default=MODEL, help="trained surrogate (.npz)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-cores", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--calibrate-copy", type=int, metavar="MIB", default=0,
@@ -164,6 +176,10 @@ def main(argv=None):
         return 2
 
     c = c2_curve()
+    if args.mode == "surrogate" and not os.path.exists(args.model):
+        print(f"{args.model} is missing: train it with python -m triplelens_amd.train_surrogate "
+              f"--out {args.model}", file=sys.stderr)
+        return 2
     cpu = None if args.no_cpu else cpu_baseline(c, args.cpu_cores)
 
     from triplelens_amd import engine as E
@@ -176,19 +192,76 @@ def main(argv=None):
     flags_all = 0
     t = time.perf_counter()
     for _ in range(args.steps):
-        _, r, f = lc.lc_irs(*c.traj())
+        A_plain, r, f = lc.lc_irs(*c.traj())
         st = E.last_stats()
         fill_ms += st["fill_ms"]
         seed_ms += st["seed_ms"]
         launches += st["fill_launches"]
         rays += float(r.sum())
         flags_all |= int(np.bitwise_or.reduce(f))
-    wall = time.perf_counter() - t
-    ms_per_step = 1e3 * wall / args.steps
-    value = c.n_points * args.steps / wall
+    wall_plain = time.perf_counter() - t
+    plain_pps = c.n_points * args.steps / wall_plain
     fill_launch_ms = fill_ms / launches
     rays_per_launch = rays / launches
     achieved = FLOP_PER_RAY * rays_per_launch / (fill_launch_ms * 1e-3) / 1e12
+    config = {
+        "workload": "C2: seeded triple lens, rho=5e-4, linear limb darkening u=0.5, "
+                    "pixel rho/200, 4096-point trajectory through the central caustic",
+        "n_points": c.n_points,
+        "geometry_seed": 2,
+        "lens": {"a": c.a, "b": c.b, "th": c.th, "m2": c.m2, "m3": c.m3},
+        "device": name,
+        "cus": cus,
+        "plain_irs_points_per_s": plain_pps,
+        "plain_irs_ms_per_curve": 1e3 * wall_plain / args.steps,
+        "plain_rays_per_curve": rays / args.steps,
+        "plain_status_flags": flags_all,
+        "plain_seed_kernel_ms_per_curve": seed_ms / args.steps,
+        "plain_fill_kernel_ms_per_curve": fill_ms / args.steps,
+        "plain_fill_launches_per_curve": launches / args.steps,
+    }
+    if args.mode == "plain":
+        config["mode"] = "plain IRS (no surrogate)"
+        value, ms_per_step = plain_pps, 1e3 * wall_plain / args.steps
+    else:
+        from triplelens_amd import surrogate as S
+        sur = S.Surrogate.load(args.model)
+        lcs = E.LightCurve(*c.lens(), c.rho, c.lc_len, c.ppr)
+        lcs.set_limb_darkening("linear", c.u)
+        rng = np.random.default_rng(1)
+        for _ in range(args.warmup):
+            sur.light_curve(lcs, c, rng)
+        stats = []
+        shot_rays = n_fb = n_aud = 0
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            A_sur, info = sur.light_curve(lcs, c, rng, stats)
+            shot_rays += info["rays"]
+            n_fb += int(info["fallback"].sum())
+            n_aud += int(info["audited"].sum())
+        wall_sur = time.perf_counter() - t
+        value = c.n_points * args.steps / wall_sur
+        ms_per_step = 1e3 * wall_sur / args.steps
+        rel = np.abs(A_sur / A_plain - 1.0)
+        acc = ~info["fallback"]
+        config.update({
+            "mode": "surrogate + IRS fallback (split seed search, disk samples, BF16 network, "
+                    "gate, 1% audit)",
+            "model": os.path.relpath(args.model, HERE),
+            "gate_radii": sur.gate_radii,
+            "hex_max": sur.hex_max,
+            "band_edges": sur.band_edges.tolist(),
+            "sigma_max": sur.sigma_max.tolist(),
+            "speedup_vs_plain_irs": value / plain_pps,
+            "max_rel_err_vs_plain_irs": float(rel.max()),
+            "max_rel_err_accepted": float(rel[acc].max(initial=0.0)),
+            "bound_met": bool(rel.max() <= 1e-3),
+            "fallback_fraction": n_fb / (args.steps * c.n_points),
+            "audit_fraction": n_aud / (args.steps * c.n_points),
+            "ray_share_shot": shot_rays / rays,
+            "surrogate_seed_kernel_ms_per_curve": sum(s["seed_ms"] for s in stats) / args.steps,
+            "surrogate_fill_kernel_ms_per_curve": sum(s["fill_ms"] for s in stats) / args.steps,
+        })
     line = {
         "metric": METRIC,
         "value": value,
@@ -202,21 +275,7 @@ def main(argv=None):
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {
-            "workload": "C2 plain IRS: seeded triple lens, rho=5e-4, linear limb darkening "
-                        "u=0.5, pixel rho/200, 4096-point trajectory through the central caustic",
-            "n_points": c.n_points,
-            "geometry_seed": 2,
-            "lens": {"a": c.a, "b": c.b, "th": c.th, "m2": c.m2, "m3": c.m3},
-            "mode": "plain IRS (no surrogate)",
-            "device": name,
-            "cus": cus,
-            "rays_per_step": rays / args.steps,
-            "status_flags": flags_all,
-            "seed_kernel_ms_per_step": seed_ms / args.steps,
-            "fill_kernel_ms_per_step": fill_ms / args.steps,
-            "fill_launches_per_step": launches / args.steps,
-        },
+        "config": config,
         "roofline": {
             "bound": "mfma",
             "achieved": achieved,
@@ -227,6 +286,7 @@ def main(argv=None):
             "kernel": "fill_kernel",
             "kernel_ms": fill_launch_ms,
             "flop_per_ray": FLOP_PER_RAY,
+            "timed_loop": "plain IRS",
         },
         "cpu_baseline": cpu,
     }

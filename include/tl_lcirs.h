@@ -94,10 +94,22 @@ int lcirs_copy_point_amp(void* lc, double* amp, double* centroid, double* causti
  * of Gould's (2008) hexadecapole approximation of the finite-source
  * magnification.  No counterpart in the reference. */
 int lcirs_set_ring(void* lc, int on);
-/* The disk-sample magnifications of the last seed search, (lcLength+1) x 12,
- * and (may be NULL) the 10 roots of each position's own solve, (lcLength+1)
- * x 10 complex as re, im pairs. */
-int lcirs_copy_ring_amp(void* lc, double* ring, double* centreRoots);
+/* Split seed search.  With parts = P > 1, the seed search of the object
+ * (every call that runs one) cuts its positions into P contiguous pieces that
+ * run in parallel; the first piece continues the object's root chain, the
+ * others start from a fresh solve.  Faster for long curves (the chain is
+ * serial), but the roots, and at rare ill-conditioned positions the images
+ * and ray counts, no longer follow the reference's single chain
+ * (lcirs.cc:116-130, imgpoint.cc:160-205).  parts = 1 (the default) is the
+ * reference's behaviour.  Piece q holds positions [n q / P, n (q+1) / P) of
+ * the n = lcLength+1 positions.  No counterpart in the reference. */
+int lcirs_set_seed_split(void* lc, int parts);
+/* The disk-sample magnifications of the last seed search, (lcLength+1) x 12;
+ * (may be NULL) the 10 roots of each position's own solve, (lcLength+1) x 10
+ * complex as re, im pairs; and (may be NULL) per position the number of
+ * samples whose image count differs from the centre's (samples across a
+ * caustic from the centre). */
+int lcirs_copy_ring_amp(void* lc, double* ring, double* centreRoots, int* crossings);
 /* The seed search of get_lc_irs alone (lcirs.cc:116-165, no fill): seeds
  * (lcirs_copy_seeds), point magnifications, centroids and caustic distances
  * (lcirs_copy_point_amp), with the root chain advanced as get_lc_irs
@@ -132,6 +144,15 @@ int lcirs_copy_seeds(void* lc, int* nSeeds, int* pix);
  * counts, magnifications and flags per position.  For tests of the fill. */
 int lcirs_fill_seeds(void* lc, int n, const double* pos, const int* nSeeds, const int* pix,
                      uint64_t* rays, double* amp, uint32_t* flags);
+
+/* Ray-shoot positions idx[0..n) (indices 0..lcLength) of the object's last
+ * seed search with that search's seeds, the fill ordered and its tables
+ * sized by the search's predicted rays (point magnification x pi ppr^2) as
+ * in get_lc_irs.  Per position: rays, magnification, status flags.  For
+ * callers that shoot a subset of a curve (a surrogate's fallback).  No
+ * counterpart in the reference. */
+int lcirs_fill_positions(void* lc, int n, const int* idx, uint64_t* rays, double* amp,
+                         uint32_t* flags);
 
 /* Engine counters of the last run: out[0] fill tasks, out[1] evaluated
  * pixels, out[2] runs inserted, out[3] duplicate runs, out[4] seed-kernel

@@ -60,7 +60,7 @@ class Oracle:
         lib.tlo_irs.argtypes = [_P, _D, _D, _D, _D]
         lib.tlo_irs.restype = _D
         lib.tlo_point_amp.argtypes = [_D] * 5 + [ctypes.c_int, _P, _P]
-        lib.tlo_ring_amp.argtypes = [_D] * 6 + [ctypes.c_int, _P, _P, _P]
+        lib.tlo_ring_amp.argtypes = [_D] * 6 + [ctypes.c_int, _P, _P, _P, _P]
         lib.tlo_hypot.argtypes = [_D, _D]
         lib.tlo_hypot.restype = _D
         lib.tlo_csqrt.argtypes = [_D, _D, _P]
@@ -133,14 +133,18 @@ class Oracle:
 
     def ring_amp(self, a, b, th, m2, m3, rho, pos, roots):
         """Point magnification at the 12 disk samples around each position
-        (complex pos (n,)), solved from the given roots (n, 10): (n, 12)."""
+        (complex pos (n,)), solved from the given roots (n, 10): (n, 12), and
+        the samples per position whose image count differs from the
+        centre's (n,)."""
         p = np.ascontiguousarray(pos, np.complex128)
         r = np.ascontiguousarray(roots, np.complex128)
         if r.shape != (len(p), 10):
             raise ValueError("roots must be (n, 10)")
         out = np.zeros((len(p), 12))
-        self.lib.tlo_ring_amp(a, b, th, m2, m3, rho, len(p), _ptr(p), _ptr(r), _ptr(out))
-        return out
+        cross = np.zeros(len(p), np.int32)
+        self.lib.tlo_ring_amp(a, b, th, m2, m3, rho, len(p), _ptr(p), _ptr(r), _ptr(out),
+                              _ptr(cross))
+        return out, cross
 
     def batch(self, params, traj, lens, nthreads):
         """Independent curves over host threads.  params: (n, 8) rows of

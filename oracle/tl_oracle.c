@@ -608,23 +608,33 @@ int tlo_point_amp(double a, double b, double th, double m2, double m3, int n,
  * samples of Gould's 2008 hexadecapole approximation: rho/2 on the axes,
  * rho on the axes, rho on the diagonals), each solved from the given roots
  * of the position's own solve: the root-reuse solve of getRootsPrecalculated
- * (imgpoint.cc:160-205) and the sum of getPointAmp (lcbase.cc:17-38).  No
- * counterpart in the reference; restates the engine's ring_amp_kernel. */
+ * (imgpoint.cc:160-205) and the sum of getPointAmp (lcbase.cc:17-38);
+ * cross[i] (may be NULL) counts the samples whose image count differs from
+ * the centre roots' image count.  No counterpart in the reference; restates
+ * the engine's ring_amp_kernel. */
 int tlo_ring_amp(double a, double b, double th, double m2, double m3, double rho, int n,
-                 const double* pos, const double* roots, double* out) {
+                 const double* pos, const double* roots, double* out, int* cross) {
   static const double c = 0.70710678118654752;
   static const double off[12][2] = {{0.5, 0.0}, {0.0, 0.5}, {-0.5, 0.0}, {0.0, -0.5},
                                     {1.0, 0.0}, {0.0, 1.0}, {-1.0, 0.0}, {0.0, -1.0},
                                     {c, c}, {-c, c}, {-c, -c}, {c, -c}};
   imgpoint_t ip;
   imgpoint_init(&ip, a, b, th, m2, m3);
-  for (int i = 0; i < n; i++)
+  for (int i = 0; i < n; i++) {
+    int n0 = 0;
+    for (int k = 0; k < 10; k++)
+      n0 += img_check(&ip.L, mk(roots[20 * i + 2 * k], roots[20 * i + 2 * k + 1]),
+                      mk(pos[2 * i], pos[2 * i + 1]), 1.0e-6);
+    int nc = 0;
     for (int s = 0; s < 12; s++) {
       for (int k = 0; k < 10; k++) ip.tempRoots[k] = mk(roots[20 * i + 2 * k], roots[20 * i + 2 * k + 1]);
       ip.tempRoots[10] = mk(0.0, 0.0);
       ip.haveRoots = 1;
       out[12 * i + s] = point_amp(&ip, mk(pos[2 * i] + rho * off[s][0], pos[2 * i + 1] + rho * off[s][1]));
+      nc += ip.nImg != n0;
     }
+    if (cross) cross[i] = nc;
+  }
   return 0;
 }
 

@@ -106,9 +106,11 @@ int tlo_hit_mask(const tlo_lc* lc, long x0, long x1, long y0, long y1, double sx
 int tlo_point_amp(double a, double b, double th, double m2, double m3, int n,
                   const double* pos, double* out);
 /* Point magnification at the 12 disk samples around each position, each
- * solved from the position's given 10 roots (re, im pairs): out[12 i + s]. */
+ * solved from the position's given 10 roots (re, im pairs): out[12 i + s];
+ * cross[i] (may be NULL): samples whose image count differs from the
+ * centre's. */
 int tlo_ring_amp(double a, double b, double th, double m2, double m3, double rho, int n,
-                 const double* pos, const double* roots, double* out);
+                 const double* pos, const double* roots, double* out, int* cross);
 
 /* The complex-arithmetic helpers, exposed for tests against libm/libgcc, and
  * the system routines they restate. */

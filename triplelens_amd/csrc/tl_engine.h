@@ -42,6 +42,7 @@ struct CurveDev {
   int causticBase;             // first caustic sample (6*kCccLen per curve)
   int chainHave;               // root-reuse state at the start of the run
   int ring;                    // compute the disk-sample magnifications (ring_amp_kernel)
+  int split;                   // independent pieces of the seed search (1: one chain)
   cx chain[11];
   cx boxMin[6], boxMax[6];
 };
@@ -76,6 +77,7 @@ struct SeedArgs {
   const RootChain* chainIn;  // per curve: state to start from (nullptr: the curve's own)
   int chunk, nChunks;  // positions [n chunk / nChunks, n (chunk+1) / nChunks) of every
                        // curve of n positions
+  int maxSplit;        // workgroups per curve: the largest CurveDev::split of the launch
 };
 
 struct FillArgs {
@@ -120,10 +122,11 @@ struct FillArgs {
 // Host-side launchers (tl_kernels.hip).  All return hipError_t as int.
 int launch_seed(const SeedArgs& a, hipStream_t s);
 // Point magnification at the kRingSamples disk samples around each position
-// of the curves with `ring` set: ring[kRingSamples p + s].
+// of the curves with `ring` set: ring[kRingSamples p + s], and cross[p], the
+// number of samples whose image count differs from the centre's.
 constexpr int kRingSamples = 12;
 int launch_ring_amp(const CurveDev* curves, const int* ptCurve, const double2* ptPos,
-                    const cx* ptRoots, int nPts, const TrigTable* trig, double* ring,
+                    const cx* ptRoots, int nPts, const TrigTable* trig, double* ring, int* cross,
                     hipStream_t s);
 int launch_ccc(const Lens* lenses, int n, const CccTable* tab, const TrigTable* trig, cx* cc,
                cx* ca, hipStream_t s);

@@ -425,17 +425,25 @@ __device__ __forceinline__ void emit_seeds(const CurveDev& C, const cx& chainv,
   cnt = cnt + n > kMaxSeeds ? kMaxSeeds : cnt + n;
 }
 
+// Workgroup (curve c, piece q): a curve with split = 1 walks the chunk's
+// positions as one chain, continuing the curve's root-reuse state.  With
+// split = S > 1 the chunk's positions are cut into S contiguous pieces that
+// run in parallel; piece 0 continues the chain, the others start from a
+// fresh solve.
 __global__ void __launch_bounds__(64) seed_kernel(SeedArgs a) {
-  const int c = blockIdx.x;
+  const int c = blockIdx.x / a.maxSplit, q = blockIdx.x % a.maxSplit;
   if (c >= a.nCurves) return;
   const int ln = __lane_id();
   const CurveDev& C = a.curves[c];
+  if (q >= C.split) return;
   const Lens L = C.L;
   const dcx Dv = ln < 4 ? C.pb.D[ln < 4 ? ln : 0] : dc_zero();
   const dcx Sv = ln < 3 ? C.pb.S[ln < 3 ? ln : 0] : dc_zero();
   cx chainv = mk(0.0, 0.0);
   int have = 0;
-  if (a.chainIn) {
+  if (q > 0) {
+    // fresh solve at the piece's first position
+  } else if (a.chainIn) {
     if (ln <= 10) chainv = a.chainIn[c].tempRoots[ln <= 10 ? ln : 0];
     have = a.chainIn[c].have;
   } else {
@@ -445,8 +453,9 @@ __global__ void __launch_bounds__(64) seed_kernel(SeedArgs a) {
   const cx s = mk(C.sx, C.sy), e = mk(C.ex, C.ey);
   const cx* caus = a.caustics + C.causticBase;
   const long long n = (long long)C.lcLength + 1;
-  const unsigned iBegin = (unsigned)(n * a.chunk / a.nChunks);
-  const unsigned iEnd = (unsigned)(n * (a.chunk + 1) / a.nChunks);
+  const long long cBegin = n * a.chunk / a.nChunks, cEnd = n * (a.chunk + 1) / a.nChunks;
+  const unsigned iBegin = (unsigned)(cBegin + (cEnd - cBegin) * q / C.split);
+  const unsigned iEnd = (unsigned)(cBegin + (cEnd - cBegin) * (q + 1) / C.split);
   for (unsigned i = iBegin; i < iEnd; i++) {
     const int p = C.pointBase + (int)i;
     const cx pos = traj_pos(s, e, i, C.lcLength);
@@ -559,8 +568,10 @@ __global__ void __launch_bounds__(64) seed_kernel(SeedArgs a) {
       a.cost[p] = (float)(amp * 3.141592653589793 * C.ppr * C.ppr);
     }
   }
-  if (ln <= 10) a.chainOut[c].tempRoots[ln] = chainv;
-  if (ln == 0) a.chainOut[c].have = have;
+  if (q == C.split - 1) {
+    if (ln <= 10) a.chainOut[c].tempRoots[ln] = chainv;
+    if (ln == 0) a.chainOut[c].have = have;
+  }
 }
 
 // LightCurveBase::getLC / getPointAmp (lcbase.cc:17-52): positions at
@@ -617,11 +628,12 @@ __constant__ double kRingOff[kRingSamples][2] = {
 // samples around each position, one wavefront per position.  Every sample
 // is solved from the position's own centre roots (the reference's
 // root-reuse solve, imgpoint.cc:160-274), so the samples are independent of
-// each other and of the order they are visited in.
+// each other and of the order they are visited in.  A sample whose image
+// count differs from the centre's lies across a caustic from it.
 __global__ void __launch_bounds__(64) ring_amp_kernel(const CurveDev* curves, const int* ptCurve,
                                                       const double2* ptPos, const cx* ptRoots,
                                                       int nPts, const TrigTable* trig,
-                                                      double* ring) {
+                                                      double* ring, int* cross) {
   const int p = blockIdx.x;
   if (p >= nPts) return;
   const CurveDev& C = curves[ptCurve[p]];
@@ -632,11 +644,15 @@ __global__ void __launch_bounds__(64) ring_amp_kernel(const CurveDev* curves, co
   const dcx Sv = ln < 3 ? C.pb.S[ln < 3 ? ln : 0] : dc_zero();
   const cx centre = ln < 10 ? ptRoots[(size_t)p * 10 + (ln < 10 ? ln : 0)] : mk(0.0, 0.0);
   const double2 pp = ptPos[p];
+  const int n0 = __builtin_popcountll(__ballot(ln < 10 && img_check(L, centre, mk(pp.x, pp.y),
+                                                                     1.0e-6)));
+  int nCross = 0;
   for (int s = 0; s < kRingSamples; s++) {
     cx chainv = centre;
     int have = 1;
     const cx zeta = mk(pp.x + C.rho * kRingOff[s][0], pp.y + C.rho * kRingOff[s][1]);
     unsigned long long m = wl_get_images(L, Dv, Sv, chainv, have, zeta, trig);
+    nCross += __builtin_popcountll(m) != n0 ? 1 : 0;
     const double term = ((m >> ln) & 1ull) ? point_amp_term(L, chainv) : 0.0;
     double amp = 0.0;
     while (m) {
@@ -646,14 +662,15 @@ __global__ void __launch_bounds__(64) ring_amp_kernel(const CurveDev* curves, co
     }
     if (ln == 0) ring[(size_t)p * kRingSamples + s] = amp;
   }
+  if (ln == 0) cross[p] = nCross;
 }
 
 int launch_ring_amp(const CurveDev* curves, const int* ptCurve, const double2* ptPos,
-                    const cx* ptRoots, int nPts, const TrigTable* trig, double* ring,
+                    const cx* ptRoots, int nPts, const TrigTable* trig, double* ring, int* cross,
                     hipStream_t s) {
   if (nPts <= 0) return 0;
   hipLaunchKernelGGL(ring_amp_kernel, dim3(nPts), dim3(64), 0, s, curves, ptCurve, ptPos, ptRoots,
-                     nPts, trig, ring);
+                     nPts, trig, ring, cross);
   return (int)hipGetLastError();
 }
 
@@ -690,7 +707,9 @@ int launch_ccc(const Lens* lenses, int n, const CccTable* tab, const TrigTable* 
 
 int launch_seed(const SeedArgs& a, hipStream_t s) {
   if (a.nCurves <= 0) return 0;
-  hipLaunchKernelGGL(seed_kernel, dim3(a.nCurves), dim3(64), 0, s, a);
+  if (a.maxSplit < 1 || (long long)a.nCurves * a.maxSplit > (1ll << 30))
+    return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(seed_kernel, dim3(a.nCurves * a.maxSplit), dim3(64), 0, s, a);
   return (int)hipGetLastError();
 }
 

@@ -160,15 +160,24 @@ class LightCurve:
         self._lib.lcirs_set_ring.argtypes = [_P, ctypes.c_int]
         _check(self._lib.lcirs_set_ring(self.h, 1 if on else 0))
 
+    def set_seed_split(self, parts: int):
+        """Run the seed search in `parts` independent pieces
+        (lcirs_set_seed_split); 1 restores the single chain."""
+        self._lib.lcirs_set_seed_split.argtypes = [_P, ctypes.c_int]
+        _check(self._lib.lcirs_set_seed_split(self.h, int(parts)))
+
     def ring_amp(self):
-        """Disk-sample magnifications (n, 12) and each position's own roots
-        (n, 10) of the last seed search, n = lcLength+1."""
+        """Disk-sample magnifications (n, 12), each position's own roots
+        (n, 10) and the number of samples across a caustic from the centre
+        (n,) of the last seed search, n = lcLength+1."""
         n = self.lc_len + 1
         ring = np.zeros((n, 12))
         roots = np.zeros((n, 10), np.complex128)
-        self._lib.lcirs_copy_ring_amp.argtypes = [_P, _P, _P]
-        _check(self._lib.lcirs_copy_ring_amp(self.h, ring.ctypes.data, roots.ctypes.data))
-        return ring, roots
+        cross = np.zeros(n, np.int32)
+        self._lib.lcirs_copy_ring_amp.argtypes = [_P, _P, _P, _P]
+        _check(self._lib.lcirs_copy_ring_amp(self.h, ring.ctypes.data, roots.ctypes.data,
+                                             cross.ctypes.data))
+        return ring, roots, cross
 
     def find_seeds(self, sx, sy, ex, ey):
         """The seed search alone (no fill); see point_amp and seeds."""
@@ -217,6 +226,21 @@ class LightCurve:
         _check(self._lib.lcirs_fill_seeds(self.h, n, pos.ctypes.data, nseeds.ctypes.data,
                                           pix.ctypes.data, rays.ctypes.data, amp.ctypes.data,
                                           flags.ctypes.data))
+        return amp, rays, flags
+
+    def fill_positions(self, idx):
+        """GPU fill of positions idx (indices 0..lcLength) of the last seed
+        search with its seeds (lcirs_fill_positions): amp, rays, flags."""
+        idx = np.ascontiguousarray(idx, np.int32)
+        n = len(idx)
+        rays = np.zeros(n, np.uint64)
+        amp = np.zeros(n)
+        flags = np.zeros(n, np.uint32)
+        if n == 0:
+            return amp, rays, flags
+        self._lib.lcirs_fill_positions.argtypes = [_P, ctypes.c_int, _P, _P, _P, _P]
+        _check(self._lib.lcirs_fill_positions(self.h, n, idx.ctypes.data, rays.ctypes.data,
+                                              amp.ctypes.data, flags.ctypes.data))
         return amp, rays, flags
 
 

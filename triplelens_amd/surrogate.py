@@ -129,8 +129,12 @@ class Net:
         target standardisation (median and scaled median absolute
         deviation), so that the rare large targets do not set the scale."""
         X = np.asarray(X, np.float64)
-        sd = X.std(axis=0)
-        self.set_scaling(X.mean(axis=0), np.where(sd > 0, sd, 1.0))
+        mean, sd = X.mean(axis=0), X.std(axis=0)
+        # a column that is constant up to rounding (rho of a fixed-rho
+        # workload) keeps scale 1: dividing its rounding noise by a ~1e-16
+        # deviation would feed the network large meaningless inputs
+        const = sd <= 1e-9 * np.maximum(1.0, np.abs(mean))
+        self.set_scaling(mean, np.where(const, 1.0, sd))
         if y is not None:
             y = np.asarray(y, np.float64)
             self.y_mean = float(np.median(y))
@@ -225,10 +229,18 @@ def features(lens, rho, u, pos, centroid, caus, amp, pair, ring) -> np.ndarray:
     return X
 
 
+def hex_term(X) -> np.ndarray:
+    """|hexadecapole term| / A_point per row, from the compressed feature."""
+    return (10.0 ** (6.0 * np.abs(np.asarray(X)[:, 13])) - 1.0) * 1e-6
+
+
 def _point_features(lc, c):
+    """A_point, the feature rows and the disk samples' caustic crossings of
+    the last seed search of lc over curve c."""
     ap, cen, caus = lc.point_amp()
-    ring, _ = lc.ring_amp()
-    return ap, features(c.lens(), c.rho, c.u, positions(c), cen, caus, ap, lc.pair_share(), ring)
+    ring, _, cross = lc.ring_amp()
+    X = features(c.lens(), c.rho, c.u, positions(c), cen, caus, ap, lc.pair_share(), ring)
+    return ap, X, cross
 
 
 def caustic_zoom(c, half_length: float):
@@ -258,66 +270,129 @@ def _new_curve(c):
 
 def training_set(curves):
     """Ray-shot curves (one batched engine run) -> X (n, 14), y = ln(A/A_point),
-    and the raw arrays (A_irs, A_point, caustic distance / rho)."""
+    and the raw arrays (A_irs, A_point, caustic distance / rho, disk samples
+    across a caustic)."""
     lcs = [_new_curve(c) for c in curves]
     res = _E.batch_irs(lcs, np.array([c.traj() for c in curves]))
-    Xs, ys, A, Ap, d = [], [], [], [], []
+    Xs, ys, A, Ap, d, cr = [], [], [], [], [], []
     for c, lc, (amp, rays, flags) in zip(curves, lcs, res):
-        ap, X = _point_features(lc, c)
+        ap, X, cross = _point_features(lc, c)
         ok = np.isfinite(ap) & (ap > 0) & (amp > 0)
+        cr.append(cross[ok])
         Xs.append(X[ok])
         ys.append(np.log(amp[ok] / ap[ok]))
         A.append(amp[ok])
         Ap.append(ap[ok])
         d.append(X[ok, F_DIST])
     return (np.concatenate(Xs), np.concatenate(ys),
-            {"A": np.concatenate(A), "A_point": np.concatenate(Ap), "d": np.concatenate(d)})
+            {"A": np.concatenate(A), "A_point": np.concatenate(Ap), "d": np.concatenate(d),
+             "cross": np.concatenate(cr)})
 
 
 class Surrogate:
     """Network plus gate: surrogate where it is trusted, ray shooting
-    elsewhere and on a random audit sample."""
+    elsewhere and on a random audit sample.  The gate accepts a position
+    when it lies at least gate_radii source radii from the caustic samples,
+    no disk sample lies across a caustic from it, the hexadecapole term is
+    at most hex_max of A_point (the expansion in rho has converged), and the
+    network's sigma is at most the sigma_max of the position's distance band
+    (bands split at band_edges source radii).  Near the caustics sigma has
+    to separate the positions where the expansion fails; far from them it
+    mostly reflects the ray-shooting noise of low magnifications, so each
+    band gets its own calibrated threshold."""
 
-    def __init__(self, net: Net, gate_radii: float = 5.0, sigma_max: float = 3e-4,
-                 audit: float = 0.01):
+    def __init__(self, net: Net, gate_radii: float = 2.0, sigma_max=3e-5,
+                 audit: float = 0.01, seed_piece: int | None = 64, hex_max: float = 1e-3,
+                 band_edges=(5.0,)):
         self.net = net
         self.gate_radii = gate_radii
-        self.sigma_max = sigma_max
+        self.hex_max = hex_max
+        self.band_edges = np.asarray(band_edges, np.float64)
+        self.sigma_max = np.broadcast_to(np.asarray(sigma_max, np.float64),
+                                         (len(self.band_edges) + 1,)).copy()
         self.audit = audit
+        # positions per independent piece of the seed search
+        # (lcirs_set_seed_split); None keeps the reference's single chain
+        self.seed_piece = seed_piece
 
-    def fit(self, X, y, steps: int, batch: int = 512, lr: float = 1e-3, seed: int = 1,
-            margin: float = 0.5):
-        """Train the network on the rows the gate can accept (|d| / rho at
-        least gate_radii - margin), for the residual y - h of the
-        hexadecapole estimate.  Returns the per-step losses."""
+    def seed_parts(self, n_positions: int) -> int:
+        if not self.seed_piece:
+            return 1
+        return max(1, min(65536, n_positions // self.seed_piece))
+
+    def trainable(self, X, y, cross, margin: float = 0.5):
+        """Rows near what the gate can accept (|d| / rho at least
+        gate_radii - margin, no disk sample across a caustic, hexadecapole
+        term at most 3 hex_max): the rows to train on."""
+        return (np.abs(X[:, F_DIST]) >= self.gate_radii - margin) & (np.asarray(cross) == 0) & \
+            (hex_term(X) <= 3.0 * self.hex_max) & np.isfinite(y) & np.isfinite(X).all(axis=1)
+
+    def fit(self, X, y, cross, steps: int, batch: int = 512, lr: float = 1e-3, seed: int = 1):
+        """Train the network on the trainable rows, for the residual y - h of
+        the hexadecapole estimate.  Returns the per-step losses."""
         X = np.asarray(X, np.float64)
         y = np.asarray(y, np.float64)
-        keep = (np.abs(X[:, F_DIST]) >= self.gate_radii - margin) & np.isfinite(y) & \
-            np.isfinite(X).all(axis=1)
+        keep = self.trainable(X, y, cross)
         Xk, rk = X[keep], y[keep] - X[keep, F_HEX]
         if len(rk) < batch:
             raise ValueError("fewer trainable rows than one batch")
         self.net.fit_scaling(Xk, rk)
         return self.net.train(Xk, rk, steps=steps, batch=batch, lr=lr, seed=seed)
 
-    def calibrate(self, X, y, bound: float = 1e-3, safety: float = 2.0) -> float:
-        """Set sigma_max from calibration rows the network was not trained
-        on: the largest threshold at which every calibration row that the
-        gate accepts (|d| / rho >= gate_radii, sigma <= threshold) is within
+    def save(self, path: str):
+        """Weights, scalings and gate in one .npz file."""
+        n = self.net
+        np.savez(path, params=n.params(), mean=n.mean, scale=n.scale,
+                 y=np.array([n.y_mean, n.y_scale]),
+                 gate=np.array([self.gate_radii, self.audit, float(self.seed_piece or 0),
+                                self.hex_max]),
+                 band_edges=self.band_edges, sigma_max=self.sigma_max)
+
+    @classmethod
+    def load(cls, path: str) -> "Surrogate":
+        z = np.load(path, allow_pickle=False)
+        net = Net()
+        net.set_params(z["params"])
+        net.set_scaling(z["mean"], z["scale"])
+        net.y_mean, net.y_scale = float(z["y"][0]), float(z["y"][1])
+        g = z["gate"]
+        return cls(net, gate_radii=float(g[0]), sigma_max=z["sigma_max"], audit=float(g[1]),
+                   seed_piece=int(g[2]) or None, hex_max=float(g[3]),
+                   band_edges=z["band_edges"])
+
+    def band(self, X) -> np.ndarray:
+        return np.searchsorted(self.band_edges, np.abs(np.asarray(X)[:, F_DIST]), side="right")
+
+    def candidates(self, X, cross) -> np.ndarray:
+        """Rows the gate considers (everything but sigma)."""
+        return (np.abs(X[:, F_DIST]) >= self.gate_radii) & (np.asarray(cross) == 0) & \
+            (hex_term(X) <= self.hex_max)
+
+    def accepts(self, X, cross, sigma) -> np.ndarray:
+        """The gate's rows (before the audit)."""
+        return self.candidates(X, cross) & (sigma <= self.sigma_max[self.band(X)])
+
+    def calibrate(self, X, y, cross, bound: float = 1e-3, safety: float = 2.0) -> np.ndarray:
+        """Set sigma_max, per distance band, from calibration rows the
+        network was not trained on: the largest threshold at which every
+        calibration row of the band that the gate accepts is within
         bound / safety relative error.  Returns sigma_max."""
         X = np.asarray(X, np.float64)
         y = np.asarray(y, np.float64)
         mu, sigma = self.predict(X)
         err = np.abs(np.expm1(mu - y))
-        cand = (np.abs(X[:, F_DIST]) >= self.gate_radii) & np.isfinite(err) & np.isfinite(sigma)
-        sig, e = sigma[cand], err[cand]
-        order = np.argsort(sig, kind="stable")
-        # the running maximum is non-decreasing, so the rows within the
-        # bound are a prefix of the sigma order
-        worst = np.maximum.accumulate(e[order])
-        k = int(np.count_nonzero(worst <= bound / safety)) - 1
-        self.sigma_max = float(sig[order[k]]) if k >= 0 else 0.0
-        return self.sigma_max
+        cand = self.candidates(X, cross) & np.isfinite(err) & np.isfinite(sigma)
+        bands = self.band(X)
+        for b in range(len(self.sigma_max)):
+            sel = cand & (bands == b)
+            sig, e = sigma[sel], err[sel]
+            order = np.argsort(sig, kind="stable")
+            # the running maximum is non-decreasing, so the rows within the
+            # bound are a prefix of the sigma order
+            worst = np.maximum.accumulate(e[order])
+            k = int(np.count_nonzero(worst <= bound / safety)) - 1
+            self.sigma_max[b] = float(sig[order[k]]) if k >= 0 else 0.0
+        return self.sigma_max.copy()
 
     def predict(self, X):
         """(mu, sigma) per row: mu = h + the network's correction."""
@@ -325,26 +400,29 @@ class Surrogate:
         p = self.net.predict(X)
         return X[:, F_HEX] + p[:, 0], np.exp(np.clip(p[:, 1], -30.0, 30.0))
 
-    def light_curve(self, lc, c, rng=None):
+    def light_curve(self, lc, c, rng=None, stats=None):
         """Magnification at the lcLength+1 positions of curve c on engine object
         lc.  Returns (A, info) with info = fallback, audited masks and the
-        audit's relative errors."""
+        audit's relative errors.  With a list `stats`, the engine statistics
+        (engine.last_stats) of each engine run are appended to it."""
         rng = np.random.default_rng(0) if rng is None else rng
         lc.set_ring(True)
+        lc.set_seed_split(self.seed_parts(c.n_points))
         lc.find_seeds(*c.traj())
-        ap, X = _point_features(lc, c)
-        pos = positions(c)
+        if stats is not None:
+            stats.append(_E.last_stats())
+        ap, X, cross = _point_features(lc, c)
         mu, sigma = self.predict(X)
         A = ap * np.exp(mu)
-        fallback = (np.abs(X[:, F_DIST]) < self.gate_radii) | (sigma > self.sigma_max) | \
-            ~np.isfinite(A) | ~(ap > 0)
+        fallback = ~self.accepts(X, cross, sigma) | ~np.isfinite(A) | ~(ap > 0)
         audited = ~fallback & (rng.random(len(A)) < self.audit)
         shoot = fallback | audited
         errs = np.zeros(0)
         if shoot.any():
-            ns, pix = lc.seeds()
             idx = np.nonzero(shoot)[0]
-            A_irs, _, flags = lc.fill_seeds(pos[idx], ns[idx], pix[idx])
+            A_irs, rays, flags = lc.fill_positions(idx)
+            if stats is not None:
+                stats.append(_E.last_stats())
             bad = int(np.bitwise_or.reduce(flags)) & ~_E.FLAG_EDGE
             if bad:
                 raise SurrogateError(f"ray shooting reported status flags {bad:#x}")
@@ -352,4 +430,8 @@ class Surrogate:
             full[idx] = A_irs
             errs = np.abs(A[audited] - full[audited]) / full[audited]
             A = np.where(fallback, full, A)
-        return A, {"fallback": fallback, "audited": audited, "audit_rel_err": errs}
+            info_rays = int(rays.sum())
+        else:
+            info_rays = 0
+        return A, {"fallback": fallback, "audited": audited, "audit_rel_err": errs,
+                   "rays": info_rays}
