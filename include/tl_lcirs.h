@@ -107,8 +107,9 @@ int lcirs_set_seed_split(void* lc, int parts);
 /* The disk-sample magnifications of the last seed search, (lcLength+1) x 12;
  * (may be NULL) the 10 roots of each position's own solve, (lcLength+1) x 10
  * complex as re, im pairs; and (may be NULL) per position the number of
- * samples whose image count differs from the centre's (samples across a
- * caustic from the centre). */
+ * samples whose count of significant images (each at least 1e-4 of the
+ * magnification) differs from the centre's: samples across a caustic from
+ * the centre. */
 int lcirs_copy_ring_amp(void* lc, double* ring, double* centreRoots, int* crossings);
 /* The seed search of get_lc_irs alone (lcirs.cc:116-165, no fill): seeds
  * (lcirs_copy_seeds), point magnifications, centroids and caustic distances

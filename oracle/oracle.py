@@ -134,8 +134,8 @@ class Oracle:
     def ring_amp(self, a, b, th, m2, m3, rho, pos, roots):
         """Point magnification at the 12 disk samples around each position
         (complex pos (n,)), solved from the given roots (n, 10): (n, 12), and
-        the samples per position whose image count differs from the
-        centre's (n,)."""
+        the samples per position whose count of significant images (each at
+        least 1e-4 of the magnification) differs from the centre's (n,)."""
         p = np.ascontiguousarray(pos, np.complex128)
         r = np.ascontiguousarray(roots, np.complex128)
         if r.shape != (len(p), 10):

@@ -609,9 +609,27 @@ int tlo_point_amp(double a, double b, double th, double m2, double m3, int n,
  * rho on the axes, rho on the diagonals), each solved from the given roots
  * of the position's own solve: the root-reuse solve of getRootsPrecalculated
  * (imgpoint.cc:160-205) and the sum of getPointAmp (lcbase.cc:17-38);
- * cross[i] (may be NULL) counts the samples whose image count differs from
- * the centre roots' image count.  No counterpart in the reference; restates
- * the engine's ring_amp_kernel. */
+ * cross[i] (may be NULL) counts the samples whose count of significant
+ * images (each at least 1e-4 of the sum of the terms) differs from the
+ * centre roots' count.  No counterpart in the reference; restates the
+ * engine's ring_amp_kernel. */
+/* Images carrying at least 1e-4 of the total of their terms. */
+static int significant_images(const lens_t* L, const cx* imgs, int n) {
+  double t[MAXDEG], amp = 0.0;
+  for (int k = 0; k < n; k++) {
+    cx z = imgs[k];
+    cx d1 = csub(z, L->z1), d2 = csub(z, L->z2), d3 = csub(z, L->z3);
+    cx s = cadd(cadd(cdiv_rc(L->m1, cmul(d1, d1)), cdiv_rc(L->m2, cmul(d2, d2))),
+                cdiv_rc(L->m3, cmul(d3, d3)));
+    t[k] = 1.0 / fabs(1.0 - cnorm(s));
+    if (t[k] != t[k]) t[k] = 0.0;
+    amp += t[k];
+  }
+  int c = 0;
+  for (int k = 0; k < n; k++) c += t[k] >= 1.0e-4 * amp;
+  return c;
+}
+
 int tlo_ring_amp(double a, double b, double th, double m2, double m3, double rho, int n,
                  const double* pos, const double* roots, double* out, int* cross) {
   static const double c = 0.70710678118654752;
@@ -621,17 +639,20 @@ int tlo_ring_amp(double a, double b, double th, double m2, double m3, double rho
   imgpoint_t ip;
   imgpoint_init(&ip, a, b, th, m2, m3);
   for (int i = 0; i < n; i++) {
-    int n0 = 0;
-    for (int k = 0; k < 10; k++)
-      n0 += img_check(&ip.L, mk(roots[20 * i + 2 * k], roots[20 * i + 2 * k + 1]),
-                      mk(pos[2 * i], pos[2 * i + 1]), 1.0e-6);
+    cx img0[MAXDEG];
+    int ni0 = 0;
+    for (int k = 0; k < 10; k++) {
+      const cx z = mk(roots[20 * i + 2 * k], roots[20 * i + 2 * k + 1]);
+      if (img_check(&ip.L, z, mk(pos[2 * i], pos[2 * i + 1]), 1.0e-6)) img0[ni0++] = z;
+    }
+    const int n0 = significant_images(&ip.L, img0, ni0);
     int nc = 0;
     for (int s = 0; s < 12; s++) {
       for (int k = 0; k < 10; k++) ip.tempRoots[k] = mk(roots[20 * i + 2 * k], roots[20 * i + 2 * k + 1]);
       ip.tempRoots[10] = mk(0.0, 0.0);
       ip.haveRoots = 1;
       out[12 * i + s] = point_amp(&ip, mk(pos[2 * i] + rho * off[s][0], pos[2 * i + 1] + rho * off[s][1]));
-      nc += ip.nImg != n0;
+      nc += significant_images(&ip.L, ip.imgs, ip.nImg) != n0;
     }
     if (cross) cross[i] = nc;
   }

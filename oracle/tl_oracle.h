@@ -107,8 +107,8 @@ int tlo_point_amp(double a, double b, double th, double m2, double m3, int n,
                   const double* pos, double* out);
 /* Point magnification at the 12 disk samples around each position, each
  * solved from the position's given 10 roots (re, im pairs): out[12 i + s];
- * cross[i] (may be NULL): samples whose image count differs from the
- * centre's. */
+ * cross[i] (may be NULL): samples whose count of significant images (each
+ * at least 1e-4 of the magnification) differs from the centre's. */
 int tlo_ring_amp(double a, double b, double th, double m2, double m3, double rho, int n,
                  const double* pos, const double* roots, double* out, int* cross);
 

@@ -84,7 +84,7 @@ def test_oracle_ring_amp_matches_independent_images(oracle):
     idx = [0, 40, 120, 199]
     t = np.array(idx) / c.lc_len
     pos = (c.sx + (c.ex - c.sx) * t) + 1j * (c.sy + (c.ey - c.sy) * t)
-    roots, _, nimg = oracle.images(*lens, pos)
+    roots, _, _ = oracle.images(*lens, pos)
     ring, cross = oracle.ring_amp(*lens, c.rho, pos, roots)
     cc = math.sqrt(0.5)
     off = np.array([0.5, 0.5j, -0.5, -0.5j, 1, 1j, -1, -1j,
@@ -92,8 +92,7 @@ def test_oracle_ring_amp_matches_independent_images(oracle):
     for i, p in enumerate(pos):
         want = np.array([_numpy_point_amp(lens, p + c.rho * o) for o in off])
         np.testing.assert_allclose(ring[i], want, rtol=1e-9)
-        _, _, ns = oracle.images(*lens, p + c.rho * off)
-        assert cross[i] == np.count_nonzero(ns != nimg[i])
+        assert cross[i] == 0  # these positions are far from every caustic
 
 
 def test_oracle_ring_crossings_at_a_caustic(oracle):
@@ -111,6 +110,29 @@ def test_oracle_ring_crossings_at_a_caustic(oracle):
     far = np.array([3.0 + 3.0j, -3.0 - 2.0j])
     roots, _, _ = oracle.images(*lens, far)
     _, cross = oracle.ring_amp(*lens, 1e-3, far, roots)
+    assert np.all(cross == 0)
+
+
+def test_oracle_ring_ignores_a_flickering_faint_image(oracle):
+    """Far from the caustics of the C2 geometry with seed 2 (hundreds of
+    source radii), a faint image (magnification ~4e-6) next to a lens has a
+    lens-equation residual at the 1e-6 acceptance threshold and is accepted
+    at some disk samples but not at the centre: the image counts differ, the
+    counts of significant images do not."""
+    from triplelens_amd import workloads as W
+    c = W.c2(seed=2, ca_fn=lambda *g: oracle.ccc(*g)[1])
+    t = np.array([2, 4, 5, 7, 8]) / c.lc_len
+    pos = (c.sx + (c.ex - c.sx) * t) + 1j * (c.sy + (c.ey - c.sy) * t)
+    roots, _, nimg = oracle.images(*c.lens(), pos)
+    ring, cross = oracle.ring_amp(*c.lens(), c.rho, pos, roots)
+    cc = math.sqrt(0.5)
+    off = np.array([0.5, 0.5j, -0.5, -0.5j, 1, 1j, -1, -1j,
+                    cc + cc * 1j, -cc + cc * 1j, -cc - cc * 1j, cc - cc * 1j])
+    flicker = 0
+    for i, p in enumerate(pos):
+        _, _, ns = oracle.images(*c.lens(), p + c.rho * off)
+        flicker += np.count_nonzero(ns != nimg[i])
+    assert flicker > 0
     assert np.all(cross == 0)
 
 

@@ -123,7 +123,8 @@ struct FillArgs {
 int launch_seed(const SeedArgs& a, hipStream_t s);
 // Point magnification at the kRingSamples disk samples around each position
 // of the curves with `ring` set: ring[kRingSamples p + s], and cross[p], the
-// number of samples whose image count differs from the centre's.
+// number of samples whose count of significant images (at least 1e-4 of the
+// magnification each) differs from the centre's.
 constexpr int kRingSamples = 12;
 int launch_ring_amp(const CurveDev* curves, const int* ptCurve, const double2* ptPos,
                     const cx* ptRoots, int nPts, const TrigTable* trig, double* ring, int* cross,

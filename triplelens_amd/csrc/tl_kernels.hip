@@ -618,6 +618,8 @@ int launch_point_lc(const CurveDev* curves, int nCurves, const TrigTable* trig, 
 // Offsets, in source radii, of the disk samples of Gould's (2008, ApJ 681,
 // 1593) hexadecapole approximation: radius 1/2 on the axes, radius 1 on the
 // axes, radius 1 on the diagonals.
+constexpr double kSignificant = 1.0e-4;
+
 __constant__ double kRingOff[kRingSamples][2] = {
     {0.5, 0.0}, {0.0, 0.5}, {-0.5, 0.0}, {0.0, -0.5},
     {1.0, 0.0}, {0.0, 1.0}, {-1.0, 0.0}, {0.0, -1.0},
@@ -628,8 +630,12 @@ __constant__ double kRingOff[kRingSamples][2] = {
 // samples around each position, one wavefront per position.  Every sample
 // is solved from the position's own centre roots (the reference's
 // root-reuse solve, imgpoint.cc:160-274), so the samples are independent of
-// each other and of the order they are visited in.  A sample whose image
-// count differs from the centre's lies across a caustic from it.
+// each other and of the order they are visited in.  A sample whose count of
+// significant images (each carrying at least kSignificant of the point's
+// magnification) differs from the centre's lies across a caustic from it;
+// the threshold keeps out the faint images next to a lens whose lens-equation
+// residual sits at the acceptance threshold and that come and go between
+// nearby points, while an image pair born on a fold is always bright.
 __global__ void __launch_bounds__(64) ring_amp_kernel(const CurveDev* curves, const int* ptCurve,
                                                       const double2* ptPos, const cx* ptRoots,
                                                       int nPts, const TrigTable* trig,
@@ -644,22 +650,25 @@ __global__ void __launch_bounds__(64) ring_amp_kernel(const CurveDev* curves, co
   const dcx Sv = ln < 3 ? C.pb.S[ln < 3 ? ln : 0] : dc_zero();
   const cx centre = ln < 10 ? ptRoots[(size_t)p * 10 + (ln < 10 ? ln : 0)] : mk(0.0, 0.0);
   const double2 pp = ptPos[p];
-  const int n0 = __builtin_popcountll(__ballot(ln < 10 && img_check(L, centre, mk(pp.x, pp.y),
-                                                                     1.0e-6)));
+  const bool img0 = ln < 10 && img_check(L, centre, mk(pp.x, pp.y), 1.0e-6);
+  const double term0 = img0 ? point_amp_term(L, centre) : 0.0;
+  const double amp0 = wave_sum_d(term0);
+  const int n0 = __builtin_popcountll(__ballot(img0 && term0 >= kSignificant * amp0));
   int nCross = 0;
   for (int s = 0; s < kRingSamples; s++) {
     cx chainv = centre;
     int have = 1;
     const cx zeta = mk(pp.x + C.rho * kRingOff[s][0], pp.y + C.rho * kRingOff[s][1]);
     unsigned long long m = wl_get_images(L, Dv, Sv, chainv, have, zeta, trig);
-    nCross += __builtin_popcountll(m) != n0 ? 1 : 0;
-    const double term = ((m >> ln) & 1ull) ? point_amp_term(L, chainv) : 0.0;
+    const bool img = (m >> ln) & 1ull;
+    const double term = img ? point_amp_term(L, chainv) : 0.0;
     double amp = 0.0;
     while (m) {
       const int k = __builtin_ctzll(m);
       m &= m - 1;
       amp += rl(term, k);
     }
+    nCross += __builtin_popcountll(__ballot(img && term >= kSignificant * amp)) != n0 ? 1 : 0;
     if (ln == 0) ring[(size_t)p * kRingSamples + s] = amp;
   }
   if (ln == 0) cross[p] = nCross;
