@@ -160,8 +160,9 @@ int lcirs_fill_positions(void* lc, int n, const int* idx, uint64_t* rays, double
  * milliseconds, out[5] fill milliseconds (sum over launches), out[6] fill
  * launches, out[7] total milliseconds of the last run, out[8] summed busy
  * time of the fill's wavefronts (ms), out[9] of that, time in run-table
- * inserts (ms), out[10] idle polls, out[11] wavefronts of the last launch.
- * out must hold 12 values. */
+ * inserts (ms), out[10] idle polls, out[11] wavefronts of the last launch,
+ * out[12] scout tasks, out[13] runs found by scouts.  out must hold 14
+ * values. */
 int tl_last_stats(double* out);
 
 /* Critical curves and caustics of n geometries on the GPU (one lane each):

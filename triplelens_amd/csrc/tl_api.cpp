@@ -206,7 +206,8 @@ struct Options {
 
 struct Stats {
   double tasks = 0, evals = 0, ins = 0, dup = 0, seedMs = 0, fillMs = 0, fillLaunches = 0,
-         totalMs = 0, busyMs = 0, insertMs = 0, idlePolls = 0, waves = 0;
+         totalMs = 0, busyMs = 0, insertMs = 0, idlePolls = 0, waves = 0, scouts = 0,
+         scoutRuns = 0;
 };
 
 // One recorded run of the amoeba print-out: row, leftmost and rightmost pixel.
@@ -613,7 +614,7 @@ void Engine::run(Job& job) {
       std::vector<unsigned long long> sync(7 * kLine);
       HIPCHK(hipMemcpy(sync.data(), dSync.p, sync.size() * sizeof(unsigned long long),
                        hipMemcpyDeviceToHost));
-      const unsigned long long* st = sync.data() + 6 * kLine - 2;  // st[2..9] = stats[0..7]
+      const unsigned long long* st = sync.data() + 6 * kLine - 2;  // st[2..11] = stats[0..9]
       const bool aborted = *reinterpret_cast<const int*>(sync.data() + 5 * kLine) != 0;
       std::vector<uint32_t> bflags(nPts);
       HIPCHK(hipMemcpy(bflags.data(), dFlags.p, nPts * sizeof(uint32_t), hipMemcpyDeviceToHost));
@@ -631,6 +632,8 @@ void Engine::run(Job& job) {
       stats.insertMs += (double)st[7] * 1e-5;
       stats.idlePolls += (double)st[8];
       stats.waves = (double)st[9];
+      stats.scouts += (double)st[10];
+      stats.scoutRuns += (double)st[11];
       if (aborted) {
         for (int b = 0; b < nb; b++) flags[batchPts[b]] |= kFlagTimeout;
         throw std::runtime_error("fill kernel hit its time limit");
@@ -1198,6 +1201,7 @@ int tl_last_stats(double* out) {
   out[0] = s.tasks; out[1] = s.evals; out[2] = s.ins; out[3] = s.dup;
   out[4] = s.seedMs; out[5] = s.fillMs; out[6] = s.fillLaunches; out[7] = s.totalMs;
   out[8] = s.busyMs; out[9] = s.insertMs; out[10] = s.idlePolls; out[11] = s.waves;
+  out[12] = s.scouts; out[13] = s.scoutRuns;
   return 0;
 }
 

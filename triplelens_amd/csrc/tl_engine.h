@@ -113,7 +113,8 @@ struct FillArgs {
   uint32_t* flags;            // per global point
   int* abortFlag;
   unsigned long long* stats;  // [0] tasks, [1] evaluated pixels, [2] runs inserted, [3] duplicates,
-                              // [4] busy ticks, [5] insert ticks, [6] idle polls, [7] waves
+                              // [4] busy ticks, [5] insert ticks, [6] idle polls, [7] waves,
+                              // [8] scout tasks, [9] runs found by scouts
   long long timeLimitTicks;   // s_memrealtime ticks (100 MHz)
   int scoutRows;              // rows swept before a scout is sent (0 = no scouts)
   int scoutJump;              // rows a scout jumps ahead

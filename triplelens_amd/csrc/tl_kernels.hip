@@ -995,6 +995,7 @@ struct TaskState {
   long long accSum;         // the label changes or the sweep ends
   unsigned long long accCnt;
   unsigned long long evals, nIns, nDup, insTicks;
+  unsigned long long nScout, nScoutRuns;  // scout tasks taken, runs they found
   // this wave's ring of waiting tasks in LDS: lcount tasks from lhead on
   Task* ring;
   int lhead, lcount;
@@ -1053,6 +1054,8 @@ struct Followups {
   int n;
 };
 constexpr int kQueueOnly = 8;  // kd flag: never kept by the wave
+constexpr int kShare = 16;     // kd flag: to the shared queue at once (the wave's
+                               // ring when the queue is full)
 
 __device__ __forceinline__ void fu_add(Followups& F, const Task& f) {
   if (F.n == 0) F.f0 = f;
@@ -1093,6 +1096,7 @@ __device__ __forceinline__ void handle_run(TaskState& S, int xl, int xr, long lo
     return;
   }
   S.nIns++;
+  S.nScoutRuns += S.kind == 2 ? 1 : 0;
   if (xl <= 1 || xr >= S.N - 2) flag_point(A, S.gp, kFlagEdge);
   if (label != S.accLabel) {
     flush_acc(S);
@@ -1104,9 +1108,33 @@ __device__ __forceinline__ void handle_run(TaskState& S, int xl, int xr, long lo
   Followups F;
   F.n = 0;
   const int d = S.dir;  // direction of travel
-  if (S.kind == 1 || S.kind == 2) {
+  if (S.kind == 1) {
     if (row + 1 < N) fu_add(F, make_task(t.b, row + 1, xl, xr, label, 0, 1, center, 0));
     if (row - 1 >= 0) fu_add(F, make_task(t.b, row - 1, xl, xr, label, 0, -1, center, 0));
+  } else if (S.kind == 2) {
+    // A scout landed (t.f1: the column of the run it was sent from, one jump
+    // back).  The wave sends the next scout straight on at once, with the
+    // shift measured over the last jump, and keeps it as its next step, so
+    // that the scouts run ahead one task per jump; the two sweeps of the new
+    // class go to the shared queue for idle waves.  They start as if they
+    // had just sent a scout.
+    const long long rt = (long long)row + (long long)d * A.scoutJump;
+    if (A.scoutRows > 0 && rt >= 0 && rt < N) {
+      const long long shift = (long long)center - t.f1;
+      const long long pc = center + shift;
+      const long long ash = shift < 0 ? -shift : shift;
+      const long long hw = ((long long)xr - xl) / 2 + ash / 4 + 32;
+      long long sa = pc - hw, sb = pc + hw;
+      if (sa < 0) sa = 0;
+      if (sb > N - 1) sb = N - 1;
+      if (sa <= sb) fu_add(F, make_task(t.b, (int)rt, (int)sa, (int)sb, -1, 2, d, center, 0));
+    }
+    Task up = make_task(t.b, row + 1, xl, xr, label, 0, 1, center, -1);
+    Task dn = make_task(t.b, row - 1, xl, xr, label, 0, -1, center, -1);
+    up.kd |= kShare;
+    dn.kd |= kShare;
+    if (row + 1 < N) fu_add(F, up);
+    if (row - 1 >= 0) fu_add(F, dn);
   } else if (S.kind == 3) {
     // a run found looking back: sweep on away from the run it was reached
     // from, and look back towards that run's row where this one reaches past it
@@ -1138,8 +1166,8 @@ __device__ __forceinline__ void handle_run(TaskState& S, int xl, int xr, long lo
           if (sa < 0) sa = 0;
           if (sb > N - 1) sb = N - 1;
           if (sa <= sb) {
-            Task f = make_task(t.b, (int)rt, (int)sa, (int)sb, -1, 2, 0, 0, 0);
-            f.kd |= kQueueOnly;
+            Task f = make_task(t.b, (int)rt, (int)sa, (int)sb, -1, 2, d, center, 0);
+            f.kd |= kQueueOnly | kShare;
             fu_add(F, f);
           }
         }
@@ -1171,12 +1199,12 @@ __device__ __forceinline__ void handle_run(TaskState& S, int xl, int xr, long lo
   for (int k = 0; k < 3; k++) {
     if (k >= F.n) break;
     Task f = k == 0 ? F.f0 : (k == 1 ? F.f1 : F.f2);
-    const bool queueOnly = (f.kd & kQueueOnly) != 0;
-    f.kd &= ~kQueueOnly;
-    if (!queueOnly && !S.haveCont) {
+    const bool queueOnly = (f.kd & kQueueOnly) != 0, share = (f.kd & kShare) != 0;
+    f.kd &= ~(kQueueOnly | kShare);
+    if (!queueOnly && !share && !S.haveCont) {
       S.cont = f;
       S.haveCont = true;
-    } else {
+    } else if (!(share && q_push(A, f))) {
       ring_push(S, f);
     }
   }
@@ -1287,6 +1315,7 @@ __device__ __forceinline__ void process_task(TaskState& S) {
   S.t.f1 = uni(S.t.f1);
   S.t.f2 = uni(S.t.f2);
   S.kind = S.t.kd & 3;
+  S.nScout += S.kind == 2 ? 1 : 0;
   S.dir = (S.t.kd & 4) ? -1 : 1;
   if (S.t.b != S.curB) {
     // constants of this source position, kept while the wave stays on it
@@ -1371,6 +1400,8 @@ __global__ void __launch_bounds__(kFillThreads) fill_kernel(FillArgs A) {
   S.nIns = 0;
   S.nDup = 0;
   S.insTicks = 0;
+  S.nScout = 0;
+  S.nScoutRuns = 0;
   S.accLabel = -1;
   S.accSum = 0;
   S.accCnt = 0;
@@ -1493,6 +1524,8 @@ __global__ void __launch_bounds__(kFillThreads) fill_kernel(FillArgs A) {
     atomicAdd(&A.stats[5], S.insTicks);
     atomicAdd(&A.stats[6], idlePolls);
     atomicAdd(&A.stats[7], 1ull);
+    atomicAdd(&A.stats[8], S.nScout);
+    atomicAdd(&A.stats[9], S.nScoutRuns);
   }
 }
 
@@ -1553,7 +1586,7 @@ __global__ void fill_reset_kernel(FillArgs A) {
     *A.seedNext = 0;
     *A.labelNext = A.nBatch;
     *A.abortFlag = 0;
-    for (int k = 0; k < 8; k++) A.stats[k] = 0;
+    for (int k = 0; k < 10; k++) A.stats[k] = 0;
   }
 }
 

@@ -82,10 +82,10 @@ def set_option(name: str, value: float) -> None:
 
 
 def last_stats() -> dict:
-    out = np.zeros(12)
+    out = np.zeros(14)
     load().tl_last_stats(out.ctypes.data)
     keys = ["tasks", "evals", "runs", "dups", "seed_ms", "fill_ms", "fill_launches", "total_ms",
-            "busy_ms", "insert_ms", "idle_polls", "waves"]
+            "busy_ms", "insert_ms", "idle_polls", "waves", "scouts", "scout_runs"]
     return dict(zip(keys, out.tolist()))
 
 
