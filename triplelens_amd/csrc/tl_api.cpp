@@ -199,6 +199,10 @@ struct Options {
   int queueLog2 = 21;
   int scoutRows = 8;
   int scoutJump = 256;
+  int pollBackoff = 6;
+  int shareIdle = 0;
+  double runPixels = 128.0;  // predicted rays per run: sizes the run table
+  double tableLoad = 4.0;   // run-table slots per predicted run
   double timeLimitS = 60.0;
   int seedChunks = 3;       // the seed search is cut into this many chunks of positions
   int seedChunkMin = 64;    // ... of at least this many positions per curve
@@ -539,7 +543,7 @@ void Engine::run(Job& job) {
     double predicted = 0;
     size_t b1 = b0;
     while (b1 < order.size() && b1 - b0 < (size_t)kMaxBatch) {
-      const double pc = std::max(256.0, (double)cost[order[b1]] / 64.0);
+      const double pc = std::max(256.0, (double)cost[order[b1]] / opt.runPixels);
       if (b1 > b0 && predicted + pc > kRunBudget) break;
       predicted += pc;
       b1++;
@@ -560,7 +564,7 @@ void Engine::run(Job& job) {
     HIPCHK(hipMemcpyAsync(dSeedTasks.p, seedTasks.data(), seedTasks.size() * sizeof(int2),
                           hipMemcpyHostToDevice, stream));
     unsigned long long hcap = 1ull << 16;
-    while ((double)hcap < 4.0 * predicted && hcap < (1ull << 28)) hcap <<= 1;
+    while ((double)hcap < opt.tableLoad * predicted && hcap < (1ull << 28)) hcap <<= 1;
     int labelCap = nb + std::max(1 << 16, 16 * nb);
     for (int attempt = 0; attempt < 4; attempt++) {
       dHtab.ensure(2 * hcap);
@@ -601,6 +605,8 @@ void Engine::run(Job& job) {
       fa.timeLimitTicks = (long long)(opt.timeLimitS * 1.0e8);
       fa.scoutRows = opt.scoutRows;
       fa.scoutJump = opt.scoutJump;
+      fa.pollBackoff = opt.pollBackoff;
+      fa.shareIdle = opt.shareIdle;
       HIPCHK((hipError_t)launch_fill_reset(fa, stream));
       HIPCHK((hipError_t)launch_clear_u64(dPtCount.p, nb, 0, stream));
       HIPCHK((hipError_t)launch_clear_u64((unsigned long long*)dPtSum.p, nb, 0, stream));
@@ -1214,6 +1220,10 @@ int tl_set_option(const char* name, double value) {
   else if (n == "queue_log2") o.queueLog2 = std::min(28, std::max(10, (int)value));
   else if (n == "scout_rows") o.scoutRows = std::max(0, (int)value);
   else if (n == "scout_jump") o.scoutJump = std::max(1, (int)value);
+  else if (n == "poll_backoff") o.pollBackoff = std::min(6, std::max(0, (int)value));
+  else if (n == "share_idle") o.shareIdle = value != 0.0 ? 1 : 0;
+  else if (n == "run_pixels") o.runPixels = std::min(65536.0, std::max(1.0, value));
+  else if (n == "table_load") o.tableLoad = std::min(64.0, std::max(1.5, value));
   else if (n == "time_limit_s") o.timeLimitS = std::max(0.1, value);
   else if (n == "seed_chunks") o.seedChunks = std::max(1, (int)value);
   else if (n == "seed_chunk_min") o.seedChunkMin = std::max(1, (int)value);

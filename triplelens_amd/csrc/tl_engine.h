@@ -118,6 +118,8 @@ struct FillArgs {
   long long timeLimitTicks;   // s_memrealtime ticks (100 MHz)
   int scoutRows;              // rows swept before a scout is sent (0 = no scouts)
   int scoutJump;              // rows a scout jumps ahead
+  int pollBackoff;            // idle polls sleep up to 2^pollBackoff x 1024 cycles
+  int shareIdle;              // while some wave is idle, queue follow-ups in the shared ring
 };
 
 // Host-side launchers (tl_kernels.hip).  All return hipError_t as int.

@@ -989,6 +989,7 @@ struct TaskState {
                             // the parent, kind 3: the child)
   Task cont;                // follow-up kept by this wave (the next sweep step)
   bool haveCont;
+  bool share;               // some wave was idle at the last look: queue follow-ups
   int nBack;
   LookBack back0, back1;
   int accLabel;             // totals of new runs, credited to accLabel when
@@ -1046,21 +1047,24 @@ __device__ __forceinline__ Task make_task(int b, int row, int xa, int xb, int la
   return f;
 }
 
-// Follow-ups of one run, at most three, kept in named slots and dispatched
+// Follow-ups of one run, at most four, kept in named slots and dispatched
 // from one place: the first one that may stay with the wave becomes its next
 // step, the others are queued.
 struct Followups {
-  Task f0, f1, f2;
+  Task f0, f1, f2, f3;
   int n;
 };
 constexpr int kQueueOnly = 8;  // kd flag: never kept by the wave
 constexpr int kShare = 16;     // kd flag: to the shared queue at once (the wave's
                                // ring when the queue is full)
+constexpr int kLeapEvery = 4;  // a scout chain sends a leap scout every this many scouts
+constexpr int kLeap = 8;       // ... this many jumps ahead
 
 __device__ __forceinline__ void fu_add(Followups& F, const Task& f) {
   if (F.n == 0) F.f0 = f;
   else if (F.n == 1) F.f1 = f;
-  else F.f2 = f;
+  else if (F.n == 2) F.f2 = f;
+  else F.f3 = f;
   F.n++;
 }
 
@@ -1113,21 +1117,35 @@ __device__ __forceinline__ void handle_run(TaskState& S, int xl, int xr, long lo
     if (row - 1 >= 0) fu_add(F, make_task(t.b, row - 1, xl, xr, label, 0, -1, center, 0));
   } else if (S.kind == 2) {
     // A scout landed (t.f1: the column of the run it was sent from, one jump
-    // back).  The wave sends the next scout straight on at once, with the
-    // shift measured over the last jump, and keeps it as its next step, so
-    // that the scouts run ahead one task per jump; the two sweeps of the new
-    // class go to the shared queue for idle waves.  They start as if they
-    // had just sent a scout.
-    const long long rt = (long long)row + (long long)d * A.scoutJump;
-    if (A.scoutRows > 0 && rt >= 0 && rt < N) {
-      const long long shift = (long long)center - t.f1;
-      const long long pc = center + shift;
-      const long long ash = shift < 0 ? -shift : shift;
-      const long long hw = ((long long)xr - xl) / 2 + ash / 4 + 32;
+    // back; t.f2: scouts in its chain since the last leap).  The wave sends
+    // the next scout straight on at once, with the shift measured over the
+    // last jump, and keeps it as its next step, so that the scouts run ahead
+    // one task per jump; every kLeapEvery-th scout also sends a leap scout
+    // kLeap jumps ahead to the shared queue, which starts a chain of its
+    // own, so that long images get many chains.  The two sweeps of the new
+    // class go to the shared queue for idle waves; they start as if they had
+    // just sent a scout.
+    const long long shift = (long long)center - t.f1;
+    const long long ash = shift < 0 ? -shift : shift;
+    for (int leap = 0; leap < 2 && A.scoutRows > 0; leap++) {
+      if (leap == 1 && (t.f2 + 1) % kLeapEvery != 0) break;
+      const long long k = leap ? kLeap : 1;
+      const long long rt = (long long)row + (long long)d * A.scoutJump * k;
+      if (rt < 0 || rt >= N) continue;
+      const long long pc = center + shift * k;
+      const long long hw = ((long long)xr - xl) / 2 + ash * k / 4 + 32 * k;
       long long sa = pc - hw, sb = pc + hw;
       if (sa < 0) sa = 0;
       if (sb > N - 1) sb = N - 1;
-      if (sa <= sb) fu_add(F, make_task(t.b, (int)rt, (int)sa, (int)sb, -1, 2, d, center, 0));
+      if (sa > sb) continue;
+      Task f = make_task(t.b, (int)rt, (int)sa, (int)sb, -1, 2, d, center, leap ? 0 : t.f2 + 1);
+      if (leap) {
+        // the leap's chain measures its next shift from the column expected
+        // one jump before its landing row
+        f.f1 = (int)(center + shift * (kLeap - 1));
+        f.kd |= kQueueOnly | kShare;
+      }
+      fu_add(F, f);
     }
     Task up = make_task(t.b, row + 1, xl, xr, label, 0, 1, center, -1);
     Task dn = make_task(t.b, row - 1, xl, xr, label, 0, -1, center, -1);
@@ -1196,15 +1214,15 @@ __device__ __forceinline__ void handle_run(TaskState& S, int xl, int xr, long lo
       }
     }
   }
-  for (int k = 0; k < 3; k++) {
+  for (int k = 0; k < 4; k++) {
     if (k >= F.n) break;
-    Task f = k == 0 ? F.f0 : (k == 1 ? F.f1 : F.f2);
+    Task f = k == 0 ? F.f0 : (k == 1 ? F.f1 : (k == 2 ? F.f2 : F.f3));
     const bool queueOnly = (f.kd & kQueueOnly) != 0, share = (f.kd & kShare) != 0;
     f.kd &= ~(kQueueOnly | kShare);
     if (!queueOnly && !share && !S.haveCont) {
       S.cont = f;
       S.haveCont = true;
-    } else if (!(share && q_push(A, f))) {
+    } else if (!((share || S.share) && q_push(A, f))) {
       ring_push(S, f);
     }
   }
@@ -1408,6 +1426,7 @@ __global__ void __launch_bounds__(kFillThreads) fill_kernel(FillArgs A) {
   S.ring = rings[threadIdx.x / 64];
   S.lhead = 0;
   S.lcount = 0;
+  S.share = false;
   unsigned long long nTasks = 0, busyTicks = 0, idlePolls = 0;
   bool have = false;
   while (true) {
@@ -1444,7 +1463,7 @@ __global__ void __launch_bounds__(kFillThreads) fill_kernel(FillArgs A) {
           // poll at most every ~30 us once idle for a while, reading the
           // active count and the abort flag only every fourth time
           idlePolls++;
-          if (backoff < 6) backoff++;
+          if (backoff < A.pollBackoff) backoff++;
           for (int k = 0; k < (1 << backoff); k++) __builtin_amdgcn_s_sleep(16);
           const bool full = (idlePolls & 3) == 0;
           int act = 1, abortNow = 0;
@@ -1496,12 +1515,13 @@ __global__ void __launch_bounds__(kFillThreads) fill_kernel(FillArgs A) {
       flush_acc(S);
       have = false;
     }
-    if ((nTasks & 7) == 0 && S.lcount > 0) {
+    if ((nTasks & 7) == 0 && (S.lcount > 0 || A.shareIdle)) {
       // hand the oldest waiting tasks to idle waves
       int act = 0;
       if (ln == 0) act = aldi(A.active);
       act = bcast0(act);
       int give = A.nWaves - act;
+      S.share = A.shareIdle && give > 0;
       if (give > 4) give = 4;
       for (int k = 0; k < give && S.lcount > 0; k++)
         if (!ring_donate(S)) break;
