@@ -26,8 +26,9 @@ extern "C" {
 /* LightCurveIRS constructor (lcirs.cc:517-536).  Lens a, b, th, m2, m3 in the
  * reference frame (z1 = 0, z2 = a, z3 = b e^{i th}), source radius, lcLength
  * and pointsPerRadius.  Builds the critical curves, caustic boxes and
- * image-plane grid on the host.  Returns NULL on invalid input (the reference
- * has no such check). */
+ * image-plane grid on the host.  m3 = 0 is the reference's binary lens
+ * (lcirs.cc:29-36): irsBinary and the binary image-count rule.  Returns NULL
+ * on invalid input (the reference has no such check). */
 void* lcirs_new(double a, double b, double th, double m2, double m3, double sourceSize,
                 unsigned lcLength, long pointsPerRadius);
 

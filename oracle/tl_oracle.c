@@ -477,6 +477,7 @@ int tlo_coeffs(double a, double b, double th, double m2, double m3,
 
 typedef struct {
   lens_t L;
+  int binary;               /* the binary image-count rule (a, b or m3 zero) */
   cx tempRoots[MAXDEG + 1]; /* [10] is the slot read past the end */
   int haveRoots;
   cx roots[MAXDEG];
@@ -485,9 +486,18 @@ typedef struct {
   int nImg;
 } imgpoint_t;
 
+/* ImgPoint::ImgPoint (imgpoint.cc:57-67): a binary lens when a, b or m3 is 0 */
 static void imgpoint_init(imgpoint_t* ip, double a, double b, double th, double m2, double m3) {
   memset(ip, 0, sizeof(*ip));
   lens_init(&ip->L, a, b, th, m2, m3);
+  ip->binary = a == 0.0 || b == 0.0 || m3 == 0.0;
+}
+
+/* A valid image count (imgpoint.cc:236-237): even and at least 4, or for
+ * the binary lens odd and at least 3. */
+static int img_count_ok(const imgpoint_t* ip) {
+  return ip->binary ? (ip->nImg % 2 == 1 && ip->nImg >= 3)
+                    : (ip->nImg % 2 == 0 && ip->nImg >= 4);
 }
 
 /* ImgPoint::getRootsPrecalculated (imgpoint.cc:160-205) */
@@ -553,7 +563,7 @@ static int img_get(imgpoint_t* ip, cx zeta) {
   g_work[0]++;
   img_roots(ip, zeta, 0);
   img_classify(ip, zeta);
-  if (ip->nImg % 2 != 0 || ip->nImg < 4) {
+  if (!img_count_ok(ip)) {
     /* the chained solve gave an invalid image count: also a lost root */
     g_margin = 0.0;
     img_roots(ip, zeta, 1);
@@ -666,7 +676,9 @@ int tlo_ring_amp(double a, double b, double th, double m2, double m3, double rho
 /* CriticalCurveCaustic::getCC + getCa for a triple lens (ccc.cc:42-98,
  * 160-203).  The coefficient expressions keep the reference's operation
  * order so that the critical curve, and with it the image-plane grid, is the
- * same to the last bit. */
+ * same to the last bit.  The light curve constructs its critical curves
+ * from a LensPar (lcirs.cc:16), which leaves the binary flag unset
+ * (ccc.cc:28-39): this form also serves m3 = 0. */
 static void ccc_compute(const lens_t* L, int len, cx* cc, cx* ca) {
   const cx z2 = L->z2, z3 = L->z3;
   const double m2 = L->m2, m3 = L->m3;
@@ -870,8 +882,26 @@ static inline double irs_ray(const lens_t* L, double x, double y, double sx, dou
   return (r0 + r1) * (r0 + r1) + (i0 + i1) * (i0 + i1);
 }
 
+/* irsBinary (lcirs.cc:345-353): img - m1/conj(img-z1) - m2/conj(img-z2)
+ * with libgcc's complex division, then std::norm (x*x + y*y) of the
+ * difference from the source. */
+static inline double irs_binary(const lens_t* L, double x, double y, double sx, double sy) {
+  cx img = mk(x, y);
+  cx t = csub(img, cdiv_rc(L->m1, cconj(csub(img, L->z1))));
+  t = csub(t, cdiv_rc(L->m2, cconj(csub(img, L->z2))));
+  cx d = csub(t, mk(sx, sy));
+  return d.re * d.re + d.im * d.im;
+}
+
+/* The ray test of the light-curve object: irsSIMD, or irsBinary for m3 = 0
+ * (the dispatch of lcirs.cc:29-36). */
+static inline double lc_irs(const tlo_lc* lc, double x, double y, double sx, double sy) {
+  return lc->ip.L.m3 == 0.0 ? irs_binary(&lc->ip.L, x, y, sx, sy)
+                            : irs_ray(&lc->ip.L, x, y, sx, sy);
+}
+
 double tlo_irs(const tlo_lc* lc, double x, double y, double sx, double sy) {
-  return irs_ray(&lc->ip.L, x, y, sx, sy);
+  return lc_irs(lc, x, y, sx, sy);
 }
 
 int tlo_hit_mask(const tlo_lc* lc, long x0, long x1, long y0, long y1, double sx, double sy,
@@ -881,7 +911,7 @@ int tlo_hit_mask(const tlo_lc* lc, long x0, long x1, long y0, long y1, double sx
   for (long j = y0; j <= y1; j++) {
     const double y = ny_to_y(lc, j);
     for (long i = x0; i <= x1; i++)
-      out[(j - y0) * w + (i - x0)] = irs_ray(&lc->ip.L, nx_to_x(lc, i), y, sx, sy) < lc->rho2;
+      out[(j - y0) * w + (i - x0)] = lc_irs(lc, nx_to_x(lc, i), y, sx, sy) < lc->rho2;
   }
   return 0;
 }
@@ -972,7 +1002,7 @@ typedef struct {
 
 static inline int ray_hit(fillacc_t* acc, long nx, double y) {
   const tlo_lc* lc = acc->lc;
-  double rsq = irs_ray(&lc->ip.L, nx_to_x(lc, nx), y, acc->sx, acc->sy);
+  double rsq = lc_irs(lc, nx_to_x(lc, nx), y, acc->sx, acc->sy);
   if (!(rsq < lc->rho2)) return 0;
   rsq = rsq / lc->rho2;
   acc->sum += lc->ldA + lc->ldB * sqrt(1.0 - rsq);

@@ -22,6 +22,11 @@ Fixtures:
   amoeba_c1.npz   the reference's amoeba print-out (setAmoebaPrintOut) for
                   five C1 points (20..24 of 199): every (ny, xleft, xright)
                   run per point, sorted, and the parameter file's text.
+  binary.npz      the binary lens (m3 = 0): C1 with masses (0.8, 0.2) and a
+                  limb-darkened planetary binary through its central caustic
+                  (workloads.binary_c1, binary_planet): grid, critical
+                  curves and caustics (the light curve's: the triple-lens
+                  form with m3 = 0), per-point rays, magnifications, seeds.
 
 Usage: make_golden.py [name ...] writes only the named fixtures.
 """
@@ -80,6 +85,25 @@ def amoeba_fixture(R):
     return int(off[-1])
 
 
+def binary_fixture(R):
+    curves = [W.binary_c1(), W.binary_planet(lambda *g: R.ccc(*g)[1])]
+    rows, amps, raysl, grids, ccs, cas, offs, flats, nimgs = [], [], [], [], [], [], [], [], []
+    for c in curves:
+        amp, rays, grid = curve_fixture(R, c)
+        xy, ns, ni = R.seeds(*c.lens(), c.rho, c.lc_len, c.ppr, *c.traj(), max_seeds=64)
+        off, flat = pack_seeds(xy, ns)
+        cc, ca = R.ccc(*c.lens())
+        rows.append([*c.lens(), c.rho, c.ppr, c.u, *c.traj(), c.n_points])
+        amps.append(amp); raysl.append(rays); grids.append(grid)
+        ccs.append(cc); cas.append(ca); offs.append(off); flats.append(flat)
+        nimgs.append(ni)
+    np.savez_compressed(os.path.join(HERE, "binary.npz"), params=np.array(rows),
+                        amp=np.array(amps), rays=np.array(raysl), grid=np.array(grids),
+                        cc=np.array(ccs), ca=np.array(cas), seed_off=np.array(offs),
+                        seed_xy0=flats[0], seed_xy1=flats[1], n_img=np.array(nimgs))
+    return [int(r.sum()) for r in raysl]
+
+
 def main(only=()):
     orc.build(ref=True)
     R = orc.Reference()
@@ -89,6 +113,8 @@ def main(only=()):
     if only:
         if "amoeba_c1" in only:
             out["amoeba_c1"] = amoeba_fixture(R)
+        if "binary" in only:
+            out["binary"] = binary_fixture(R)
         print(out)
         return
 
@@ -177,6 +203,7 @@ def main(only=()):
     pa = R.point_amp(*c.lens(), pos)
     np.savez_compressed(os.path.join(HERE, "point_amp.npz"), pos=pos, amp=pa)
     out["amoeba_c1"] = amoeba_fixture(R)
+    out["binary"] = binary_fixture(R)
     print(out)
 
 

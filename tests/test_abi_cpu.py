@@ -62,8 +62,22 @@ def test_invalid_input_returns_null_with_error(lib):
     from triplelens_amd import engine as E
     with pytest.raises(E.EngineError, match="sourceSize"):
         E.LightCurve(1.0, 1.0, 1.0, 0.1, 0.1, 0.0, 10, 50)
-    with pytest.raises(E.EngineError, match="binary"):
-        E.LightCurve(1.0, 1.0, 1.0, 0.1, 0.0, 1e-3, 10, 50)
+    with pytest.raises(E.EngineError, match="lcLength"):
+        E.LightCurve(1.0, 1.0, 1.0, 0.1, 0.0, 1e-3, 0, 50)
+
+
+def test_binary_lens_grid_and_caustics(lib, oracle):
+    """m3 = 0 builds the reference's binary lens (lcirs.cc:29-36): the
+    critical curves of the light curve's triple-lens form with m3 = 0 and
+    the grid from them, every value the oracle's to the bit."""
+    from triplelens_amd import engine as E
+    geo = (1.0, 0.9433981132056605, 1.0121970114513341, 0.2, 0.0)
+    lc = E.LightCurve(*geo, 1e-3, 10, 50)
+    cc, ca = lc.cc_ca()
+    occ, oca = oracle.ccc(*geo)
+    assert np.array_equal(cc, occ) and np.array_equal(ca, oca)
+    o = oracle.new(*geo, 1e-3, 10, 50)
+    assert np.array_equal(lc.grid()[:32], o.grid()[:32])
 
 
 def test_limb_darkening_models(lib):

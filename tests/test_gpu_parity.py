@@ -398,3 +398,30 @@ def test_fill_positions_of_the_last_seed_search(E):
     assert np.array_equal(rays, rays0) and np.array_equal(A, A0)
     with pytest.raises(E.EngineError):
         lc.fill_positions(np.array([c.n_points]))
+
+
+def test_binary_lens_matches_oracle_and_golden(E, oracle):
+    """The binary lens (m3 = 0, irsBinary and the binary image-count rule):
+    the C1 binary's ray counts equal the reference's at every point and its
+    seeds the reference's; the planetary binary equals the oracle at every
+    point (lost roots included) and the reference wherever the oracle
+    does."""
+    g = golden("binary.npz")
+    for k, p in enumerate(g["params"]):
+        c = curve_from_params(p)
+        lc = _new(E, c)
+        amp, rays, flags = lc.lc_irs(*c.traj())
+        _assert_no_status(E, flags)
+        amp_o, rays_o, _ = oracle.new(*c.lens(), c.rho, c.lc_len, c.ppr, c.u).lc_irs(*c.traj())
+        assert np.array_equal(rays, rays_o), (k, np.where(rays != rays_o)[0][:10])
+        assert _rel(amp, amp_o) < AMP_RTOL, k
+        same = rays_o == g["rays"][k]
+        assert np.array_equal(rays[same], g["rays"][k][same]) and (~same).sum() <= 1, k
+        if k == 0:
+            assert same.all()
+            ns, pix = lc.seeds()
+            off, xy = g["seed_off"][0], g["seed_xy0"]
+            px, py = pixel_seeds(g["grid"][0], xy)
+            for i in range(c.n_points):
+                ref = set(zip(px[off[i]:off[i + 1]].tolist(), py[off[i]:off[i + 1]].tolist()))
+                assert set(map(tuple, pix[i, :ns[i]].tolist())) == ref, i

@@ -143,3 +143,48 @@ def test_amoeba_print_out_matches_reference(oracle):
         t = i / c.lc_len
         lc.fill_point(c.sx + (c.ex - c.sx) * t, c.sy + (c.ey - c.sy) * t, xy[i, :ns[i]])
         assert np.array_equal(lc.last_segments(), g["segs"][off[i]:off[i + 1]]), i
+
+
+def test_binary_lens_grid_curves_and_seeds(oracle):
+    """The binary lens (m3 = 0; fixtures from the reference build): the grid
+    and the light curve's critical curves and caustics (the triple-lens form
+    with m3 = 0: ccc.cc:28-39 leaves the binary flag unset) to the bit, and
+    every seed of the C1 binary with the binary image-count rule
+    (imgpoint.cc:236-237)."""
+    g = golden("binary.npz")
+    for k, p in enumerate(g["params"]):
+        c = curve_from_params(p)
+        lc = oracle.new(*c.lens(), c.rho, c.lc_len, c.ppr, c.u)
+        assert np.array_equal(lc.grid()[:32], g["grid"][k]), k
+        cc, ca = oracle.ccc(*c.lens())
+        assert np.array_equal(cc, g["cc"][k]) and np.array_equal(ca, g["ca"][k]), k
+    c = curve_from_params(g["params"][0])
+    lc = oracle.new(*c.lens(), c.rho, c.lc_len, c.ppr, c.u)
+    xy, ns, ni = lc.seeds(*c.traj())
+    assert np.array_equal(ni, g["n_img"][0])
+    assert set(ni.tolist()) <= {3, 5}  # odd counts: the binary lens
+    off = g["seed_off"][0]
+    gx, gy = pixel_seeds(lc.grid(), g["seed_xy0"])
+    ox, oy = pixel_seeds(lc.grid(), xy)
+    for i in range(len(ns)):
+        want = sorted(zip(gx[off[i]:off[i + 1]], gy[off[i]:off[i + 1]]))
+        assert sorted(zip(ox[i, :ns[i]], oy[i, :ns[i]])) == want, i
+
+
+def test_binary_c1_light_curve_exact(oracle):
+    """irsBinary (lcirs.cc:345-353): every ray count and magnification of
+    the C1 binary equal to the reference's."""
+    g = golden("binary.npz")
+    c = curve_from_params(g["params"][0])
+    amp, rays, edge = oracle.new(*c.lens(), c.rho, c.lc_len, c.ppr, c.u).lc_irs(*c.traj())
+    assert np.array_equal(rays, g["rays"][0]) and np.array_equal(amp, g["amp"][0])
+
+
+def test_binary_planet_light_curve(oracle):
+    """The limb-darkened planetary binary through its central caustic: the
+    reference's ray counts except where a root is lost (an even image count
+    for the binary lens, margin 0), at most one point."""
+    g = golden("binary.npz")
+    c = curve_from_params(g["params"][1])
+    nbad, n = _compare_curve(oracle, c, g["rays"][1], g["amp"][1])
+    assert nbad <= 1

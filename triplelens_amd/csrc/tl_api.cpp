@@ -64,6 +64,7 @@ struct Curve {
   long long N;
   double Ldbl, step, blx, bly, trx, try_;
   cx boxMin[6], boxMax[6];
+  bool binImg = false, binIrs = false;  // the binary lens's image-count rule and ray map
   std::vector<cx> cc, ca;      // 6 x kCccLen, root-major
   RootChain chain;             // root-reuse state carried between calls
   std::vector<double> lcVec;   // lcLength values (copy_lc)
@@ -130,7 +131,6 @@ Curve* make_curve(double a, double b, double th, double m2, double m3, double rh
   if (lcLength == 0) throw std::invalid_argument("lcLength must be > 0");
   for (double v : {a, b, th, m2, m3})
     if (!std::isfinite(v)) throw std::invalid_argument("lens parameters must be finite");
-  if (m3 == 0.0) throw std::invalid_argument("m3 == 0 (binary lens) is not supported");
   auto* c = new Curve();
   c->a = a; c->b = b; c->th = th; c->m2 = m2; c->m3 = m3; c->rho = rho;
   c->lcLength = lcLength;
@@ -141,6 +141,11 @@ Curve* make_curve(double a, double b, double th, double m2, double m3, double rh
   c->rho2pow = std::pow(rho, 2.0);
   ld_default(*c);
   critical_curves(c->L, c->cc, c->ca);
+  // the binary lens (m3 = 0): irsBinary (lcirs.cc:29-36) and the binary
+  // image-count rule, which also holds for a or b zero (imgpoint.cc:64-67);
+  // the critical curves stay the triple-lens form with m3 = 0
+  c->binIrs = m3 == 0.0;
+  c->binImg = a == 0.0 || b == 0.0 || m3 == 0.0;
   // _getCaBoxes (lcirs.cc:39-48): cumulative, origin included
   cx mn = mk(0, 0), mx = mk(0, 0);
   for (int k = 0; k < 6; k++) {
@@ -371,6 +376,8 @@ CurveDev to_dev(const Curve& c, cx s, cx e, int pointBase, int causticBase) {
   d.chainHave = c.chain.have;
   d.ring = c.ring ? 1 : 0;
   d.split = c.split;
+  d.binImg = c.binImg ? 1 : 0;
+  d.binIrs = c.binIrs ? 1 : 0;
   for (int k = 0; k < 11; k++) d.chain[k] = c.chain.tempRoots[k];
   for (int k = 0; k < 6; k++) {
     d.boxMin[k] = c.boxMin[k];

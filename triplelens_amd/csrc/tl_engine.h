@@ -43,6 +43,8 @@ struct CurveDev {
   int chainHave;               // root-reuse state at the start of the run
   int ring;                    // compute the disk-sample magnifications (ring_amp_kernel)
   int split;                   // independent pieces of the seed search (1: one chain)
+  int binImg;                  // binary image-count rule (a, b or m3 zero)
+  int binIrs;                  // binary ray map (m3 = 0)
   cx chain[11];
   cx boxMin[6], boxMax[6];
 };

@@ -377,7 +377,8 @@ __device__ __forceinline__ bool wl_check(const cx& cv, const cx& rootv) {
 // accepted as images (bit k: root k on lane k of chainv).
 __device__ __forceinline__ unsigned long long wl_get_images(const Lens& L, const dcx& Dv,
                                                             const dcx& Sv, cx& chainv, int& have,
-                                                            cx zeta, const TrigTable* trig) {
+                                                            cx zeta, const TrigTable* trig,
+                                                            bool binImg) {
   const cx cv = wl_poly(L, Dv, Sv, zeta);
   const int ln = __lane_id();
   // The reference's sequence (imgpoint.cc:181-206,262-270): a solve from the
@@ -398,7 +399,10 @@ __device__ __forceinline__ unsigned long long wl_get_images(const Lens& L, const
     const bool ok = ln < 10 && img_check(L, chainv, zeta, 1.0e-6);
     const unsigned long long m = __ballot(ok);
     const int n = __builtin_popcountll(m);
-    if (!(n % 2 != 0 || n < 4) || second || !withInit) return m;
+    // a valid count: even and at least 4, or for the binary lens odd and at
+    // least 3 (imgpoint.cc:236-237)
+    const bool bad = binImg ? (n % 2 != 1 || n < 3) : (n % 2 != 0 || n < 4);
+    if (!bad || second || !withInit) return m;
     second = true;
     withInit = false;
   }
@@ -512,7 +516,7 @@ __global__ void __launch_bounds__(64) seed_kernel(SeedArgs a) {
         f &= f - 1;
         zeta = rlc(trial, j);
       }
-      const unsigned long long m = wl_get_images(L, Dv, Sv, chainv, have, zeta, a.trig);
+      const unsigned long long m = wl_get_images(L, Dv, Sv, chainv, have, zeta, a.trig, C.binImg);
       emit_seeds(C, chainv, m, out, cnt, fl);
       if (centre) {
         // point-source magnification (getPointAmp, lcbase.cc:17-38) and the
@@ -557,7 +561,8 @@ __global__ void __launch_bounds__(64) seed_kernel(SeedArgs a) {
           a.ptAmp[p] = amp;
           a.ptCen[p] = make_double2(sx / amp, sy / amp);
           const double dist = sqrt(d2);
-          a.ptCaus[p] = __builtin_popcountll(m) > 4 ? -dist : dist;
+          // inside a caustic: more images than the 4 (binary: 3) outside
+          a.ptCaus[p] = __builtin_popcountll(m) > (C.binImg ? 3 : 4) ? -dist : dist;
           a.ptPair[p] = amp > 0.0 ? (t1 + t2) / amp : 0.0;
         }
       }
@@ -593,7 +598,7 @@ __global__ void __launch_bounds__(64) point_lc_kernel(const CurveDev* curves, in
   const cx s = mk(C.sx, C.sy), e = mk(C.ex, C.ey);
   for (unsigned i = 0; i < C.lcLength; i++) {
     const cx pos = cadd(s, cscale(csub(e, s), i / (C.lcLength - 1.0)));
-    unsigned long long m = wl_get_images(L, Dv, Sv, chainv, have, pos, trig);
+    unsigned long long m = wl_get_images(L, Dv, Sv, chainv, have, pos, trig, C.binImg);
     const double term = ((m >> ln) & 1ull) ? point_amp_term(L, chainv) : 0.0;
     double amp = 0.0;
     while (m) {
@@ -659,7 +664,7 @@ __global__ void __launch_bounds__(64) ring_amp_kernel(const CurveDev* curves, co
     cx chainv = centre;
     int have = 1;
     const cx zeta = mk(pp.x + C.rho * kRingOff[s][0], pp.y + C.rho * kRingOff[s][1]);
-    unsigned long long m = wl_get_images(L, Dv, Sv, chainv, have, zeta, trig);
+    unsigned long long m = wl_get_images(L, Dv, Sv, chainv, have, zeta, trig, C.binImg);
     const bool img = (m >> ln) & 1ull;
     const double term = img ? point_amp_term(L, chainv) : 0.0;
     double amp = 0.0;
@@ -916,16 +921,18 @@ struct RowCtx {
   double blx, bly, step, rho2, ldA, ldB;
   double y, px, py;
   int N;
+  int binIrs;  // binary ray map (irsBinary)
 };
 
 // Ray test of pixel x of the row (lcirs.cc:400-402) and its brightness
 // A + B sqrt(1 - r^2) (lcirs.cc:406-407, limbdarkeningmodel.cc:45-49) as a
-// multiple of 2^-34.  The test is the reference's to the bit; the
-// brightness (a sum that only has to match to 1e-6) uses the reciprocal of
-// rho^2 and the hardware reciprocal square root, within a few ulps of the
-// reference's value, and is rounded to fixed point by adding 2^52.
+// multiple of 2^-34.  The test is the reference's to the bit (irsSIMD, or
+// irsBinary for m3 = 0); the brightness is computed with IEEE division and
+// square root and rounded to fixed point by adding 2^52.
 __device__ __forceinline__ bool eval_px(const RowCtx& R, int x, long long& val) {
-  const double rsq = irs_ray(R.L, R.blx + R.step * (double)x, R.y, R.px, R.py);
+  const double xx = R.blx + R.step * (double)x;
+  const double rsq = R.binIrs ? irs_binary(R.L, xx, R.y, R.px, R.py)
+                              : irs_ray(R.L, xx, R.y, R.px, R.py);
   if (!(rsq < R.rho2)) return false;
   // sourceBrightnessRSq(rsq / rho^2) (lcirs.cc:406-407,
   // limbdarkeningmodel.cc:45-49): IEEE division and square root
@@ -1351,6 +1358,7 @@ __device__ __forceinline__ void process_task(TaskState& S) {
     S.R.rho2 = unid(C.rho2);
     S.R.ldA = unid(C.ldA);
     S.R.ldB = unid(C.ldB);
+    S.R.binIrs = uni(C.binIrs);
     S.R.N = S.N;
     const double2 pos = A.ptPos[S.gp];
     S.R.px = unid(pos.x);

@@ -341,7 +341,10 @@ struct CccTable { cx e[kCccSteps]; };
 // critical-curve polynomial in the reference's operation order, solved at
 // phi_j = 2*3.14159265359*j/kCccSteps with root polishing continued from step
 // to step, and the caustic as the lens map of every critical point.  cc and
-// ca receive 6 x kCccSteps values, branch-major.
+// ca receive 6 x kCccSteps values, branch-major.  The light curve builds
+// its CriticalCurveCaustic from a LensPar (lcirs.cc:16), a constructor that
+// leaves the binary flag unset (ccc.cc:28-39), so this triple-lens form
+// also serves m3 = 0: two of its branches then sit at the massless z3.
 TL_HD void critical_curves_into(const Lens& L, const CccTable& tab, const TrigTable* trig,
                                 cx* cc, cx* ca) {
   const int len = kCccSteps;
@@ -572,6 +575,17 @@ TL_HD cx traj_pos(cx s, cx e, unsigned i, unsigned len) {
 // lane order (lcirs.cc:290-341): lanes (z3, z2, z1, zeta), weights
 // (m3, m2, m1, 1), the zeta lane's squared distance replaced by -1, halves
 // added lane-wise (z3 + z1, z2 + zeta) and then together.
+// LightCurveIRS::irsBinary (lcirs.cc:345-353): img - m1/conj(img-z1) -
+// m2/conj(img-z2) in std::complex arithmetic (each real/complex quotient is
+// libgcc's complex division), and std::norm of its distance to the source.
+TL_HD double irs_binary(const Lens& L, double x, double y, double sx, double sy) {
+  const cx img = mk(x, y);
+  cx t = csub(img, cdiv_rc_sel(L.m1, cconj(csub(img, L.z1))));
+  t = csub(t, cdiv_rc_sel(L.m2, cconj(csub(img, L.z2))));
+  const double dr = t.re - sx, di = t.im - sy;
+  return dr * dr + di * di;
+}
+
 TL_HD double irs_ray(const Lens& L, double x, double y, double sx, double sy) {
   double dr3 = L.z3.re - x, dr2 = L.z2.re - x, dr1 = L.z1.re - x, drs = sx - x;
   double di3 = L.z3.im - y, di2 = L.z2.im - y, di1 = L.z1.im - y, dis = sy - y;

@@ -54,6 +54,27 @@ def c1() -> Curve:
     return Curve(a, b, th, 0.1, 0.1, 1e-3, 50, 0.0, -0.5, -0.1, 1.5, 0.1, 200)
 
 
+def binary_c1() -> Curve:
+    """C1 as a binary lens: the lenses z1=(-0.5,0), z2=(0.5,0) with masses
+    (0.8, 0.2) and m3 = 0 (the reference's binary lens); C1's source, pixel
+    and trajectory."""
+    c = c1()
+    return Curve(c.a, c.b, c.th, 0.2, 0.0, c.rho, c.ppr, 0.0, c.sx, c.sy, c.ex, c.ey, c.n_points)
+
+
+def binary_planet(ca_fn) -> Curve:
+    """A planetary binary lens: mass ratio 1e-3 at separation 1.1 (m3 = 0),
+    linear limb darkening u = 0.5, rho = 5e-4, pixel rho/50, 200 positions
+    on a chord of length 0.008 through the central caustic (the caustic
+    sample nearest the primary).  ``ca_fn(a, b, th, m2, m3)`` returns the
+    caustic samples."""
+    a, b, th, m2, m3 = 1.1, 1.0, 1.0, 1e-3 / (1.0 + 1e-3), 0.0
+    p = central_caustic_point(a, b, th, m2, m3, ca_fn(a, b, th, m2, m3))
+    d = complex(0.004, 0.0015) / abs(complex(0.004, 0.0015)) * 0.004
+    s, e = p - d, p + d
+    return Curve(a, b, th, m2, m3, 5e-4, 50, 0.5, s.real, s.imag, e.real, e.imag, 200)
+
+
 def _logu(rng, lo, hi):
     return float(math.exp(rng.uniform(math.log(lo), math.log(hi))))
 
