@@ -841,7 +841,27 @@ __device__ __forceinline__ void unite(const FillArgs& A, int a, int b) {
 // publishes it is written (mem_order between them), and a slot is handed back
 // only after its payload has been read.  Lane 0 operates.
 // ---------------------------------------------------------------------------
-__device__ bool q_push(const FillArgs& A, const Task& t) {
+// A task as four 64-bit words (two ints each, low half first), and back:
+// built from the fields, so no task is copied through memory.
+__device__ __forceinline__ unsigned long long pack2(int lo, int hi) {
+  return (unsigned long long)(unsigned)lo | ((unsigned long long)(unsigned)hi << 32);
+}
+__device__ __forceinline__ void task_words(const Task& t, unsigned long long (&w)[4]) {
+  w[0] = pack2(t.b, t.row);
+  w[1] = pack2(t.xa, t.xb);
+  w[2] = pack2(t.label, t.kd);
+  w[3] = pack2(t.f1, t.f2);
+}
+__device__ __forceinline__ Task words_task(const unsigned long long (&w)[4]) {
+  Task t;
+  t.b = (int)(unsigned)w[0]; t.row = (int)(unsigned)(w[0] >> 32);
+  t.xa = (int)(unsigned)w[1]; t.xb = (int)(unsigned)(w[1] >> 32);
+  t.label = (int)(unsigned)w[2]; t.kd = (int)(unsigned)(w[2] >> 32);
+  t.f1 = (int)(unsigned)w[3]; t.f2 = (int)(unsigned)(w[3] >> 32);
+  return t;
+}
+
+__device__ __forceinline__ bool q_push(const FillArgs& A, const Task& t) {
   int ok = 0;
   if (__lane_id() == 0) {
     unsigned long long pos = ald(A.qtail);
@@ -852,9 +872,12 @@ __device__ bool q_push(const FillArgs& A, const Task& t) {
       if (dif == 0) {
         if (acas(A.qtail, &pos, pos + 1)) {
           unsigned long long w[4];
-          __builtin_memcpy(w, &t, sizeof(Task));
+          task_words(t, w);
           unsigned long long* dst = reinterpret_cast<unsigned long long*>(&A.qbuf[idx]);
-          for (int k = 0; k < 4; k++) ast(&dst[k], w[k]);
+          ast(&dst[0], w[0]);
+          ast(&dst[1], w[1]);
+          ast(&dst[2], w[2]);
+          ast(&dst[3], w[3]);
           mem_order();
           ast(&A.qseq[idx], pos + 1);
           ok = 1;
@@ -870,7 +893,7 @@ __device__ bool q_push(const FillArgs& A, const Task& t) {
   return bcast0(ok) != 0;
 }
 
-__device__ bool q_pop(const FillArgs& A, Task& t) {
+__device__ __forceinline__ bool q_pop(const FillArgs& A, Task& t) {
   int ok = 0;
   unsigned long long w[4] = {0, 0, 0, 0};
   if (__lane_id() == 0) {
@@ -883,7 +906,10 @@ __device__ bool q_pop(const FillArgs& A, Task& t) {
         if (dif == 0) {
           if (acas(A.qhead, &pos, pos + 1)) {
             unsigned long long* src = reinterpret_cast<unsigned long long*>(&A.qbuf[idx]);
-            for (int k = 0; k < 4; k++) w[k] = ald(&src[k]);
+            w[0] = ald(&src[0]);
+            w[1] = ald(&src[1]);
+            w[2] = ald(&src[2]);
+            w[3] = ald(&src[3]);
             mem_order();
             ast(&A.qseq[idx], pos + A.qcap);
             ok = 1;
@@ -898,8 +924,11 @@ __device__ bool q_pop(const FillArgs& A, Task& t) {
     }
   }
   ok = bcast0(ok);
-  for (int k = 0; k < 4; k++) w[k] = bcast0(w[k]);
-  __builtin_memcpy(&t, w, sizeof(Task));
+  w[0] = bcast0(w[0]);
+  w[1] = bcast0(w[1]);
+  w[2] = bcast0(w[2]);
+  w[3] = bcast0(w[3]);
+  if (ok) t = words_task(w);
   return ok != 0;
 }
 
@@ -1054,25 +1083,26 @@ __device__ __forceinline__ Task make_task(int b, int row, int xa, int xb, int la
   return f;
 }
 
-// Follow-ups of one run, at most four, kept in named slots and dispatched
-// from one place: the first one that may stay with the wave becomes its next
-// step, the others are queued.
-struct Followups {
-  Task f0, f1, f2, f3;
-  int n;
-};
 constexpr int kQueueOnly = 8;  // kd flag: never kept by the wave
 constexpr int kShare = 16;     // kd flag: to the shared queue at once (the wave's
                                // ring when the queue is full)
 constexpr int kLeapEvery = 4;  // a scout chain sends a leap scout every this many scouts
 constexpr int kLeap = 8;       // ... this many jumps ahead
 
-__device__ __forceinline__ void fu_add(Followups& F, const Task& f) {
-  if (F.n == 0) F.f0 = f;
-  else if (F.n == 1) F.f1 = f;
-  else if (F.n == 2) F.f2 = f;
-  else F.f3 = f;
-  F.n++;
+// A follow-up of a run, dispatched as soon as it is made: the first one that
+// may stay with the wave becomes its next step, tasks marked kShare go to the
+// shared queue (as do all while some wave is idle and sharing is on), the
+// others to the wave's ring.  (Collecting them first made the compiler keep
+// them in a scratch array.)
+__device__ __forceinline__ void follow(TaskState& S, Task f) {
+  const bool queueOnly = (f.kd & kQueueOnly) != 0, share = (f.kd & kShare) != 0;
+  f.kd &= ~(kQueueOnly | kShare);
+  if (!queueOnly && !share && !S.haveCont) {
+    S.cont = f;
+    S.haveCont = true;
+  } else if (!((share || S.share) && q_push(*S.A, f))) {
+    ring_push(S, f);
+  }
 }
 
 // A maximal hit run [xl, xr] of the row being scanned was found: record it,
@@ -1116,12 +1146,10 @@ __device__ __forceinline__ void handle_run(TaskState& S, int xl, int xr, long lo
   S.accCnt += cnt;
   S.accSum += sum;
   const int row = t.row, N = S.N, center = (int)(((long long)xl + xr) / 2);
-  Followups F;
-  F.n = 0;
   const int d = S.dir;  // direction of travel
   if (S.kind == 1) {
-    if (row + 1 < N) fu_add(F, make_task(t.b, row + 1, xl, xr, label, 0, 1, center, 0));
-    if (row - 1 >= 0) fu_add(F, make_task(t.b, row - 1, xl, xr, label, 0, -1, center, 0));
+    if (row + 1 < N) follow(S, make_task(t.b, row + 1, xl, xr, label, 0, 1, center, 0));
+    if (row - 1 >= 0) follow(S, make_task(t.b, row - 1, xl, xr, label, 0, -1, center, 0));
   } else if (S.kind == 2) {
     // A scout landed (t.f1: the column of the run it was sent from, one jump
     // back; t.f2: scouts in its chain since the last leap).  The wave sends
@@ -1152,23 +1180,23 @@ __device__ __forceinline__ void handle_run(TaskState& S, int xl, int xr, long lo
         f.f1 = (int)(center + shift * (kLeap - 1));
         f.kd |= kQueueOnly | kShare;
       }
-      fu_add(F, f);
+      follow(S, f);
     }
     Task up = make_task(t.b, row + 1, xl, xr, label, 0, 1, center, -1);
     Task dn = make_task(t.b, row - 1, xl, xr, label, 0, -1, center, -1);
     up.kd |= kShare;
     dn.kd |= kShare;
-    if (row + 1 < N) fu_add(F, up);
-    if (row - 1 >= 0) fu_add(F, dn);
+    if (row + 1 < N) follow(S, up);
+    if (row - 1 >= 0) follow(S, dn);
   } else if (S.kind == 3) {
     // a run found looking back: sweep on away from the run it was reached
     // from, and look back towards that run's row where this one reaches past it
     if (row + d >= 0 && row + d < N)
-      fu_add(F, make_task(t.b, row + d, xl, xr, label, 0, d, center, 0));
+      follow(S, make_task(t.b, row + d, xl, xr, label, 0, d, center, 0));
     if (row - d >= 0 && row - d < N && (xl < S.xa - 1 || xr > S.xb + 1)) {
       Task f = make_task(t.b, row - d, xl, xr, label, 3, -d, S.xa - 1, S.xb + 1);
       f.kd |= kQueueOnly;
-      fu_add(F, f);
+      follow(S, f);
     }
   } else {
     // sweep step: 1. the next row in the direction of travel
@@ -1193,13 +1221,13 @@ __device__ __forceinline__ void handle_run(TaskState& S, int xl, int xr, long lo
           if (sa <= sb) {
             Task f = make_task(t.b, (int)rt, (int)sa, (int)sb, -1, 2, d, center, 0);
             f.kd |= kQueueOnly | kShare;
-            fu_add(F, f);
+            follow(S, f);
           }
         }
         a1 = center;
         a2 = -1;
       }
-      fu_add(F, make_task(t.b, row + d, xl, xr, label, 0, d, a1, a2));
+      follow(S, make_task(t.b, row + d, xl, xr, label, 0, d, a1, a2));
     }
     // 2. the row the sweep came from, where the new run reaches past the
     //    parent run [xa, xb] and its two bounding misses: looked at after
@@ -1217,20 +1245,8 @@ __device__ __forceinline__ void handle_run(TaskState& S, int xl, int xr, long lo
       } else {
         Task f = make_task(t.b, lb.row, xl, xr, label, 3, -d, lb.k1, lb.k2);
         f.kd |= kQueueOnly;
-        fu_add(F, f);
+        follow(S, f);
       }
-    }
-  }
-  for (int k = 0; k < 4; k++) {
-    if (k >= F.n) break;
-    Task f = k == 0 ? F.f0 : (k == 1 ? F.f1 : (k == 2 ? F.f2 : F.f3));
-    const bool queueOnly = (f.kd & kQueueOnly) != 0, share = (f.kd & kShare) != 0;
-    f.kd &= ~(kQueueOnly | kShare);
-    if (!queueOnly && !share && !S.haveCont) {
-      S.cont = f;
-      S.haveCont = true;
-    } else if (!((share || S.share) && q_push(A, f))) {
-      ring_push(S, f);
     }
   }
 }
