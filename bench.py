@@ -162,6 +162,7 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--mode", choices=["surrogate", "plain"], default="surrogate")
     ap.add_argument("--model", default=MODEL, help="trained surrogate (.npz)")
+    ap.add_argument("--tuning", default="", help="engine options saved by triplelens_amd.autotune")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-cores", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--calibrate-copy", type=int, metavar="MIB", default=0,
@@ -184,6 +185,10 @@ def main(argv=None):
 
     from triplelens_amd import engine as E
     name, cus = E.device_info()
+    tuning = None
+    if args.tuning:
+        from triplelens_amd import autotune
+        tuning = autotune.apply(args.tuning)
     lc = E.LightCurve(*c.lens(), c.rho, c.lc_len, c.ppr)
     lc.set_limb_darkening("linear", c.u)
     for _ in range(args.warmup):
@@ -212,6 +217,7 @@ def main(argv=None):
         "lens": {"a": c.a, "b": c.b, "th": c.th, "m2": c.m2, "m3": c.m3},
         "device": name,
         "cus": cus,
+        "engine_options": tuning or "defaults",
         "plain_irs_points_per_s": plain_pps,
         "plain_irs_ms_per_curve": 1e3 * wall_plain / args.steps,
         "plain_rays_per_curve": rays / args.steps,

@@ -425,3 +425,19 @@ def test_binary_lens_matches_oracle_and_golden(E, oracle):
             for i in range(c.n_points):
                 ref = set(zip(px[off[i]:off[i + 1]].tolist(), py[off[i]:off[i + 1]].tolist()))
                 assert set(map(tuple, pix[i, :ns[i]].tolist())) == ref, i
+
+
+def test_autotuner_keeps_results(E, tmp_path):
+    """The autotuner times option sets on C1, finds every set's ray counts
+    identical, and its saved choice applies."""
+    from triplelens_amd import autotune, workloads as W
+    space = {"fill_workgroups": [0, 512], "scout_jump": [64, 256], "run_pixels": [128]}
+    best, table = autotune.tune(W.c1(), space, repeats=1, log=lambda s: None)
+    assert len(table) == 4 and best in [row[0] for row in table]
+    p = tmp_path / "tuning.json"
+    p.write_text(__import__("json").dumps({"options": best}))
+    try:
+        assert autotune.apply(str(p)) == best
+    finally:
+        for n, v in autotune.DEFAULTS.items():
+            E.set_option(n, v)
