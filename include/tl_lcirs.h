@@ -181,8 +181,10 @@ int tl_ccc_batch(int n, const double* params, double* cc, double* ca);
 int tl_math_selftest(int n, unsigned long long seed);
 
 /* Engine settings: name/value pairs ("fill_workgroups", "probe_limit",
- * "queue_log2", "scout_rows", "scout_jump", "time_limit_s").  Returns 0 or
- * -1 for an unknown name. */
+ * "queue_log2", "scout_rows", "scout_jump", "scout_retry", "scout_width",
+ * "poll_backoff", "share_idle", "run_pixels", "table_load", "seed_chunks",
+ * "seed_chunk_min", "time_limit_s").  None changes a result.  Returns 0 or
+ * -1 for an unknown name.  No counterpart in the reference. */
 int tl_set_option(const char* name, double value);
 
 /* Description of the last error on this thread ("" if none). */

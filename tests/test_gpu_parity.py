@@ -179,6 +179,31 @@ def test_full_c2_curve_properties(E, oracle):
         assert abs(amp[i] - s * g[7]) <= AMP_RTOL * abs(s * g[7]), i
 
 
+def test_scout_options_keep_results(E):
+    """Scouts only add labels that merge: without them, with the retry of
+    missed scouts off, and with the jump shortened on wide runs, the window
+    around the C2 peak gives the reference's ray counts and the same
+    magnifications."""
+    g = golden("c2_window.npz")
+    c = curve_from_params(g["params"])
+    runs = []
+    try:
+        for opts in [{"scout_rows": 0}, {"scout_retry": 0}, {"scout_width": 2048}, {}]:
+            for n, v in opts.items():
+                E.set_option(n, v)
+            amp, rays, flags = _new(E, c).lc_irs(*c.traj())
+            for n in opts:
+                E.set_option(n, {"scout_rows": 8, "scout_retry": 1, "scout_width": 0}[n])
+            assert np.array_equal(rays, g["rays"]), opts
+            _assert_no_status(E, flags)
+            runs.append(amp)
+    finally:
+        for n, v in {"scout_rows": 8, "scout_retry": 1, "scout_width": 0}.items():
+            E.set_option(n, v)
+    for amp in runs[1:]:
+        assert np.array_equal(amp, runs[0])
+
+
 # ------------------------------------------------------------ seeded fills
 def test_fill_from_given_seeds(E, oracle):
     """lcirs_fill_seeds on hand-made seed lists: none, duplicates, misses,

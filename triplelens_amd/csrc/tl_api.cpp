@@ -204,6 +204,8 @@ struct Options {
   int queueLog2 = 21;
   int scoutRows = 8;
   int scoutJump = 256;
+  int scoutWidth = 0;       // runs wider than this shorten the scout jump (0: never)
+  int scoutRetry = 1;       // a scout that finds nothing is sent again half as far
   int pollBackoff = 6;
   int shareIdle = 0;
   double runPixels = 128.0;  // predicted rays per run: sizes the run table
@@ -612,6 +614,8 @@ void Engine::run(Job& job) {
       fa.timeLimitTicks = (long long)(opt.timeLimitS * 1.0e8);
       fa.scoutRows = opt.scoutRows;
       fa.scoutJump = opt.scoutJump;
+      fa.scoutWidth = opt.scoutWidth;
+      fa.scoutRetry = opt.scoutRetry;
       fa.pollBackoff = opt.pollBackoff;
       fa.shareIdle = opt.shareIdle;
       HIPCHK((hipError_t)launch_fill_reset(fa, stream));
@@ -1226,7 +1230,9 @@ int tl_set_option(const char* name, double value) {
   else if (n == "probe_limit") o.probeLimit = std::max(64, (int)value);
   else if (n == "queue_log2") o.queueLog2 = std::min(28, std::max(10, (int)value));
   else if (n == "scout_rows") o.scoutRows = std::max(0, (int)value);
-  else if (n == "scout_jump") o.scoutJump = std::max(1, (int)value);
+  else if (n == "scout_jump") o.scoutJump = std::min(32767, std::max(1, (int)value));
+  else if (n == "scout_width") o.scoutWidth = std::max(0, (int)value);
+  else if (n == "scout_retry") o.scoutRetry = value != 0.0 ? 1 : 0;
   else if (n == "poll_backoff") o.pollBackoff = std::min(6, std::max(0, (int)value));
   else if (n == "share_idle") o.shareIdle = value != 0.0 ? 1 : 0;
   else if (n == "run_pixels") o.runPixels = std::min(65536.0, std::max(1.0, value));

@@ -120,6 +120,8 @@ struct FillArgs {
   long long timeLimitTicks;   // s_memrealtime ticks (100 MHz)
   int scoutRows;              // rows swept before a scout is sent (0 = no scouts)
   int scoutJump;              // rows a scout jumps ahead
+  int scoutWidth;             // runs wider than this shorten the jump (0: never)
+  int scoutRetry;             // a scout that finds nothing is sent again half as far
   int pollBackoff;            // idle polls sleep up to 2^pollBackoff x 1024 cycles
   int shareIdle;              // while some wave is idle, queue follow-ups in the shared ring
 };

@@ -1033,6 +1033,7 @@ struct TaskState {
   unsigned long long accCnt;
   unsigned long long evals, nIns, nDup, insTicks;
   unsigned long long nScout, nScoutRuns;  // scout tasks taken, runs they found
+  bool landed;              // the scout task found a run (new or known)
   // this wave's ring of waiting tasks in LDS: lcount tasks from lhead on
   Task* ring;
   int lhead, lcount;
@@ -1105,6 +1106,20 @@ __device__ __forceinline__ void follow(TaskState& S, Task f) {
   }
 }
 
+// Rows a scout jumps from a run of w pixels: scout_jump, shortened in
+// proportion where runs are wider than scout_width pixels (0: never), so
+// that the rows between two scouts hold about the same number of pixels on
+// wide stretches of an image as on narrow ones.  A scout task carries the
+// jump it was sent with in the high half of f2 (the low half counts the
+// scouts of its chain since the last leap).
+constexpr int kMinJump = 16;
+__device__ __forceinline__ int scout_jump_for(const FillArgs& A, int w) {
+  if (A.scoutWidth <= 0 || w <= A.scoutWidth) return A.scoutJump;
+  const long long j = (long long)A.scoutJump * A.scoutWidth / w;
+  const int lo = kMinJump < A.scoutJump ? kMinJump : A.scoutJump;
+  return j < lo ? lo : (int)j;
+}
+
 // A maximal hit run [xl, xr] of the row being scanned was found: record it,
 // credit its pixels to its label and schedule the neighbouring rows.
 __device__ __forceinline__ void handle_run(TaskState& S, int xl, int xr, long long sum,
@@ -1113,6 +1128,7 @@ __device__ __forceinline__ void handle_run(TaskState& S, int xl, int xr, long lo
   const Task& t = S.t;
   int label = t.label;
   if (S.kind == 2) {
+    S.landed = true;
     // a scout run starts a class of its own
     int nl = 0;
     if (__lane_id() == 0) nl = atomicAdd(A.labelNext, 1);
@@ -1152,20 +1168,23 @@ __device__ __forceinline__ void handle_run(TaskState& S, int xl, int xr, long lo
     if (row - 1 >= 0) follow(S, make_task(t.b, row - 1, xl, xr, label, 0, -1, center, 0));
   } else if (S.kind == 2) {
     // A scout landed (t.f1: the column of the run it was sent from, one jump
-    // back; t.f2: scouts in its chain since the last leap).  The wave sends
-    // the next scout straight on at once, with the shift measured over the
-    // last jump, and keeps it as its next step, so that the scouts run ahead
+    // back; t.f2: that jump's length and the scouts in its chain since the
+    // last leap).  The wave sends the next scout straight on at once, with
+    // the shift measured over the last jump, and keeps it as its next step, so that the scouts run ahead
     // one task per jump; every kLeapEvery-th scout also sends a leap scout
     // kLeap jumps ahead to the shared queue, which starts a chain of its
     // own, so that long images get many chains.  The two sweeps of the new
     // class go to the shared queue for idle waves; they start as if they had
     // just sent a scout.
-    const long long shift = (long long)center - t.f1;
+    const int count = t.f2 & 0xFFFF, prevJump = (int)((unsigned)t.f2 >> 16);
+    const int J = scout_jump_for(A, xr - xl + 1);
+    // the shift over the last jump, rescaled to this jump
+    const long long shift = ((long long)center - t.f1) * J / (prevJump > 0 ? prevJump : J);
     const long long ash = shift < 0 ? -shift : shift;
     for (int leap = 0; leap < 2 && A.scoutRows > 0; leap++) {
-      if (leap == 1 && (t.f2 + 1) % kLeapEvery != 0) break;
+      if (leap == 1 && (count + 1) % kLeapEvery != 0) break;
       const long long k = leap ? kLeap : 1;
-      const long long rt = (long long)row + (long long)d * A.scoutJump * k;
+      const long long rt = (long long)row + (long long)d * J * k;
       if (rt < 0 || rt >= N) continue;
       const long long pc = center + shift * k;
       const long long hw = ((long long)xr - xl) / 2 + ash * k / 4 + 32 * k;
@@ -1173,7 +1192,8 @@ __device__ __forceinline__ void handle_run(TaskState& S, int xl, int xr, long lo
       if (sa < 0) sa = 0;
       if (sb > N - 1) sb = N - 1;
       if (sa > sb) continue;
-      Task f = make_task(t.b, (int)rt, (int)sa, (int)sb, -1, 2, d, center, leap ? 0 : t.f2 + 1);
+      Task f = make_task(t.b, (int)rt, (int)sa, (int)sb, -1, 2, d, center,
+                         (J << 16) | (leap ? 0 : (count + 1) & 0xFFFF));
       if (leap) {
         // the leap's chain measures its next shift from the column expected
         // one jump before its landing row
@@ -1205,21 +1225,22 @@ __device__ __forceinline__ void handle_run(TaskState& S, int xl, int xr, long lo
       // f2 < 0: -(rows since the last scout) - 1
       int a1 = t.f1, a2 = t.f2 >= 0 ? t.f2 + 1 : t.f2 - 1;
       const int rows = a2 >= 0 ? a2 : -a2 - 1;
+      const int J = scout_jump_for(A, xr - xl + 1);
       const bool due = A.scoutRows > 0 && ((a2 >= 0 && rows >= A.scoutRows) ||
-                                           (a2 < 0 && rows >= A.scoutJump));
+                                           (a2 < 0 && rows >= J));
       if (due) {
         // send a scout ahead along the run's drift, then re-anchor
-        const long long rt = (long long)row + (long long)d * A.scoutJump;
+        const long long rt = (long long)row + (long long)d * J;
         if (rt >= 0 && rt < N) {
           const long long drift = rows > 0 ? ((long long)center - a1) / rows : 0;
-          const long long pc = center + drift * A.scoutJump;
+          const long long pc = center + drift * J;
           const long long adr = drift < 0 ? -drift : drift;
-          const long long hw = ((long long)xr - xl) / 2 + adr * A.scoutJump / 4 + 32;
+          const long long hw = ((long long)xr - xl) / 2 + adr * J / 4 + 32;
           long long sa = pc - hw, sb = pc + hw;
           if (sa < 0) sa = 0;
           if (sb > N - 1) sb = N - 1;
           if (sa <= sb) {
-            Task f = make_task(t.b, (int)rt, (int)sa, (int)sb, -1, 2, d, center, 0);
+            Task f = make_task(t.b, (int)rt, (int)sa, (int)sb, -1, 2, d, center, J << 16);
             f.kd |= kQueueOnly | kShare;
             follow(S, f);
           }
@@ -1388,6 +1409,9 @@ __device__ __forceinline__ void process_task(TaskState& S) {
   int k1 = S.kind == 3 ? S.t.f1 : 0, k2 = S.kind == 3 ? S.t.f2 : -1;
   S.xa = S.t.xa;
   S.xb = S.t.xb;
+  const bool scout = S.kind == 2;
+  const int trow = S.t.row, txa = S.t.xa, txb = S.t.xb, tf1 = S.t.f1, tf2 = S.t.f2, tdir = S.dir;
+  S.landed = false;
   for (int job = 0; job < 6; job++) {
     if (job >= 2) {
       const int li = (job - 2) >> 1;
@@ -1424,6 +1448,27 @@ __device__ __forceinline__ void process_task(TaskState& S) {
     scan_window(S.R, s, e, extL, extR, S);
   }
   S.nBack = 0;
+  if (scout && !S.landed && A.scoutRetry) {
+    // A scout that found nothing (its chain's prediction failed, or the
+    // image ends before its row): a retry from the same origin, half the
+    // jump ahead, with a window of the same width about the midpoint of the
+    // origin column and the failed window's centre.
+    const int J = (int)((unsigned)tf2 >> 16), count = tf2 & 0xFFFF;
+    const int J2 = J / 2;
+    if (J2 >= kMinJump) {
+      const long long rt = (long long)trow - (long long)tdir * (J - J2);
+      const long long pc = ((long long)txa + txb) / 2;
+      const long long c2 = tf1 + (pc - tf1) / 2;
+      const long long hw = ((long long)txb - txa) / 2;
+      long long sa = c2 - hw, sb = c2 + hw;
+      if (sa < 0) sa = 0;
+      if (sb > S.N - 1) sb = S.N - 1;
+      if (rt >= 0 && rt < S.N && sa <= sb) {
+        follow(S, make_task(S.t.b, (int)rt, (int)sa, (int)sb, -1, 2, tdir, tf1,
+                            (J2 << 16) | count));
+      }
+    }
+  }
 }
 
 // Work comes from the seed list, then from this wave's ring, then from the
