@@ -7,6 +7,7 @@ combination of the fill's options (`tl_set_option`):
 
     fill_workgroups  workgroups of 4 wavefronts (0 = one per CU)
     scout_jump       rows a scout jumps ahead
+    scout_width      runs wider than this shorten the jump (0 = never)
     run_pixels       predicted rays per run (sizes the run table and batches)
     queue_log2       capacity of the shared task queue (2^k tasks)
 
@@ -28,11 +29,12 @@ from . import workloads as W
 
 DEFAULT_SPACE = {
     "fill_workgroups": [0, 512],
-    "scout_jump": [128, 256, 512],
-    "run_pixels": [64, 128, 256],
-    "queue_log2": [20, 21],
+    "scout_jump": [256, 512, 1024],
+    "scout_width": [0, 1024, 2048],
+    "run_pixels": [64, 128],
 }
-DEFAULTS = {"fill_workgroups": 0, "scout_jump": 256, "run_pixels": 128, "queue_log2": 21}
+DEFAULTS = {"fill_workgroups": 0, "scout_jump": 512, "scout_width": 2048, "run_pixels": 128,
+            "queue_log2": 21}
 
 
 def workload(name: str):

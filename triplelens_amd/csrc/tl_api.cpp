@@ -203,8 +203,8 @@ struct Options {
   int probeLimit = 4096;
   int queueLog2 = 21;
   int scoutRows = 8;
-  int scoutJump = 256;
-  int scoutWidth = 0;       // runs wider than this shorten the scout jump (0: never)
+  int scoutJump = 512;
+  int scoutWidth = 2048;    // runs wider than this shorten the scout jump (0: never)
   int scoutRetry = 1;       // a scout that finds nothing is sent again half as far
   int pollBackoff = 6;
   int shareIdle = 0;
