@@ -206,6 +206,8 @@ struct Options {
   int scoutJump = 512;
   int scoutWidth = 2048;    // runs wider than this shorten the scout jump (0: never)
   int scoutRetry = 1;       // a scout that finds nothing is sent again half as far
+  int leapEvery = 4;        // every this many scouts a chain sends a leap scout (0: never)
+  int leapJumps = 8;        // ... this many jumps ahead
   int pollBackoff = 6;
   int shareIdle = 0;
   double runPixels = 128.0;  // predicted rays per run: sizes the run table
@@ -616,6 +618,8 @@ void Engine::run(Job& job) {
       fa.scoutJump = opt.scoutJump;
       fa.scoutWidth = opt.scoutWidth;
       fa.scoutRetry = opt.scoutRetry;
+      fa.leapEvery = opt.leapEvery;
+      fa.leapJumps = opt.leapJumps;
       fa.pollBackoff = opt.pollBackoff;
       fa.shareIdle = opt.shareIdle;
       HIPCHK((hipError_t)launch_fill_reset(fa, stream));
@@ -1233,6 +1237,8 @@ int tl_set_option(const char* name, double value) {
   else if (n == "scout_jump") o.scoutJump = std::min(32767, std::max(1, (int)value));
   else if (n == "scout_width") o.scoutWidth = std::max(0, (int)value);
   else if (n == "scout_retry") o.scoutRetry = value != 0.0 ? 1 : 0;
+  else if (n == "leap_every") o.leapEvery = std::max(0, (int)value);
+  else if (n == "leap_jumps") o.leapJumps = std::min(64, std::max(2, (int)value));
   else if (n == "poll_backoff") o.pollBackoff = std::min(6, std::max(0, (int)value));
   else if (n == "share_idle") o.shareIdle = value != 0.0 ? 1 : 0;
   else if (n == "run_pixels") o.runPixels = std::min(65536.0, std::max(1.0, value));

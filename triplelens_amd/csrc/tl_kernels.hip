@@ -1087,8 +1087,6 @@ __device__ __forceinline__ Task make_task(int b, int row, int xa, int xb, int la
 constexpr int kQueueOnly = 8;  // kd flag: never kept by the wave
 constexpr int kShare = 16;     // kd flag: to the shared queue at once (the wave's
                                // ring when the queue is full)
-constexpr int kLeapEvery = 4;  // a scout chain sends a leap scout every this many scouts
-constexpr int kLeap = 8;       // ... this many jumps ahead
 
 // A follow-up of a run, dispatched as soon as it is made: the first one that
 // may stay with the wave becomes its next step, tasks marked kShare go to the
@@ -1171,8 +1169,8 @@ __device__ __forceinline__ void handle_run(TaskState& S, int xl, int xr, long lo
     // back; t.f2: that jump's length and the scouts in its chain since the
     // last leap).  The wave sends the next scout straight on at once, with
     // the shift measured over the last jump, and keeps it as its next step, so that the scouts run ahead
-    // one task per jump; every kLeapEvery-th scout also sends a leap scout
-    // kLeap jumps ahead to the shared queue, which starts a chain of its
+    // one task per jump; every leapEvery-th scout also sends a leap scout
+    // leapJumps jumps ahead to the shared queue, which starts a chain of its
     // own, so that long images get many chains.  The two sweeps of the new
     // class go to the shared queue for idle waves; they start as if they had
     // just sent a scout.
@@ -1182,8 +1180,8 @@ __device__ __forceinline__ void handle_run(TaskState& S, int xl, int xr, long lo
     const long long shift = ((long long)center - t.f1) * J / (prevJump > 0 ? prevJump : J);
     const long long ash = shift < 0 ? -shift : shift;
     for (int leap = 0; leap < 2 && A.scoutRows > 0; leap++) {
-      if (leap == 1 && (count + 1) % kLeapEvery != 0) break;
-      const long long k = leap ? kLeap : 1;
+      if (leap == 1 && (A.leapEvery <= 0 || (count + 1) % A.leapEvery != 0)) break;
+      const long long k = leap ? A.leapJumps : 1;
       const long long rt = (long long)row + (long long)d * J * k;
       if (rt < 0 || rt >= N) continue;
       const long long pc = center + shift * k;
@@ -1197,7 +1195,7 @@ __device__ __forceinline__ void handle_run(TaskState& S, int xl, int xr, long lo
       if (leap) {
         // the leap's chain measures its next shift from the column expected
         // one jump before its landing row
-        f.f1 = (int)(center + shift * (kLeap - 1));
+        f.f1 = (int)(center + shift * (A.leapJumps - 1));
         f.kd |= kQueueOnly | kShare;
       }
       follow(S, f);
