@@ -154,13 +154,80 @@ def calibrate_copy(mib):
     return 0
 
 
+# ---------------------------------------------------------------- Metropolis
+def metropolis_bench(args, c2):
+    """The Metropolis fitting workload at reduced size: args.chains chains of
+    args.proposals proposals, each a light curve of args.points positions on
+    the C2 trajectory, 1% Gaussian steps of the lens parameters.  Times the
+    surrogate + IRS fallback chains and all-IRS chains (batched across
+    chains) from the same starts."""
+    import dataclasses
+    from triplelens_amd import engine as E, metropolis as M, surrogate as S
+    if args.tuning:
+        from triplelens_amd import autotune
+        autotune.apply(args.tuning)
+    name, cus = E.device_info()
+    truth = dataclasses.replace(c2, n_points=args.points)
+    obs, err = M.observed(truth)
+    starts = M.starts_near(truth, args.chains)
+    sur = S.Surrogate.load(args.model)
+    for _ in range(args.warmup):
+        M.run(starts, obs, err, 1, "surrogate", sur)
+        M.run(starts, obs, err, 1, "irs")
+    rs = M.run(starts, obs, err, args.proposals, "surrogate", sur)
+    ri = M.run(starts, obs, err, args.proposals, "irs")
+    sur_pps = rs["points"] / rs["wall_s"]
+    irs_pps = ri["points"] / ri["wall_s"]
+    errs = [float(np.max(np.abs(rs["first"][k] / ri["first"][k] - 1.0)))
+            for k in rs["first"] if k in ri["first"]]
+    n_prop = args.chains * args.proposals
+    line = {
+        "metric": "Metropolis fitting: light-curve points/s, surrogate+IRS fallback, 1 GPU",
+        "value": sur_pps,
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.proposals,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * rs["wall_s"] / args.proposals,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"Metropolis fitting: {args.chains} chains x {args.proposals} proposals x "
+                        f"{args.points}-point light curves on the C2 trajectory, 1% Gaussian steps "
+                        "of a, b, th, m2, m3, observed curve with 1% noise",
+            "device": name,
+            "cus": cus,
+            "all_irs_points_per_s": irs_pps,
+            "speedup_vs_all_irs": sur_pps / irs_pps,
+            "fallback_fraction": rs["fallback"] / max(1, rs["points"]),
+            "audit_fraction": rs["audited"] / max(1, rs["points"]),
+            "acceptance": sum(ch.accepted for ch in rs["chains"]) / n_prop,
+            "acceptance_all_irs": sum(ch.accepted for ch in ri["chains"]) / n_prop,
+            "max_rel_err_first_proposals": max(errs) if errs else None,
+            "bound_met": bool(errs) and max(errs) <= 1e-3,
+        },
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 # ---------------------------------------------------------------- main
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", choices=["surrogate", "plain"], default="surrogate")
+    ap.add_argument("--mode", choices=["surrogate", "plain", "metropolis"], default="surrogate",
+                    help="surrogate: the C2 headline (surrogate + IRS vs plain IRS); plain: "
+                         "plain IRS alone; metropolis: the Metropolis fitting workload")
+    ap.add_argument("--chains", type=int, default=8, help="metropolis: chains")
+    ap.add_argument("--proposals", type=int, default=8, help="metropolis: proposals per chain")
+    ap.add_argument("--points", type=int, default=2048, help="metropolis: points per curve")
     ap.add_argument("--model", default=MODEL, help="trained surrogate (.npz)")
     ap.add_argument("--tuning", default="", help="engine options saved by triplelens_amd.autotune")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -177,10 +244,12 @@ def main(argv=None):
         return 2
 
     c = c2_curve()
-    if args.mode == "surrogate" and not os.path.exists(args.model):
+    if args.mode in ("surrogate", "metropolis") and not os.path.exists(args.model):
         print(f"{args.model} is missing: train it with python -m triplelens_amd.train_surrogate "
               f"--out {args.model}", file=sys.stderr)
         return 2
+    if args.mode == "metropolis":
+        return metropolis_bench(args, c)
     cpu = None if args.no_cpu else cpu_baseline(c, args.cpu_cores)
 
     from triplelens_amd import engine as E

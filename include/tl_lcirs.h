@@ -70,6 +70,10 @@ void set_limb_darkening_cuda(void* lc, const char* model, double v);
 
 /* Frees an object (the reference never frees them). */
 void lcirs_delete(void* lc);
+/* A new object with the same geometry, grid, critical curves, options and
+ * root chain, without the last results (no set-up is recomputed).  NULL on
+ * error.  No counterpart in the reference. */
+void* lcirs_clone(void* lc);
 
 /* Per-point results of the last get_lc_irs, lcLength+1 values each:
  * magnifications (all of them, copy_lc stops at lcLength), ray counts and
@@ -118,6 +122,10 @@ int lcirs_copy_ring_amp(void* lc, double* ring, double* centreRoots, int* crossi
  * advances it.  For callers that fill only some positions
  * (lcirs_fill_seeds). */
 int lcirs_find_seeds(void* lc, double iniX, double iniY, double finX, double finY);
+/* lcirs_find_seeds of n objects in one engine run (one seed launch):
+ * objects lcs[k] with trajectories traj[4k..4k+3].  No counterpart in the
+ * reference. */
+int lcirs_batch_find_seeds(int n, void** lcs, const double* traj);
 int lcirs_copy_ray_counts(void* lc, uint64_t* out);
 int lcirs_copy_flags(void* lc, uint32_t* out);
 
@@ -155,6 +163,13 @@ int lcirs_fill_seeds(void* lc, int n, const double* pos, const int* nSeeds, cons
  * counterpart in the reference. */
 int lcirs_fill_positions(void* lc, int n, const int* idx, uint64_t* rays, double* amp,
                          uint32_t* flags);
+/* As lcirs_fill_positions, with the caller's seeds in place of the seed
+ * search's: position idx[i] gets nSeeds[i] pixel pairs at
+ * pix[2*(i*64+k)+{0,1}].  For seeds merged from several seed searches (a
+ * surrogate's fallback takes the union of two split searches).  No
+ * counterpart in the reference. */
+int lcirs_fill_positions_seeds(void* lc, int n, const int* idx, const int* nSeeds, const int* pix,
+                               uint64_t* rays, double* amp, uint32_t* flags);
 
 /* Engine counters of the last run: out[0] fill tasks, out[1] evaluated
  * pixels, out[2] runs inserted, out[3] duplicate runs, out[4] seed-kernel

@@ -423,6 +423,51 @@ def test_fill_positions_of_the_last_seed_search(E):
     assert np.array_equal(rays, rays0) and np.array_equal(A, A0)
     with pytest.raises(E.EngineError):
         lc.fill_positions(np.array([c.n_points]))
+    # lcirs_fill_positions_seeds: the search's own seeds give the same
+    # results; a seed set with duplicates and an extra seed of an image
+    # already seeded counts every ray once; no seeds count nothing
+    ns, pix = lc.seeds()
+    A, rays, _ = lc.fill_positions(idx, (ns[idx], pix[idx]))
+    assert np.array_equal(rays, rays0[idx]) and np.array_equal(A, A0[idx])
+    ns2, pix2 = ns[idx].copy(), pix[idx].copy()
+    for i in range(len(idx)):
+        k = ns2[i]
+        pix2[i, k:2 * k] = pix2[i, :k]
+        ns2[i] = 2 * k
+    A, rays, _ = lc.fill_positions(idx, (ns2, pix2))
+    assert np.array_equal(rays, rays0[idx]) and np.array_equal(A, A0[idx])
+    _, rays, _ = lc.fill_positions(idx, (np.zeros(len(idx), np.int32), pix[idx]))
+    assert not rays.any()
+
+
+def test_clones_and_batched_seed_searches(E):
+    """lcirs_clone copies an object's set-up (its light curve is the
+    original's to the bit), and lcirs_batch_find_seeds gives each object the
+    seeds its own lcirs_find_seeds gives."""
+    from triplelens_amd import workloads as W
+    c = W.c1()
+    lc = _new(E, c)
+    A0, rays0, _ = lc.lc_irs(*c.traj())
+    cl = _new(E, c).clone()
+    A1, rays1, _ = cl.lc_irs(*c.traj())
+    assert np.array_equal(rays0, rays1) and np.array_equal(A0, A1)
+    parts = [7, 13, 50]
+    seq = []
+    for p in parts:
+        o = _new(E, c)
+        o.set_seed_split(p)
+        o.find_seeds(*c.traj())
+        seq.append(o.seeds())
+    base = _new(E, c)
+    clones = [base.clone() for _ in parts]
+    for o, p in zip(clones, parts):
+        o.set_seed_split(p)
+    E.batch_find_seeds(clones, np.array([c.traj()] * len(parts)))
+    for o, (ns, pix) in zip(clones, seq):
+        ns2, pix2 = o.seeds()
+        assert np.array_equal(ns, ns2)
+        for i in range(len(ns)):
+            assert np.array_equal(pix[i, :ns[i]], pix2[i, :ns[i]]), i
 
 
 def test_binary_lens_matches_oracle_and_golden(E, oracle):

@@ -162,3 +162,38 @@ def test_hexadecapole_feature_round_trip():
         X = S.features((1.0, 1.0, 1.0, 0.1, 0.1), 1e-3, 0.5, np.zeros((1, 2)), np.zeros((1, 2)),
                        np.array([1.0]), np.array([2.0]), np.array([1.0]), ring)
         assert S.hex_term(X)[0] == pytest.approx(abs(t4[0]) / 2.0, rel=1e-9)
+
+
+def test_union_of_seed_sets():
+    """Shot positions take the union of two seed searches: every pixel
+    once, the first search's order first, at most 64."""
+    from triplelens_amd.surrogate import union_seeds
+    ns1 = np.array([2, 0, 64], np.int32)
+    ns2 = np.array([3, 1, 1], np.int32)
+    p1 = np.zeros((3, 64, 2), np.int32)
+    p2 = np.zeros((3, 64, 2), np.int32)
+    p1[0, :2] = [[5, 6], [7, 8]]
+    p2[0, :3] = [[7, 8], [1, 2], [5, 6]]
+    p2[1, 0] = [9, 9]
+    p1[2] = np.stack([np.arange(64), np.arange(64)], 1)
+    p2[2, 0] = [100, 100]
+    ns, pix = union_seeds(ns1, p1, ns2, p2)
+    assert ns.tolist() == [3, 1, 64]
+    assert pix[0, :3].tolist() == [[5, 6], [7, 8], [1, 2]]
+    assert pix[1, 0].tolist() == [9, 9]
+    assert np.array_equal(pix[2], p1[2])
+
+
+def test_extra_seed_searches_share_no_piece_boundary():
+    """The extra seed searches of the shot positions use piece counts whose
+    boundaries differ from the first search's everywhere but the start."""
+    from triplelens_amd import surrogate as S
+    sur = S.Surrogate(None, seed_piece=16)
+    n = 4096
+    p = sur.seed_parts(n)
+    parts = [p] + sur.seed_parts_extra(n)
+    assert len(set(parts)) == 4
+    bounds = [set(n * q // k for q in range(1, k)) for k in parts]
+    for i in range(4):
+        for j in range(i + 1, 4):
+            assert len(bounds[i] & bounds[j]) < 0.1 * len(bounds[i])

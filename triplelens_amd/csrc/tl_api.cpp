@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <memory>
 #include <mutex>
 #include <numeric>
 #include <stdexcept>
@@ -856,6 +857,24 @@ void* lcirs_new(double a, double b, double th, double m2, double m3, double sour
 
 void lcirs_delete(void* lc) { delete static_cast<Curve*>(lc); }
 
+void* lcirs_clone(void* lc) {
+  Curve* out = nullptr;
+  guarded([&] {
+    if (!lc) throw std::invalid_argument("null light-curve object");
+    std::unique_ptr<Curve> c(new Curve(*static_cast<Curve*>(lc)));
+    // the set-up, options and root chain, without the last results
+    c->lcVec.clear(); c->full.clear(); c->rays.clear(); c->flags.clear();
+    c->nSeeds.clear(); c->seedPix.clear(); c->pointAmp.clear(); c->centroid.clear();
+    c->causDist.clear(); c->pairShare.clear(); c->ringAmp.clear(); c->ringCross.clear();
+    c->centreRoots.clear();
+    c->hasSeeds = c->hasIRS = false;
+    c->amoebaOut = false;
+    out = c.release();
+    return 0;
+  });
+  return out;
+}
+
 void get_lc_irs(void* lc, double iniX, double iniY, double finX, double finY) {
   int rc = guarded([&] {
     if (!lc) throw std::invalid_argument("null light-curve object");
@@ -886,6 +905,7 @@ void get_lc(void* lc, double iniX, double iniY, double finX, double finY) {
 }
 
 void copy_lc(void* lc, double* out) {
+  g_err.clear();
   if (!lc || !out) return;
   const Curve* c = static_cast<Curve*>(lc);
   std::copy(c->lcVec.begin(), c->lcVec.end(), out);
@@ -999,7 +1019,27 @@ int lcirs_find_seeds(void* lc, double iniX, double iniY, double finX, double fin
   });
 }
 
+int lcirs_batch_find_seeds(int n, void** lcs, const double* traj) {
+  return guarded([&] {
+    if (n < 0 || (n > 0 && (!lcs || !traj))) throw std::invalid_argument("bad batch arguments");
+    if (n == 0) return 0;
+    Engine& E = engine();
+    std::lock_guard<std::mutex> lock(E.mu);
+    Job job;
+    for (int k = 0; k < n; k++) {
+      if (!lcs[k]) throw std::invalid_argument("null light-curve object in batch");
+      job.curves.push_back(static_cast<Curve*>(lcs[k]));
+      job.starts.push_back(mk(traj[4 * k], traj[4 * k + 1]));
+      job.ends.push_back(mk(traj[4 * k + 2], traj[4 * k + 3]));
+    }
+    job.runFill = false;
+    E.run(job);
+    return 0;
+  });
+}
+
 int lcirs_copy_lc_full(void* lc, double* out) {
+  g_err.clear();
   const Curve* c = static_cast<Curve*>(lc);
   if (!c || !c->hasIRS) return -1;
   std::copy(c->full.begin(), c->full.end(), out);
@@ -1007,6 +1047,7 @@ int lcirs_copy_lc_full(void* lc, double* out) {
 }
 
 int lcirs_copy_ray_counts(void* lc, uint64_t* out) {
+  g_err.clear();
   const Curve* c = static_cast<Curve*>(lc);
   if (!c || !c->hasIRS) return -1;
   std::copy(c->rays.begin(), c->rays.end(), out);
@@ -1014,6 +1055,7 @@ int lcirs_copy_ray_counts(void* lc, uint64_t* out) {
 }
 
 int lcirs_copy_flags(void* lc, uint32_t* out) {
+  g_err.clear();
   const Curve* c = static_cast<Curve*>(lc);
   if (!c || !c->hasSeeds) return -1;
   std::copy(c->flags.begin(), c->flags.end(), out);
@@ -1021,6 +1063,7 @@ int lcirs_copy_flags(void* lc, uint32_t* out) {
 }
 
 int lcirs_copy_seeds(void* lc, int* nSeeds, int* pix) {
+  g_err.clear();
   const Curve* c = static_cast<Curve*>(lc);
   if (!c || !c->hasSeeds) return -1;
   std::copy(c->nSeeds.begin(), c->nSeeds.end(), nSeeds);
@@ -1029,6 +1072,7 @@ int lcirs_copy_seeds(void* lc, int* nSeeds, int* pix) {
 }
 
 int lcirs_get_grid(void* lc, double* out) {
+  g_err.clear();
   const Curve* c = static_cast<Curve*>(lc);
   if (!c) return -1;
   out[0] = (double)c->N; out[1] = c->Ldbl; out[2] = c->step;
@@ -1043,6 +1087,7 @@ int lcirs_get_grid(void* lc, double* out) {
 }
 
 int lcirs_copy_cc_ca(void* lc, double* cc, double* ca) {
+  g_err.clear();
   const Curve* c = static_cast<Curve*>(lc);
   if (!c) return -1;
   std::memcpy(cc, c->cc.data(), c->cc.size() * sizeof(cx));
@@ -1105,8 +1150,17 @@ int lcirs_fill_seeds(void* lc, int n, const double* pos, const int* nSeeds, cons
 
 int lcirs_fill_positions(void* lc, int n, const int* idx, uint64_t* rays, double* amp,
                          uint32_t* flags) {
+  return lcirs_fill_positions_seeds(lc, n, idx, nullptr, nullptr, rays, amp, flags);
+}
+
+int lcirs_fill_positions_seeds(void* lc, int n, const int* idx, const int* nSeeds, const int* pix,
+                               uint64_t* rays, double* amp, uint32_t* flags) {
   return guarded([&] {
-    if (!lc || n <= 0 || !idx) throw std::invalid_argument("bad arguments");
+    if (!lc || n <= 0 || !idx || (!nSeeds) != (!pix)) throw std::invalid_argument("bad arguments");
+    if (nSeeds)
+      for (int i = 0; i < n; i++)
+        if (nSeeds[i] < 0 || nSeeds[i] > kMaxSeeds)
+          throw std::invalid_argument("seed count out of range (0 .. 64 per position)");
     Curve* c = static_cast<Curve*>(lc);
     if (!c->hasSeeds) throw std::runtime_error("no seed search has run on this object");
     const int npos = (int)c->nSeeds.size();
@@ -1129,9 +1183,15 @@ int lcirs_fill_positions(void* lc, int n, const int* idx, uint64_t* rays, double
       const int p = idx[i];
       const cx x = traj_pos(c->seedStart, c->seedEnd, (unsigned)p, c->lcLength);
       job.pos[i] = make_double2(x.re, x.im);
-      job.nSeeds[i] = c->nSeeds[p];
-      std::memcpy(&job.seeds[(size_t)i * kMaxSeeds], &c->seedPix[(size_t)p * kMaxSeeds * 2],
-                  kMaxSeeds * sizeof(int2));
+      if (nSeeds) {
+        job.nSeeds[i] = nSeeds[i];
+        std::memcpy(&job.seeds[(size_t)i * kMaxSeeds], &pix[(size_t)i * kMaxSeeds * 2],
+                    kMaxSeeds * sizeof(int2));
+      } else {
+        job.nSeeds[i] = c->nSeeds[p];
+        std::memcpy(&job.seeds[(size_t)i * kMaxSeeds], &c->seedPix[(size_t)p * kMaxSeeds * 2],
+                    kMaxSeeds * sizeof(int2));
+      }
       // the seed search's prediction, as get_lc_irs orders and sizes by it
       job.cost[i] = (float)(c->pointAmp[p] * area);
     }
@@ -1218,6 +1278,7 @@ int tl_math_selftest(int n, unsigned long long seed) {
 }
 
 int tl_last_stats(double* out) {
+  g_err.clear();
   const Stats& s = engine().stats;
   out[0] = s.tasks; out[1] = s.evals; out[2] = s.ins; out[3] = s.dup;
   out[4] = s.seedMs; out[5] = s.fillMs; out[6] = s.fillLaunches; out[7] = s.totalMs;
@@ -1227,6 +1288,7 @@ int tl_last_stats(double* out) {
 }
 
 int tl_set_option(const char* name, double value) {
+  g_err.clear();
   if (!name) return -1;
   Options& o = engine().opt;
   const std::string n(name);

@@ -44,6 +44,9 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lcirs_new.argtypes = [_D] * 6 + [ctypes.c_uint, ctypes.c_long]
     lib.lcirs_new.restype = _P
     lib.lcirs_delete.argtypes = [_P]
+    lib.lcirs_clone.argtypes = [_P]
+    lib.lcirs_clone.restype = _P
+    lib.lcirs_batch_find_seeds.argtypes = [ctypes.c_int, _P, _P]
     for f in ("get_lc", "get_lc_irs"):
         getattr(lib, f).argtypes = [_P, _D, _D, _D, _D]
         getattr(lib, f).restype = None
@@ -99,13 +102,21 @@ def device_info():
 class LightCurve:
     """One LightCurveIRS object of the engine."""
 
-    def __init__(self, a, b, th, m2, m3, rho, lc_len, ppr):
+    def __init__(self, a, b, th, m2, m3, rho, lc_len, ppr, _handle=None):
         lib = load()
         self._lib = lib
-        self.h = lib.lcirs_new(a, b, th, m2, m3, rho, lc_len, ppr)
+        self.h = _handle if _handle else lib.lcirs_new(a, b, th, m2, m3, rho, lc_len, ppr)
         if not self.h:
             raise EngineError(last_error() or "lcirs_new failed")
         self.lc_len = int(lc_len)
+
+    def clone(self):
+        """A new object with this one's set-up, options and root chain
+        (lcirs_clone), without its results."""
+        h = self._lib.lcirs_clone(self.h)
+        if not h:
+            raise EngineError(last_error() or "lcirs_clone failed")
+        return LightCurve(0, 0, 0, 0, 0, 0, self.lc_len, 0, _handle=h)
 
     def __del__(self):
         h = getattr(self, "h", None)
@@ -228,9 +239,11 @@ class LightCurve:
                                           flags.ctypes.data))
         return amp, rays, flags
 
-    def fill_positions(self, idx):
+    def fill_positions(self, idx, seeds=None):
         """GPU fill of positions idx (indices 0..lcLength) of the last seed
-        search with its seeds (lcirs_fill_positions): amp, rays, flags."""
+        search (lcirs_fill_positions): amp, rays, flags.  With seeds =
+        (nseeds, pix) for those positions ((n,), (n, 64, 2)), the given seeds
+        replace the search's (lcirs_fill_positions_seeds)."""
         idx = np.ascontiguousarray(idx, np.int32)
         n = len(idx)
         rays = np.zeros(n, np.uint64)
@@ -238,9 +251,19 @@ class LightCurve:
         flags = np.zeros(n, np.uint32)
         if n == 0:
             return amp, rays, flags
-        self._lib.lcirs_fill_positions.argtypes = [_P, ctypes.c_int, _P, _P, _P, _P]
-        _check(self._lib.lcirs_fill_positions(self.h, n, idx.ctypes.data, rays.ctypes.data,
-                                              amp.ctypes.data, flags.ctypes.data))
+        if seeds is None:
+            self._lib.lcirs_fill_positions.argtypes = [_P, ctypes.c_int, _P, _P, _P, _P]
+            _check(self._lib.lcirs_fill_positions(self.h, n, idx.ctypes.data, rays.ctypes.data,
+                                                  amp.ctypes.data, flags.ctypes.data))
+            return amp, rays, flags
+        ns = np.ascontiguousarray(seeds[0], np.int32)
+        pix = np.ascontiguousarray(seeds[1], np.int32).reshape(n, MAX_SEEDS, 2)
+        if len(ns) != n:
+            raise ValueError("one seed count per position")
+        self._lib.lcirs_fill_positions_seeds.argtypes = [_P, ctypes.c_int] + [_P] * 6
+        _check(self._lib.lcirs_fill_positions_seeds(self.h, n, idx.ctypes.data, ns.ctypes.data,
+                                                    pix.ctypes.data, rays.ctypes.data,
+                                                    amp.ctypes.data, flags.ctypes.data))
         return amp, rays, flags
 
 
@@ -255,6 +278,16 @@ def ccc_batch(params):
     lib.tl_ccc_batch.argtypes = [ctypes.c_int, _P, _P, _P]
     _check(lib.tl_ccc_batch(n, p.ctypes.data, cc.ctypes.data, ca.ctypes.data))
     return cc, ca
+
+
+def batch_find_seeds(curves, trajs):
+    """The seed searches of many LightCurve objects in one engine run
+    (lcirs_batch_find_seeds); trajs: (n, 4)."""
+    lib = load()
+    n = len(curves)
+    arr = (ctypes.c_void_p * n)(*[c.h for c in curves])
+    t = np.ascontiguousarray(trajs, np.float64).reshape(n, 4)
+    _check(lib.lcirs_batch_find_seeds(n, arr, t.ctypes.data))
 
 
 def batch_irs(curves, trajs):

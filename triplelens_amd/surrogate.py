@@ -289,6 +289,40 @@ def training_set(curves):
              "cross": np.concatenate(cr)})
 
 
+def union_seeds(ns1, pix1, ns2, pix2):
+    """Per position, the seed pixels of either search, each once (first
+    search's order, then the second's new ones), at most 64."""
+    n = len(ns1)
+    m = _E.MAX_SEEDS
+    ns1 = np.asarray(ns1, np.int64)
+    ns2 = np.asarray(ns2, np.int64)
+    # every seed as (position, rank, x, y); rank orders the first search's
+    # seeds before the second's
+    slot = np.arange(m)
+    take1 = slot[None, :] < ns1[:, None]
+    take2 = slot[None, :] < ns2[:, None]
+    pos = np.concatenate([np.nonzero(take1)[0], np.nonzero(take2)[0]])
+    rank = np.concatenate([np.nonzero(take1)[1], m + np.nonzero(take2)[1]])
+    xy = np.concatenate([np.asarray(pix1)[take1], np.asarray(pix2)[take2]]).astype(np.int64)
+    ns = np.zeros(n, np.int32)
+    pix = np.zeros((n, m, 2), np.int32)
+    if len(pos) == 0:
+        return ns, pix
+    order = np.lexsort((rank, xy[:, 1], xy[:, 0], pos))
+    pos, rank, xy = pos[order], rank[order], xy[order]
+    first = np.ones(len(pos), bool)
+    first[1:] = (pos[1:] != pos[:-1]) | np.any(xy[1:] != xy[:-1], axis=1)
+    pos, rank, xy = pos[first], rank[first], xy[first]
+    order = np.lexsort((rank, pos))
+    pos, xy = pos[order], xy[order]
+    start = np.searchsorted(pos, np.arange(n))
+    k = np.arange(len(pos)) - start[pos]
+    keep = k < m
+    pix[pos[keep], k[keep]] = xy[keep]
+    ns[:] = np.minimum(np.bincount(pos, minlength=n), m)
+    return ns, pix
+
+
 class Surrogate:
     """Network plus gate: surrogate where it is trusted, ray shooting
     elsewhere and on a random audit sample.  The gate accepts a position
@@ -311,12 +345,27 @@ class Surrogate:
         self.sigma_max = np.broadcast_to(np.asarray(sigma_max, np.float64),
                                          (len(self.band_edges) + 1,)).copy()
         self.audit = audit
+        # shot positions are filled from the union of four split searches;
+        # positions per piece of the three extra ones (None: seed_piece)
+        self.seed_union = True
+        self.extra_piece = None
         # positions per independent piece of the seed search
         # (lcirs_set_seed_split); None keeps the reference's single chain.
         # On the C2 curve, pieces of 64/32/16/8 positions take 13/6.7/4.8/2.8
         # ms and give ray counts other than the single chain's at 4/12/16/19
         # of 4096 positions.
         self.seed_piece = seed_piece
+
+    def seed_parts_extra(self, n_positions: int) -> list:
+        """Pieces of the extra seed searches of the shot positions: P+1, P-1
+        and P+3 for the first search's P, so that the searches share no
+        piece boundary but the curve's start and every position is reached
+        by different chain histories."""
+        p = self.seed_parts(n_positions)
+        if self.extra_piece:
+            p = max(1, min(65536, n_positions // self.extra_piece))
+        first = self.seed_parts(n_positions)
+        return [q for q in (p + 1, p - 1, p + 3) if q >= 1 and q != first]
 
     def seed_parts(self, n_positions: int) -> int:
         if not self.seed_piece:
@@ -423,7 +472,33 @@ class Surrogate:
         errs = np.zeros(0)
         if shoot.any():
             idx = np.nonzero(shoot)[0]
-            A_irs, rays, flags = lc.fill_positions(idx)
+            seeds = None
+            if self.seed_union and self.seed_piece:
+                # A split search can miss the seed of a whole image where a
+                # caustic trial solve converges differently from its fresh
+                # start.  The shot positions take the union of the seeds of
+                # four split searches with different piece boundaries; a
+                # seed only adds a component that is really there (its
+                # pixel's ray lands in the source).
+                ns, pix = lc.seeds()
+                seeds = (ns[idx], pix[idx])
+                # the extra searches run on clones of the object, in one
+                # seed launch; the clones are kept with the object
+                extra = self.seed_parts_extra(c.n_points)
+                clones = getattr(lc, "_seed_clones", None)
+                if clones is None or len(clones) != len(extra):
+                    clones = [lc.clone() for _ in extra]
+                    lc._seed_clones = clones
+                for cl, parts in zip(clones, extra):
+                    cl.set_ring(False)
+                    cl.set_seed_split(parts)
+                _E.batch_find_seeds(clones, np.array([c.traj()] * len(clones)))
+                if stats is not None:
+                    stats.append(_E.last_stats())
+                for cl in clones:
+                    ns2, pix2 = cl.seeds()
+                    seeds = union_seeds(seeds[0], seeds[1], ns2[idx], pix2[idx])
+            A_irs, rays, flags = lc.fill_positions(idx, seeds)
             if stats is not None:
                 stats.append(_E.last_stats())
             bad = int(np.bitwise_or.reduce(flags)) & ~_E.FLAG_EDGE
