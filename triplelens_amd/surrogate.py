@@ -452,6 +452,19 @@ class Surrogate:
         p = self.net.predict(X)
         return X[:, F_HEX] + p[:, 0], np.exp(np.clip(p[:, 1], -30.0, 30.0))
 
+    def _extra_clones(self, lc, c):
+        """Clones of engine object lc for the extra seed searches, made once
+        and kept with it, set to their piece counts without disk samples."""
+        extra = self.seed_parts_extra(c.n_points)
+        clones = getattr(lc, "_seed_clones", None)
+        if clones is None or len(clones) != len(extra):
+            clones = [lc.clone() for _ in extra]
+            lc._seed_clones = clones
+        for cl, parts in zip(clones, extra):
+            cl.set_ring(False)
+            cl.set_seed_split(parts)
+        return clones
+
     def light_curve(self, lc, c, rng=None, stats=None):
         """Magnification at the lcLength+1 positions of curve c on engine object
         lc.  Returns (A, info) with info = fallback, audited masks and the
@@ -460,7 +473,18 @@ class Surrogate:
         rng = np.random.default_rng(0) if rng is None else rng
         lc.set_ring(True)
         lc.set_seed_split(self.seed_parts(c.n_points))
-        lc.find_seeds(*c.traj())
+        # A split search can miss the seed of a whole image where a caustic
+        # trial solve converges differently from its fresh start.  Positions
+        # that are ray-shot take the union of the seeds of four split
+        # searches with different piece boundaries; a seed only adds a
+        # component that is really there (its pixel's ray lands in the
+        # source).  The three extra searches run on clones of the object
+        # (kept with it) in the same seed launch as the first.
+        clones = self._extra_clones(lc, c) if self.seed_union and self.seed_piece else []
+        if clones:
+            _E.batch_find_seeds([lc] + clones, np.array([c.traj()] * (1 + len(clones))))
+        else:
+            lc.find_seeds(*c.traj())
         if stats is not None:
             stats.append(_E.last_stats())
         ap, X, cross = _point_features(lc, c)
@@ -473,28 +497,9 @@ class Surrogate:
         if shoot.any():
             idx = np.nonzero(shoot)[0]
             seeds = None
-            if self.seed_union and self.seed_piece:
-                # A split search can miss the seed of a whole image where a
-                # caustic trial solve converges differently from its fresh
-                # start.  The shot positions take the union of the seeds of
-                # four split searches with different piece boundaries; a
-                # seed only adds a component that is really there (its
-                # pixel's ray lands in the source).
+            if clones:
                 ns, pix = lc.seeds()
                 seeds = (ns[idx], pix[idx])
-                # the extra searches run on clones of the object, in one
-                # seed launch; the clones are kept with the object
-                extra = self.seed_parts_extra(c.n_points)
-                clones = getattr(lc, "_seed_clones", None)
-                if clones is None or len(clones) != len(extra):
-                    clones = [lc.clone() for _ in extra]
-                    lc._seed_clones = clones
-                for cl, parts in zip(clones, extra):
-                    cl.set_ring(False)
-                    cl.set_seed_split(parts)
-                _E.batch_find_seeds(clones, np.array([c.traj()] * len(clones)))
-                if stats is not None:
-                    stats.append(_E.last_stats())
                 for cl in clones:
                     ns2, pix2 = cl.seeds()
                     seeds = union_seeds(seeds[0], seeds[1], ns2[idx], pix2[idx])
