@@ -207,6 +207,7 @@ struct Options {
   int scoutJump = 512;
   int scoutWidth = 2048;    // runs wider than this shorten the scout jump (0: never)
   int scoutRetry = 1;       // a scout that finds nothing is sent again half as far
+  int scoutShare = 0;       // sweeps queue their scouts in the shared queue at once
   int leapEvery = 4;        // every this many scouts a chain sends a leap scout (0: never)
   int leapJumps = 8;        // ... this many jumps ahead
   int pollBackoff = 6;
@@ -619,6 +620,7 @@ void Engine::run(Job& job) {
       fa.scoutJump = opt.scoutJump;
       fa.scoutWidth = opt.scoutWidth;
       fa.scoutRetry = opt.scoutRetry;
+      fa.scoutShare = opt.scoutShare;
       fa.leapEvery = opt.leapEvery;
       fa.leapJumps = opt.leapJumps;
       fa.pollBackoff = opt.pollBackoff;
@@ -1300,6 +1302,7 @@ int tl_set_option(const char* name, double value) {
   else if (n == "scout_width") o.scoutWidth = std::max(0, (int)value);
   else if (n == "scout_retry") o.scoutRetry = value != 0.0 ? 1 : 0;
   else if (n == "leap_every") o.leapEvery = std::max(0, (int)value);
+  else if (n == "scout_share") o.scoutShare = value != 0.0 ? 1 : 0;
   else if (n == "leap_jumps") o.leapJumps = std::min(64, std::max(2, (int)value));
   else if (n == "poll_backoff") o.pollBackoff = std::min(6, std::max(0, (int)value));
   else if (n == "share_idle") o.shareIdle = value != 0.0 ? 1 : 0;

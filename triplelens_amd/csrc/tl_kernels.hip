@@ -1238,8 +1238,12 @@ __device__ __forceinline__ void handle_run(TaskState& S, int xl, int xr, long lo
           if (sa < 0) sa = 0;
           if (sb > N - 1) sb = N - 1;
           if (sa <= sb) {
+            // to the wave's ring (the shared queue while some wave is idle);
+            // sweeps of many small images send many scouts that miss, and
+            // queuing each of them at once makes the shared queue's head
+            // and tail the bottleneck
             Task f = make_task(t.b, (int)rt, (int)sa, (int)sb, -1, 2, d, center, J << 16);
-            f.kd |= kQueueOnly | kShare;
+            f.kd |= kQueueOnly | (A.scoutShare ? kShare : 0);
             follow(S, f);
           }
         }
