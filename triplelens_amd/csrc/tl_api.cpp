@@ -208,6 +208,7 @@ struct Options {
   int scoutWidth = 2048;    // runs wider than this shorten the scout jump (0: never)
   int scoutRetry = 1;       // a scout that finds nothing is sent again half as far
   int scoutShare = 0;       // sweeps queue their scouts in the shared queue at once
+  double shareRays = 1e7;   // ... on positions predicted to have this many rays (0: never)
   int leapEvery = 4;        // every this many scouts a chain sends a leap scout (0: never)
   int leapJumps = 8;        // ... this many jumps ahead
   int pollBackoff = 6;
@@ -504,6 +505,7 @@ void Engine::run(Job& job) {
     HIPCHK(hipMemcpy(dSeeds.p, job.seeds.data(), (size_t)nPts * kMaxSeeds * sizeof(int2),
                      hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(dFlags.p, flags.data(), nPts * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dCost.p, cost.data(), nPts * sizeof(float), hipMemcpyHostToDevice));
   }
 
   std::vector<uint64_t> ptRays(nPts, 0);
@@ -621,6 +623,8 @@ void Engine::run(Job& job) {
       fa.scoutWidth = opt.scoutWidth;
       fa.scoutRetry = opt.scoutRetry;
       fa.scoutShare = opt.scoutShare;
+      fa.ptCost = dCost.p;
+      fa.shareRays = (float)opt.shareRays;
       fa.leapEvery = opt.leapEvery;
       fa.leapJumps = opt.leapJumps;
       fa.pollBackoff = opt.pollBackoff;
@@ -1303,6 +1307,7 @@ int tl_set_option(const char* name, double value) {
   else if (n == "scout_retry") o.scoutRetry = value != 0.0 ? 1 : 0;
   else if (n == "leap_every") o.leapEvery = std::max(0, (int)value);
   else if (n == "scout_share") o.scoutShare = value != 0.0 ? 1 : 0;
+  else if (n == "share_rays") o.shareRays = std::max(0.0, value);
   else if (n == "leap_jumps") o.leapJumps = std::min(64, std::max(2, (int)value));
   else if (n == "poll_backoff") o.pollBackoff = std::min(6, std::max(0, (int)value));
   else if (n == "share_idle") o.shareIdle = value != 0.0 ? 1 : 0;

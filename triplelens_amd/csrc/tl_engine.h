@@ -123,6 +123,8 @@ struct FillArgs {
   int scoutWidth;             // runs wider than this shorten the jump (0: never)
   int scoutRetry;             // a scout that finds nothing is sent again half as far
   int scoutShare;             // sweeps queue their scouts in the shared queue at once
+  const float* ptCost;        // predicted rays per position
+  float shareRays;            // ... from which its sweeps' scouts are queued at once (0: never)
   int leapEvery;              // every this many scouts a chain sends a leap scout (0: never)
   int leapJumps;              // ... this many jumps ahead
   int pollBackoff;            // idle polls sleep up to 2^pollBackoff x 1024 cycles

@@ -1019,6 +1019,7 @@ struct TaskState {
   const FillArgs* A;
   Task t;
   int curB;                 // batch point whose constants R holds (-1: none)
+  bool heavy;               // its predicted rays reach shareRays
   RowCtx R;
   int kind, dir, gp, N;
   int xa, xb;               // the known run next to the scanned row (kind 0:
@@ -1238,12 +1239,14 @@ __device__ __forceinline__ void handle_run(TaskState& S, int xl, int xr, long lo
           if (sa < 0) sa = 0;
           if (sb > N - 1) sb = N - 1;
           if (sa <= sb) {
-            // to the wave's ring (the shared queue while some wave is idle);
-            // sweeps of many small images send many scouts that miss, and
-            // queuing each of them at once makes the shared queue's head
-            // and tail the bottleneck
+            // to the wave's ring (the shared queue while some wave is idle),
+            // or to the shared queue at once on the images of positions
+            // predicted heavy, whose scouts mostly land: sweeps of many
+            // small images send many scouts that miss, and queuing each of
+            // them at once makes the shared queue's head and tail the
+            // bottleneck
             Task f = make_task(t.b, (int)rt, (int)sa, (int)sb, -1, 2, d, center, J << 16);
-            f.kd |= kQueueOnly | (A.scoutShare ? kShare : 0);
+            f.kd |= kQueueOnly | ((A.scoutShare || S.heavy) ? kShare : 0);
             follow(S, f);
           }
         }
@@ -1398,6 +1401,7 @@ __device__ __forceinline__ void process_task(TaskState& S) {
     S.R.ldA = unid(C.ldA);
     S.R.ldB = unid(C.ldB);
     S.R.binIrs = uni(C.binIrs);
+    S.heavy = A.shareRays > 0.0f && A.ptCost[S.gp] >= A.shareRays;
     S.R.N = S.N;
     const double2 pos = A.ptPos[S.gp];
     S.R.px = unid(pos.x);
