@@ -170,6 +170,13 @@ int lcirs_fill_positions(void* lc, int n, const int* idx, uint64_t* rays, double
  * counterpart in the reference. */
 int lcirs_fill_positions_seeds(void* lc, int n, const int* idx, const int* nSeeds, const int* pix,
                                uint64_t* rays, double* amp, uint32_t* flags);
+/* lcirs_fill_positions(_seeds) of n objects in one engine run: object k
+ * fills counts[k] positions, taken in order from the flat arrays idx (and,
+ * unless nSeeds and pix are NULL, its seeds there); the outputs are flat
+ * in the same order.  No counterpart in the reference. */
+int lcirs_batch_fill_positions(int n, void** lcs, const int* counts, const int* idx,
+                               const int* nSeeds, const int* pix, uint64_t* rays, double* amp,
+                               uint32_t* flags);
 
 /* Engine counters of the last run: out[0] fill tasks, out[1] evaluated
  * pixels, out[2] runs inserted, out[3] duplicate runs, out[4] seed-kernel

@@ -254,6 +254,7 @@ struct Job {          // one engine run over a set of curves
   std::vector<int> nSeeds;
   std::vector<int2> seeds;
   std::vector<float> cost;
+  std::vector<int> nPos;  // positions per curve (runSeeds false; empty: all pos to one curve)
 };
 
 struct Engine {
@@ -401,7 +402,8 @@ void Engine::run(Job& job) {
   std::vector<CurveDev> hc(nc);
   std::vector<int> pointBase(nc + 1, 0);
   for (int k = 0; k < nc; k++) {
-    const int npts = job.runSeeds ? (int)job.curves[k]->lcLength + 1 : (int)job.pos.size();
+    const int npts = job.runSeeds ? (int)job.curves[k]->lcLength + 1
+                                  : (job.nPos.empty() ? (int)job.pos.size() : job.nPos[k]);
     pointBase[k + 1] = pointBase[k] + npts;
   }
   const int nPts = pointBase[nc];
@@ -1157,6 +1159,73 @@ int lcirs_fill_seeds(void* lc, int n, const double* pos, const int* nSeeds, cons
 int lcirs_fill_positions(void* lc, int n, const int* idx, uint64_t* rays, double* amp,
                          uint32_t* flags) {
   return lcirs_fill_positions_seeds(lc, n, idx, nullptr, nullptr, rays, amp, flags);
+}
+
+int lcirs_batch_fill_positions(int n, void** lcs, const int* counts, const int* idx,
+                               const int* nSeeds, const int* pix, uint64_t* rays, double* amp,
+                               uint32_t* flags) {
+  return guarded([&] {
+    if (n < 0 || (n > 0 && (!lcs || !counts)) || (!nSeeds) != (!pix))
+      throw std::invalid_argument("bad batch arguments");
+    long long total = 0;
+    for (int k = 0; k < n; k++) {
+      if (!lcs[k]) throw std::invalid_argument("null light-curve object in batch");
+      if (counts[k] < 0) throw std::invalid_argument("negative position count");
+      const Curve* c = static_cast<Curve*>(lcs[k]);
+      if (counts[k] > 0 && !c->hasSeeds)
+        throw std::runtime_error("no seed search has run on an object of the batch");
+      for (long long i = total; i < total + counts[k]; i++)
+        if (idx[i] < 0 || idx[i] >= (int)c->nSeeds.size())
+          throw std::invalid_argument("position index out of range (0 .. lcLength)");
+      total += counts[k];
+    }
+    if (total == 0) return 0;
+    if (nSeeds)
+      for (long long i = 0; i < total; i++)
+        if (nSeeds[i] < 0 || nSeeds[i] > kMaxSeeds)
+          throw std::invalid_argument("seed count out of range (0 .. 64 per position)");
+    Engine& E = engine();
+    std::lock_guard<std::mutex> lock(E.mu);
+    Job job;
+    job.runSeeds = false;
+    job.pos.resize(total);
+    job.nSeeds.resize(total);
+    job.seeds.resize((size_t)total * kMaxSeeds);
+    job.cost.resize(total);
+    long long off = 0;
+    for (int k = 0; k < n; k++) {
+      if (counts[k] == 0) continue;
+      Curve* c = static_cast<Curve*>(lcs[k]);
+      job.curves.push_back(c);
+      job.starts.push_back(mk(0, 0));
+      job.ends.push_back(mk(0, 0));
+      job.nPos.push_back(counts[k]);
+      const double area = 3.141592653589793 * (double)c->ppr * (double)c->ppr;
+      for (long long i = off; i < off + counts[k]; i++) {
+        const int p = idx[i];
+        const cx x = traj_pos(c->seedStart, c->seedEnd, (unsigned)p, c->lcLength);
+        job.pos[i] = make_double2(x.re, x.im);
+        const int* src = nSeeds ? &pix[(size_t)i * kMaxSeeds * 2] : &c->seedPix[(size_t)p * kMaxSeeds * 2];
+        job.nSeeds[i] = nSeeds ? nSeeds[i] : c->nSeeds[p];
+        std::memcpy(&job.seeds[(size_t)i * kMaxSeeds], src, kMaxSeeds * sizeof(int2));
+        job.cost[i] = (float)(c->pointAmp[p] * area);
+      }
+      off += counts[k];
+    }
+    E.run(job);
+    off = 0;
+    for (int k = 0; k < n; k++) {
+      if (counts[k] == 0) continue;
+      const Curve* c = static_cast<Curve*>(lcs[k]);
+      for (int i = 0; i < counts[k]; i++) {
+        if (rays) rays[off + i] = c->rays[i];
+        if (amp) amp[off + i] = c->full[i];
+        if (flags) flags[off + i] = c->flags[i];
+      }
+      off += counts[k];
+    }
+    return 0;
+  });
 }
 
 int lcirs_fill_positions_seeds(void* lc, int n, const int* idx, const int* nSeeds, const int* pix,

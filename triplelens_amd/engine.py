@@ -290,6 +290,40 @@ def batch_find_seeds(curves, trajs):
     _check(lib.lcirs_batch_find_seeds(n, arr, t.ctypes.data))
 
 
+def batch_fill_positions(curves, idx_list, seeds_list=None):
+    """lcirs_batch_fill_positions: positions idx_list[k] of curves[k]'s last
+    seed search, all in one engine run, with (unless seeds_list is None) the
+    seeds seeds_list[k] = (nseeds, pix); returns [(amp, rays, flags)]."""
+    lib = load()
+    n = len(curves)
+    counts = np.array([len(i) for i in idx_list], np.int32)
+    tot = int(counts.sum())
+    idx = np.ascontiguousarray(np.concatenate([np.asarray(i, np.int32) for i in idx_list])
+                               if n else np.zeros(0, np.int32), np.int32)
+    rays = np.zeros(tot, np.uint64)
+    amp = np.zeros(tot)
+    flags = np.zeros(tot, np.uint32)
+    ns_p = pix_p = None
+    if seeds_list is not None:
+        ns = np.ascontiguousarray(np.concatenate([s[0] for s in seeds_list]), np.int32)
+        pix = np.ascontiguousarray(np.concatenate([np.asarray(s[1]).reshape(-1, MAX_SEEDS, 2)
+                                                   for s in seeds_list]), np.int32)
+        if len(ns) != tot:
+            raise ValueError("one seed count per position")
+        ns_p, pix_p = ns.ctypes.data, pix.ctypes.data
+    arr = (ctypes.c_void_p * max(1, n))(*[c.h for c in curves])
+    lib.lcirs_batch_fill_positions.argtypes = [ctypes.c_int] + [_P] * 8
+    _check(lib.lcirs_batch_fill_positions(n, arr, counts.ctypes.data, idx.ctypes.data, ns_p,
+                                          pix_p, rays.ctypes.data, amp.ctypes.data,
+                                          flags.ctypes.data))
+    out, o = [], 0
+    for k in range(n):
+        m = int(counts[k])
+        out.append((amp[o:o + m], rays[o:o + m], flags[o:o + m]))
+        o += m
+    return out
+
+
 def batch_irs(curves, trajs):
     """One engine run over many LightCurve objects; trajs: (n, 4)."""
     lib = load()

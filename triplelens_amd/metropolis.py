@@ -9,9 +9,9 @@ parameter's value (a, b, m2, m3) and `step` radians of th, the trajectory,
 source radius and limb darkening stay fixed, and the light curve is the
 extended-source curve:
 
-- mode "surrogate": Surrogate.light_curve (the network, with ray shooting of
-  the positions its gate rejects and of a 1% audit), one proposal after
-  another;
+- mode "surrogate": Surrogate.light_curves (the network, with ray shooting
+  of the positions its gate rejects and of a 1% audit), the chains'
+  proposals of one step sharing each engine run;
 - mode "irs": plain ray shooting of every position, the chains' proposals of
   one step in one batched engine run (lcirs_batch_irs).
 
@@ -113,7 +113,7 @@ def run(starts: list[Curve], obs, err, n_props: int, mode: str = "surrogate", su
     def curves_of(cs):
         if mode == "irs":
             return exact_curves(cs)
-        return [sur.light_curve(_new(c), c, audit_rng)[0] for c in cs]
+        return [a for a, _ in sur.light_curves([_new(c) for c in cs], cs, audit_rng)]
 
     init = curves_of(starts)
     chains = [Chain(c, a, chi2(a, obs, err), np.random.default_rng([seed, k, 1]),
@@ -128,9 +128,12 @@ def run(starts: list[Curve], obs, err, n_props: int, mode: str = "surrogate", su
         if mode == "irs":
             amps = dict(zip(live, exact_curves([props[k] for k in live]))) if live else {}
         else:
+            # the step's proposals share each engine run (seed searches,
+            # network, fills of the shot positions)
             amps = {}
-            for k in live:
-                amp, info = sur.light_curve(_new(props[k]), props[k], audit_rng)
+            res = sur.light_curves([_new(props[k]) for k in live], [props[k] for k in live],
+                                   audit_rng) if live else []
+            for k, (amp, info) in zip(live, res):
                 amps[k] = amp
                 n_fb += int(info["fallback"].sum())
                 n_aud += int(info["audited"].sum())

@@ -470,51 +470,76 @@ class Surrogate:
         lc.  Returns (A, info) with info = fallback, audited masks and the
         audit's relative errors.  With a list `stats`, the engine statistics
         (engine.last_stats) of each engine run are appended to it."""
+        return self.light_curves([lc], [c], rng, stats)[0]
+
+    def light_curves(self, lcs, curves, rng=None, stats=None):
+        """light_curve of several curves (engine objects lcs[k] over curves[k])
+        with each engine step shared: one seed launch for all of them, one
+        network call, one fill of every shot position.  Returns a list of
+        (A, info)."""
         rng = np.random.default_rng(0) if rng is None else rng
-        lc.set_ring(True)
-        lc.set_seed_split(self.seed_parts(c.n_points))
         # A split search can miss the seed of a whole image where a caustic
         # trial solve converges differently from its fresh start.  Positions
         # that are ray-shot take the union of the seeds of four split
         # searches with different piece boundaries; a seed only adds a
         # component that is really there (its pixel's ray lands in the
-        # source).  The three extra searches run on clones of the object
+        # source).  The three extra searches run on clones of each object
         # (kept with it) in the same seed launch as the first.
-        clones = self._extra_clones(lc, c) if self.seed_union and self.seed_piece else []
-        if clones:
-            _E.batch_find_seeds([lc] + clones, np.array([c.traj()] * (1 + len(clones))))
-        else:
-            lc.find_seeds(*c.traj())
+        union = bool(self.seed_union and self.seed_piece)
+        objs, trajs, clones_of = [], [], []
+        for lc, c in zip(lcs, curves):
+            lc.set_ring(True)
+            lc.set_seed_split(self.seed_parts(c.n_points))
+            cl = self._extra_clones(lc, c) if union else []
+            clones_of.append(cl)
+            objs += [lc] + cl
+            trajs += [c.traj()] * (1 + len(cl))
+        _E.batch_find_seeds(objs, np.array(trajs))
         if stats is not None:
             stats.append(_E.last_stats())
-        ap, X, cross = _point_features(lc, c)
-        mu, sigma = self.predict(X)
-        A = ap * np.exp(mu)
-        fallback = ~self.accepts(X, cross, sigma) | ~np.isfinite(A) | ~(ap > 0)
-        audited = ~fallback & (rng.random(len(A)) < self.audit)
-        shoot = fallback | audited
-        errs = np.zeros(0)
-        if shoot.any():
-            idx = np.nonzero(shoot)[0]
-            seeds = None
-            if clones:
+        feats = [_point_features(lc, c) for lc, c in zip(lcs, curves)]
+        mu_all, sigma_all = self.predict(np.concatenate([f[1] for f in feats]))
+        parts, shots, seeds_list, shot_k = [], [], [], []
+        o = 0
+        for k, ((ap, X, cross), lc) in enumerate(zip(feats, lcs)):
+            m = len(ap)
+            mu, sigma = mu_all[o:o + m], sigma_all[o:o + m]
+            o += m
+            A = ap * np.exp(mu)
+            fallback = ~self.accepts(X, cross, sigma) | ~np.isfinite(A) | ~(ap > 0)
+            audited = ~fallback & (rng.random(len(A)) < self.audit)
+            shoot = fallback | audited
+            parts.append([A, fallback, audited])
+            if shoot.any():
+                idx = np.nonzero(shoot)[0]
                 ns, pix = lc.seeds()
-                seeds = (ns[idx], pix[idx])
-                for cl in clones:
+                sd = (ns[idx], pix[idx])
+                for cl in clones_of[k]:
                     ns2, pix2 = cl.seeds()
-                    seeds = union_seeds(seeds[0], seeds[1], ns2[idx], pix2[idx])
-            A_irs, rays, flags = lc.fill_positions(idx, seeds)
+                    sd = union_seeds(sd[0], sd[1], ns2[idx], pix2[idx])
+                shots.append(idx)
+                seeds_list.append(sd)
+                shot_k.append(k)
+        out = [None] * len(lcs)
+        if shots:
+            res = _E.batch_fill_positions([lcs[k] for k in shot_k], shots,
+                                          seeds_list if union else None)
             if stats is not None:
                 stats.append(_E.last_stats())
-            bad = int(np.bitwise_or.reduce(flags)) & ~_E.FLAG_EDGE
-            if bad:
-                raise SurrogateError(f"ray shooting reported status flags {bad:#x}")
-            full = np.zeros(len(A))
-            full[idx] = A_irs
-            errs = np.abs(A[audited] - full[audited]) / full[audited]
-            A = np.where(fallback, full, A)
-            info_rays = int(rays.sum())
-        else:
-            info_rays = 0
-        return A, {"fallback": fallback, "audited": audited, "audit_rel_err": errs,
-                   "rays": info_rays}
+            for k, idx, (A_irs, rays, flags) in zip(shot_k, shots, res):
+                bad = int(np.bitwise_or.reduce(flags)) & ~_E.FLAG_EDGE
+                if bad:
+                    raise SurrogateError(f"ray shooting reported status flags {bad:#x}")
+                A, fallback, audited = parts[k]
+                full = np.zeros(len(A))
+                full[idx] = A_irs
+                errs = np.abs(A[audited] - full[audited]) / full[audited]
+                A = np.where(fallback, full, A)
+                out[k] = (A, {"fallback": fallback, "audited": audited, "audit_rel_err": errs,
+                              "rays": int(rays.sum())})
+        for k in range(len(lcs)):
+            if out[k] is None:
+                A, fallback, audited = parts[k]
+                out[k] = (A, {"fallback": fallback, "audited": audited,
+                              "audit_rel_err": np.zeros(0), "rays": 0})
+        return out
