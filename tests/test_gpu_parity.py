@@ -204,6 +204,28 @@ def test_scout_options_keep_results(E):
         assert np.array_equal(amp, runs[0])
 
 
+def test_graph_launches_keep_results(E):
+    """With use_graph each fill batch (reset, clears, fill, resolve) runs as
+    one captured hipGraph.  Alternating graph and direct runs of C1 and of
+    the window around the C2 peak give the direct runs' ray counts and
+    magnifications every time."""
+    from triplelens_amd import workloads as W
+    g = golden("c2_window.npz")
+    cases = [W.c1(), curve_from_params(g["params"])]
+    ref = [_new(E, c).lc_irs(*c.traj()) for c in cases]
+    try:
+        for rep in range(3):
+            for use in (1, 0):
+                E.set_option("use_graph", use)
+                for c, (A0, r0, _) in zip(cases, ref):
+                    A, r, flags = _new(E, c).lc_irs(*c.traj())
+                    assert np.array_equal(r, r0) and np.array_equal(A, A0), (rep, use)
+                    _assert_no_status(E, flags)
+    finally:
+        E.set_option("use_graph", 1)
+    assert np.array_equal(ref[1][1], g["rays"])
+
+
 # ------------------------------------------------------------ seeded fills
 def test_fill_from_given_seeds(E, oracle):
     """lcirs_fill_seeds on hand-made seed lists: none, duplicates, misses,

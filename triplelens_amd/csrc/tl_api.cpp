@@ -207,6 +207,7 @@ struct Options {
   int scoutJump = 512;
   int scoutWidth = 2048;    // runs wider than this shorten the scout jump (0: never)
   int scoutRetry = 1;       // a scout that finds nothing is sent again half as far
+  int useGraph = 1;         // each batch's reset, fill and resolve launched as one hipGraph
   int scoutShare = 0;       // sweeps queue their scouts in the shared queue at once
   double shareRays = 1e7;   // ... on positions predicted to have this many rays (0: never)
   int leapEvery = 4;        // every this many scouts a chain sends a leap scout (0: never)
@@ -631,14 +632,42 @@ void Engine::run(Job& job) {
       fa.leapJumps = opt.leapJumps;
       fa.pollBackoff = opt.pollBackoff;
       fa.shareIdle = opt.shareIdle;
-      HIPCHK((hipError_t)launch_fill_reset(fa, stream));
-      HIPCHK((hipError_t)launch_clear_u64(dPtCount.p, nb, 0, stream));
-      HIPCHK((hipError_t)launch_clear_u64((unsigned long long*)dPtSum.p, nb, 0, stream));
-      HIPCHK(hipEventRecord(ev0, stream));
-      HIPCHK((hipError_t)launch_fill(fa, nWg, stream));
-      HIPCHK(hipEventRecord(ev1, stream));
-      HIPCHK((hipError_t)launch_resolve(fa, labelCap, dPtCount.p, dPtSum.p, stream));
-      HIPCHK(hipStreamSynchronize(stream));
+      if (opt.useGraph) {
+        // the batch's reset, clears, fill and resolve as one graph (kernel
+        // nodes only: the resets are kernels, not memsets); the events
+        // around its launch time all four
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        int rc = launch_fill_reset(fa, stream);
+        if (!rc) rc = launch_clear_u64(dPtCount.p, nb, 0, stream);
+        if (!rc) rc = launch_clear_u64((unsigned long long*)dPtSum.p, nb, 0, stream);
+        if (!rc) rc = launch_fill(fa, nWg, stream);
+        if (!rc) rc = launch_resolve(fa, labelCap, dPtCount.p, dPtSum.p, stream);
+        const hipError_t ec = hipStreamEndCapture(stream, &graph);
+        if ((rc != 0 || ec != hipSuccess) && graph) hipGraphDestroy(graph);
+        HIPCHK((hipError_t)rc);
+        HIPCHK(ec);
+        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (ei != hipSuccess) hipGraphDestroy(graph);
+        HIPCHK(ei);
+        HIPCHK(hipEventRecord(ev0, stream));
+        const hipError_t el = hipGraphLaunch(exec, stream);
+        if (el == hipSuccess) HIPCHK(hipEventRecord(ev1, stream));
+        const hipError_t es = el == hipSuccess ? hipStreamSynchronize(stream) : el;
+        hipGraphExecDestroy(exec);
+        hipGraphDestroy(graph);
+        HIPCHK(es);
+      } else {
+        HIPCHK((hipError_t)launch_fill_reset(fa, stream));
+        HIPCHK((hipError_t)launch_clear_u64(dPtCount.p, nb, 0, stream));
+        HIPCHK((hipError_t)launch_clear_u64((unsigned long long*)dPtSum.p, nb, 0, stream));
+        HIPCHK(hipEventRecord(ev0, stream));
+        HIPCHK((hipError_t)launch_fill(fa, nWg, stream));
+        HIPCHK(hipEventRecord(ev1, stream));
+        HIPCHK((hipError_t)launch_resolve(fa, labelCap, dPtCount.p, dPtSum.p, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+      }
       stats.fillMs += elapsed();
       stats.fillLaunches += 1;
       std::vector<unsigned long long> sync(7 * kLine);
@@ -1376,6 +1405,7 @@ int tl_set_option(const char* name, double value) {
   else if (n == "scout_retry") o.scoutRetry = value != 0.0 ? 1 : 0;
   else if (n == "leap_every") o.leapEvery = std::max(0, (int)value);
   else if (n == "scout_share") o.scoutShare = value != 0.0 ? 1 : 0;
+  else if (n == "use_graph") o.useGraph = value != 0.0 ? 1 : 0;
   else if (n == "share_rays") o.shareRays = std::max(0.0, value);
   else if (n == "leap_jumps") o.leapJumps = std::min(64, std::max(2, (int)value));
   else if (n == "poll_backoff") o.pollBackoff = std::min(6, std::max(0, (int)value));
