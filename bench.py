@@ -154,6 +154,91 @@ def calibrate_copy(mib):
     return 0
 
 
+# ---------------------------------------------------------------- C3 batch
+def c3_bench(args):
+    """64 seeded light curves (separations 0.3-3, mass ratios 1e-4-1, rho
+    1e-4-1e-2, lengths 256-8192 points, pixel rho/100) by plain IRS in one
+    batched engine run per step (lcirs_batch_irs)."""
+    from triplelens_amd import engine as E, workloads as W
+    if args.tuning:
+        from triplelens_amd import autotune
+        autotune.apply(args.tuning)
+    name, cus = E.device_info()
+    curves = W.c3()
+    trajs = np.array([c.traj() for c in curves])
+
+    def objects():
+        out = []
+        for c in curves:
+            lc = E.LightCurve(*c.lens(), c.rho, c.lc_len, c.ppr)
+            lc.set_limb_darkening("linear", c.u)
+            out.append(lc)
+        return out
+
+    for _ in range(args.warmup):
+        E.batch_irs(objects(), trajs)
+    fill_ms = launches = rays = seed_ms = 0.0
+    flags_all = 0
+    wall = 0.0
+    for _ in range(args.steps):
+        lcs = objects()  # a fresh root chain per curve, as a new run has
+        t = time.perf_counter()
+        res = E.batch_irs(lcs, trajs)
+        wall += time.perf_counter() - t
+        st = E.last_stats()
+        fill_ms += st["fill_ms"]
+        seed_ms += st["seed_ms"]
+        launches += st["fill_launches"]
+        for _a, r, f in res:
+            rays += float(r.sum())
+            flags_all |= int(np.bitwise_or.reduce(f))
+    pts = sum(c.n_points for c in curves)
+    fill_launch_ms = fill_ms / launches
+    achieved = FLOP_PER_RAY * (rays / launches) / (fill_launch_ms * 1e-3) / 1e12
+    line = {
+        "metric": "C3 batched IRS light-curve points/s, 1 GPU",
+        "value": pts * args.steps / wall,
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "C3: 64 seeded triple-lens curves (separations 0.3-3, mass ratios "
+                        "1e-4-1, rho 1e-4-1e-2, u 0-0.8, 256-8192 points), pixel rho/100, one "
+                        "batched engine run per step",
+            "device": name,
+            "cus": cus,
+            "points_per_step": pts,
+            "rays_per_step": rays / args.steps,
+            "rays_per_s": rays / wall,
+            "seed_kernel_ms_per_step": seed_ms / args.steps,
+            "fill_kernel_ms_per_step": fill_ms / args.steps,
+            "fill_launches_per_step": launches / args.steps,
+            "status_flags": flags_all,
+        },
+        "roofline": {
+            "bound": "mfma",
+            "achieved": achieved,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS,
+            "traffic": None,
+            "kernel": "fill_kernel",
+            "kernel_ms": fill_launch_ms,
+            "flop_per_ray": FLOP_PER_RAY,
+        },
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 # ---------------------------------------------------------------- Metropolis
 def metropolis_bench(args, c2):
     """The Metropolis fitting workload at reduced size: args.chains chains of
@@ -222,9 +307,11 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", choices=["surrogate", "plain", "metropolis"], default="surrogate",
+    ap.add_argument("--mode", choices=["surrogate", "plain", "metropolis", "c3"],
+                    default="surrogate",
                     help="surrogate: the C2 headline (surrogate + IRS vs plain IRS); plain: "
-                         "plain IRS alone; metropolis: the Metropolis fitting workload")
+                         "plain IRS alone; metropolis: the Metropolis fitting workload; c3: "
+                         "64 seeded curves of uneven lengths in one batched engine run")
     ap.add_argument("--chains", type=int, default=8, help="metropolis: chains")
     ap.add_argument("--proposals", type=int, default=8, help="metropolis: proposals per chain")
     ap.add_argument("--points", type=int, default=2048, help="metropolis: points per curve")
@@ -250,6 +337,8 @@ def main(argv=None):
         return 2
     if args.mode == "metropolis":
         return metropolis_bench(args, c)
+    if args.mode == "c3":
+        return c3_bench(args)
     cpu = None if args.no_cpu else cpu_baseline(c, args.cpu_cores)
 
     from triplelens_amd import engine as E
