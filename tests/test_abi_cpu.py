@@ -109,3 +109,33 @@ def test_branch_free_complex_routines_match_general_ones():
     lib.tl_math_selftest.argtypes = [ctypes.c_int, ctypes.c_ulonglong]
     for seed in (1, 12345, 2 ** 40 + 7):
         assert lib.tl_math_selftest(300000, seed) == 0
+
+
+def test_new_entry_points_reject_bad_arguments(lib):
+    """The clone, batched seed search and fills of chosen positions check
+    their arguments before any GPU work and report through tl_last_error."""
+    import ctypes
+    from triplelens_amd import engine as E
+    L = E.load()
+    L.lcirs_clone.restype = ctypes.c_void_p
+    assert L.lcirs_clone(None) is None
+    assert b"null" in L.tl_last_error()
+    L.lcirs_batch_find_seeds.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    assert L.lcirs_batch_find_seeds(-1, None, None) == -1
+    assert b"bad batch arguments" in L.tl_last_error()
+    L.lcirs_fill_positions_seeds.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 6
+    assert L.lcirs_fill_positions_seeds(None, 1, None, None, None, None, None, None) == -1
+    assert b"bad arguments" in L.tl_last_error()
+    lc = E.LightCurve(1.0, 0.9, 1.0, 0.1, 0.1, 1e-3, 10, 50)
+    with pytest.raises(E.EngineError, match="no seed search"):
+        E.batch_fill_positions([lc], [np.array([0])])
+    idx = np.array([0], np.int32)
+    with pytest.raises(E.EngineError, match="no seed search"):
+        lc.fill_positions(idx, (np.array([1], np.int32), np.zeros((1, 64, 2), np.int32)))
+    # a clone is a separate object with the original's set-up
+    cl = lc.clone()
+    assert cl.h != lc.h and np.array_equal(cl.grid(), lc.grid())
+    # and a successful call clears the last error
+    assert L.lcirs_clone(None) is None
+    lc.grid()
+    assert E.last_error() == ""
