@@ -397,12 +397,14 @@ def main(argv=None):
             sur.light_curve(lcs, c, rng)
         stats = []
         shot_rays = n_fb = n_aud = 0
+        audit_err = 0.0
         t = time.perf_counter()
         for _ in range(args.steps):
             A_sur, info = sur.light_curve(lcs, c, rng, stats)
             shot_rays += info["rays"]
             n_fb += int(info["fallback"].sum())
             n_aud += int(info["audited"].sum())
+            audit_err = max(audit_err, float(info["audit_rel_err"].max(initial=0.0)))
         wall_sur = time.perf_counter() - t
         value = c.n_points * args.steps / wall_sur
         ms_per_step = 1e3 * wall_sur / args.steps
@@ -422,6 +424,7 @@ def main(argv=None):
             "bound_met": bool(rel.max() <= 1e-3),
             "fallback_fraction": n_fb / (args.steps * c.n_points),
             "audit_fraction": n_aud / (args.steps * c.n_points),
+            "max_audit_rel_err": audit_err,
             "ray_share_shot": shot_rays / rays,
             "surrogate_seed_kernel_ms_per_curve": sum(s["seed_ms"] for s in stats) / args.steps,
             "surrogate_fill_kernel_ms_per_curve": sum(s["fill_ms"] for s in stats) / args.steps,
