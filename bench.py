@@ -14,10 +14,15 @@ Two timed loops over the same curve, each of --steps light curves:
   * surrogate + IRS (the default mode): the split seed search with the
     disk samples, the network on the GPU, and ray shooting of the positions
     the gate sends back plus a 1% audit (Surrogate.light_curve).
-The printed value is the surrogate loop's light-curve points per second;
-every position of its last curve is compared with the plain-IRS curve and
-the largest relative difference is reported (the bound is 1e-3).  With
---mode plain only the plain loop runs and its rate is the value.
+The printed value is the surrogate loop's light-curve points per second.
+Accuracy is measured against the reference itself: tests/golden/c2_full.npz
+holds the reference LightCurveIRS's ray counts and magnifications of this
+curve at all 4096 positions (written by tests/golden/make_golden.py from the
+reference build), and every timed curve of both loops is compared with it:
+the plain loop's ray counts must be equal at every position, and the
+largest relative magnification error of the surrogate loop over all its
+timed curves is the reported error (the bound is 1e-3).  With --mode plain
+only the plain loop runs and its rate is the value.
 
 Roofline figures are those of the fill kernel, where the path's arithmetic
 is (43 FP64 operations per counted ray): its launch durations come from HIP
@@ -26,10 +31,12 @@ its HBM traffic per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
 summary under profiles/.
 
 The CPU baseline is the reference LightCurveIRS compiled from its own
-sources (oracle/_ref, pinned flags -O2 -ffp-contract=off -mavx) run over the
-same curve on the host cores, one contiguous part of the trajectory per
-core; it is timed before the GPU is first used.  Where the reference build
-is absent the C restatement in oracle/ is timed instead.
+sources (oracle/_ref: the parity flags -O2 -ffp-contract=off -mavx, and, as
+a second figure, the reference's own -Ofast -mavx) run over the same curve
+on the host cores, contiguous parts of the trajectory spread over one worker
+process per core (the box's CPU share for one GPU, at most 16); it is timed
+before the GPU is first used.  Where the reference build is absent the C
+restatement in oracle/ is timed instead.
 """
 from __future__ import annotations
 
@@ -46,7 +53,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 FLOP_PER_RAY = 43          # irsSIMD 35 + pixel position 2 + profile and sum 6
-FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 vector/matrix peak
+FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 vector peak (the fill runs on the vector ALUs)
+ROOF_BOUND = "fp64-valu"   # FP64 vector arithmetic, not memory and not the matrix cores
+BOUND = 1e-3               # largest relative magnification error allowed
+REF_C2 = os.path.join(HERE, "tests", "golden", "c2_full.npz")
 METRIC = "IRS light-curve points/s at ≤1e-3 rel err: surrogate+IRS vs plain IRS, 1 GPU"
 MODEL = os.path.join(HERE, "triplelens_amd", "data", "surrogate_c2.npz")
 
@@ -61,6 +71,34 @@ def c2_curve():
         return E.LightCurve(a, b, th, m2, m3, 5e-4, 1, 200).cc_ca()[1]
 
     return W.c2(seed=2, ca_fn=ca_fn)
+
+
+def reference_c2(c):
+    """The reference's ray counts and magnifications of curve c (the
+    committed C2 fixture), or None when the fixture is absent or holds
+    another curve."""
+    try:
+        z = np.load(REF_C2, allow_pickle=False)
+    except OSError:
+        return None
+    p = z["params"][0]
+    want = [*c.lens(), c.rho, c.ppr, c.u, *c.traj(), c.n_points]
+    if not np.array_equal(p, np.array(want, np.float64)):
+        return None
+    return z["rays"].astype(np.uint64), z["amp"]
+
+
+def host_cpu():
+    """CPU model name and logical CPU count of this host."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count() or 1
 
 
 # ---------------------------------------------------------------- CPU baseline
@@ -106,19 +144,30 @@ def measured_traffic():
     return None
 
 
+def _time_ref(lib, c, parts, cores):
+    jobs = [(lib, c.lens(), c.rho, c.ppr, c.u, tr, n) for tr, n in parts]
+    # largest parts first so that the pool stays balanced
+    jobs.sort(key=lambda j: -j[-1])
+    t = time.perf_counter()
+    with mp.get_context("fork").Pool(cores) as pool:
+        res = list(pool.imap_unordered(_ref_worker, jobs, chunksize=1))
+    wall = time.perf_counter() - t
+    return wall, sum(r[2] for r in res) / sum(r[0] for r in res)
+
+
 def cpu_baseline(c, cores):
     from oracle import oracle as orc
     parts = _split(c, 8 * cores)
+    extra = {}
     if os.path.exists(orc.REF_SO):
-        jobs = [(orc.REF_SO, c.lens(), c.rho, c.ppr, c.u, tr, n) for tr, n in parts]
-        # largest parts first so that the pool stays balanced
-        jobs.sort(key=lambda j: -j[-1])
-        t = time.perf_counter()
-        with mp.get_context("fork").Pool(cores) as pool:
-            res = list(pool.imap_unordered(_ref_worker, jobs, chunksize=1))
-        wall = time.perf_counter() - t
+        wall, one_core = _time_ref(orc.REF_SO, c, parts, cores)
         kind = "reference"
-        one_core = sum(r[2] for r in res) / sum(r[0] for r in res)
+        if os.path.exists(orc.REF_FAST_SO):
+            wall_f, one_f = _time_ref(orc.REF_FAST_SO, c, parts, cores)
+            pts = sum(n for _, n in parts)
+            extra = {"value_ofast": float(pts / wall_f), "value_ofast_1core": float(one_f),
+                     "ofast_flags": "-Ofast -mavx (the reference makefile's, without "
+                                    "-march=native)"}
     else:
         O = orc.Oracle()
         params = np.array([[*c.lens(), c.rho, c.ppr, c.u] for _ in parts])
@@ -130,11 +179,16 @@ def cpu_baseline(c, cores):
         kind = "port"
         one_core = None
     pts = sum(n for _, n in parts)
+    model, nproc = host_cpu()
     out = {"value": float(pts / wall), "unit": "points/s", "cores": cores, "kind": kind,
            "sample": f"the full {c.n_points}-point C2 curve split into {len(parts)} contiguous "
-                     f"parts over {cores} worker processes, wall {wall:.1f} s"}
+                     f"parts over {cores} worker processes, wall {wall:.1f} s",
+           "flags": "-O2 -ffp-contract=off -mavx (parity flags)" if kind == "reference"
+                    else "-O2 -ffp-contract=off",
+           "host_cpu": model, "host_nproc": nproc}
     if one_core is not None:
         out["value_1core"] = float(one_core)
+    out.update(extra)
     return out
 
 
@@ -223,7 +277,7 @@ def c3_bench(args):
             "status_flags": flags_all,
         },
         "roofline": {
-            "bound": "mfma",
+            "bound": ROOF_BOUND,
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
@@ -318,7 +372,9 @@ def main(argv=None):
     ap.add_argument("--model", default=MODEL, help="trained surrogate (.npz)")
     ap.add_argument("--tuning", default="", help="engine options saved by triplelens_amd.autotune")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-cores", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-cores", type=int, default=min(16, os.cpu_count() or 1),
+                    help="worker processes of the CPU baseline (default: the host's CPUs, at "
+                         "most 16, the box's CPU share for one GPU)")
     ap.add_argument("--calibrate-copy", type=int, metavar="MIB", default=0,
                     help="only run a plain device copy of MIB MiB three times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
@@ -353,8 +409,22 @@ def main(argv=None):
         lc.lc_irs(*c.traj())
     fill_ms = launches = rays = seed_ms = 0.0
     flags_all = 0
+    curves_plain = []
+
+    def fresh_objects():
+        # one object per timed curve, made before the clock starts: each
+        # timed curve starts the root chain from scratch, as the reference's
+        # first getLCIRS on a new LightCurveIRS does
+        out = []
+        for _ in range(args.steps):
+            o = E.LightCurve(*c.lens(), c.rho, c.lc_len, c.ppr)
+            o.set_limb_darkening("linear", c.u)
+            out.append(o)
+        return out
+
+    objs = fresh_objects()
     t = time.perf_counter()
-    for _ in range(args.steps):
+    for lc in objs:
         A_plain, r, f = lc.lc_irs(*c.traj())
         st = E.last_stats()
         fill_ms += st["fill_ms"]
@@ -362,7 +432,9 @@ def main(argv=None):
         launches += st["fill_launches"]
         rays += float(r.sum())
         flags_all |= int(np.bitwise_or.reduce(f))
+        curves_plain.append((A_plain, r))
     wall_plain = time.perf_counter() - t
+    ref = reference_c2(c)
     plain_pps = c.n_points * args.steps / wall_plain
     fill_launch_ms = fill_ms / launches
     rays_per_launch = rays / launches
@@ -384,6 +456,11 @@ def main(argv=None):
         "plain_fill_kernel_ms_per_curve": fill_ms / args.steps,
         "plain_fill_launches_per_curve": launches / args.steps,
     }
+    if ref is not None:
+        config["plain_ray_mismatches_vs_reference"] = int(max(
+            np.count_nonzero(r != ref[0]) for _, r in curves_plain))
+        config["plain_max_rel_err_vs_reference"] = float(max(
+            np.max(np.abs(A / ref[1] - 1.0)) for A, _ in curves_plain))
     if args.mode == "plain":
         config["mode"] = "plain IRS (no surrogate)"
         value, ms_per_step = plain_pps, 1e3 * wall_plain / args.steps
@@ -396,20 +473,37 @@ def main(argv=None):
         for _ in range(args.warmup):
             sur.light_curve(lcs, c, rng)
         stats = []
-        shot_rays = n_fb = n_aud = 0
+        shot_rays = n_fb = n_aud = n_fail = 0
         audit_err = 0.0
+        curves_sur = []
+        objs = fresh_objects()
+        for o in objs:
+            sur._extra_clones(o, c)  # the seed union's clones, made with the object
         t = time.perf_counter()
-        for _ in range(args.steps):
+        for lcs in objs:
             A_sur, info = sur.light_curve(lcs, c, rng, stats)
             shot_rays += info["rays"]
             n_fb += int(info["fallback"].sum())
             n_aud += int(info["audited"].sum())
+            n_fail += int(info["audit_failed"])
             audit_err = max(audit_err, float(info["audit_rel_err"].max(initial=0.0)))
+            curves_sur.append((A_sur, info["fallback"] | info["audited"]))
         wall_sur = time.perf_counter() - t
         value = c.n_points * args.steps / wall_sur
         ms_per_step = 1e3 * wall_sur / args.steps
-        rel = np.abs(A_sur / A_plain - 1.0)
-        acc = ~info["fallback"]
+        # every timed curve against the plain curve and the reference
+        rel = np.max([np.abs(A / A_plain - 1.0) for A, _ in curves_sur], axis=0)
+        acc = ~curves_sur[-1][1]
+        if ref is not None:
+            rel_ref = np.max([np.abs(A / ref[1] - 1.0) for A, _ in curves_sur], axis=0)
+            config.update({
+                "max_rel_err_vs_reference": float(rel_ref.max()),
+                "max_rel_err_vs_reference_accepted": float(np.max(
+                    [np.max(np.abs(A / ref[1] - 1.0)[~shot], initial=0.0)
+                     for A, shot in curves_sur])),
+                "reference": "reference LightCurveIRS ray counts and magnifications of this "
+                             "curve (tests/golden/c2_full.npz, all 4096 positions)",
+            })
         config.update({
             "mode": "surrogate + IRS fallback (split seed search, disk samples, BF16 network, "
                     "gate, 1% audit)",
@@ -421,10 +515,12 @@ def main(argv=None):
             "speedup_vs_plain_irs": value / plain_pps,
             "max_rel_err_vs_plain_irs": float(rel.max()),
             "max_rel_err_accepted": float(rel[acc].max(initial=0.0)),
-            "bound_met": bool(rel.max() <= 1e-3),
+            "bound_met": bool((rel_ref.max() if ref is not None else rel.max()) <= BOUND),
+            "bound_measured_against": "reference" if ref is not None else "plain engine IRS",
             "fallback_fraction": n_fb / (args.steps * c.n_points),
             "audit_fraction": n_aud / (args.steps * c.n_points),
             "max_audit_rel_err": audit_err,
+            "failed_audits": n_fail,
             "ray_share_shot": shot_rays / rays,
             "surrogate_seed_kernel_ms_per_curve": sum(s["seed_ms"] for s in stats) / args.steps,
             "surrogate_fill_kernel_ms_per_curve": sum(s["fill_ms"] for s in stats) / args.steps,
@@ -444,7 +540,7 @@ def main(argv=None):
         "data": "synthetic",
         "config": config,
         "roofline": {
-            "bound": "mfma",
+            "bound": ROOF_BOUND,
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",

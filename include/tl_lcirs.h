@@ -196,6 +196,30 @@ int tl_last_stats(double* out);
  * makes a new one per proposal).  Returns 0, or -1 with tl_last_error(). */
 int tl_ccc_batch(int n, const double* params, double* cc, double* ca);
 
+/* Values for the slot the reference's chained solve reads one element past
+ * the previous roots (initRoots[10]; laguerre.cc:111-128 via
+ * imgpoint.cc:187).  The reference never writes that element: it holds
+ * whatever its allocator left there, so at the few positions where it
+ * steers a root search the reference's own ray counts depend on its heap
+ * history.  The engine uses 0 there (what the reference reads from zeroed
+ * memory).  With a trace of n values set, getImages call k of this object's
+ * chain (centre and caustic-trial solves in order, counted from the next
+ * light curve on, k < n) starts its chained solve with (xy[2k], xy[2k+1])
+ * instead: a recorded trace makes the engine reproduce a given run of the
+ * reference bit for bit.  n = 0 clears it.  No counterpart in the
+ * reference.  Returns 0, or -1 with tl_last_error(). */
+int lcirs_set_slot_trace(void* lc, const double* xy, long n);
+
+/* The degree-10 image polynomial at n source positions of one lens, as the
+ * reference's getRootsPrecalculated forms it (imgpoint.cc:163-176:
+ * getCoeffsOptNoZ + getCoeffsOptJustZ, imgpointcoeff.cc:223-447): zeta[2g],
+ * zeta[2g+1] in, out[22g + 2k], out[22g + 2k + 1] = coefficient of z^k.
+ * onDevice = 0 evaluates on the host, 1 with the GPU kernel the engine
+ * uses; both give the reference's bits.  No counterpart in the reference
+ * (test access).  Returns 0, or -1 with tl_last_error(). */
+int tl_image_poly(double a, double b, double th, double m2, double m3, int n, const double* zeta,
+                  double* out, int onDevice);
+
 /* Host self-test of the branch-free complex routines the kernels use (modulus,
  * division, square root) against the general ones on n random operand pairs,
  * normal, tiny, huge, zero, infinite and NaN; returns the number of pairs
@@ -224,6 +248,11 @@ int tl_device_info(char* name, int len, int* cus);
 #define TL_FLAG_QUEUE_FULL 16u
 #define TL_FLAG_TIMEOUT 32u
 #define TL_FLAG_ROW_OVERFLOW 64u
+/* Informational: a Laguerre iteration of the point's solves reached step 64,
+ * where the reference reads its 8-entry fractional-step table past the end
+ * (laguerre.cc:22,92), i.e. heap memory: the reference's result at this point
+ * is undefined.  The engine wraps the table index modulo 8. */
+#define TL_FLAG_KICK_TABLE 128u
 
 #ifdef __cplusplus
 }

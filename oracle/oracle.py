@@ -48,12 +48,15 @@ class Oracle:
         lib.tlo_lcirs_new.restype = _P
         lib.tlo_lcirs_delete.argtypes = [_P]
         lib.tlo_set_limb_darkening.argtypes = [_P, ctypes.c_int, _D]
+        lib.tlo_set_slot_trace.argtypes = [_P, _P, ctypes.c_long]
+        lib.tlo_frac_oob.argtypes = [_P, _D, _D, _D, _D, _P]
         lib.tlo_grid.argtypes = [_P, _P]
         lib.tlo_get_lc_irs.argtypes = [_P, _D, _D, _D, _D, _P, _P, _P]
         lib.tlo_seeds.argtypes = [_P, _D, _D, _D, _D, ctypes.c_int, _P, _P, _P]
         lib.tlo_fill_point.argtypes = [_P, _D, _D, ctypes.c_int, _P, _P, _P, _P]
         lib.tlo_ccc.argtypes = [_D] * 5 + [ctypes.c_int, _P, _P]
         lib.tlo_coeffs.argtypes = [_D] * 7 + [_P]
+        lib.tlo_coeffs_exact.argtypes = [_D] * 7 + [_P]
         lib.tlo_laguerre_solve.argtypes = [ctypes.c_int, _P, _P, _P]
         lib.tlo_laguerre_one.argtypes = [ctypes.c_int, _P, _P]
         lib.tlo_images.argtypes = [_D] * 5 + [ctypes.c_int, _P, _P, _P, _P]
@@ -98,6 +101,13 @@ class Oracle:
     def coeffs(self, a, b, th, m2, m3, zeta):
         out = np.zeros(11, np.complex128)
         self.lib.tlo_coeffs(a, b, th, m2, m3, zeta.real, zeta.imag, _ptr(out))
+        return out
+
+    def coeffs_exact(self, a, b, th, m2, m3, zeta):
+        """The same polynomial from an independent derivation, each
+        coefficient the exact value rounded once."""
+        out = np.zeros(11, np.complex128)
+        self.lib.tlo_coeffs_exact(a, b, th, m2, m3, zeta.real, zeta.imag, _ptr(out))
         return out
 
     def laguerre_solve(self, coeffs, init=None):
@@ -192,6 +202,27 @@ class LC:
             raise RuntimeError(f"tlo_get_lc_irs failed ({rc})")
         return amp, rays, edge.astype(bool)
 
+    def frac_oob(self, sx, sy, ex, ey):
+        """Per point: Laguerre kick steps that read the reference's 8-entry
+        fractional-step table past its end (the result there is undefined in
+        the reference)."""
+        out = np.zeros(self.lc_len + 1, np.int32)
+        rc = self._o.lib.tlo_frac_oob(self.h, sx, sy, ex, ey, _ptr(out))
+        if rc:
+            raise RuntimeError(f"tlo_frac_oob failed ({rc})")
+        return out
+
+    def set_slot_trace(self, slots):
+        """Start the chained solve of getImages call k (from the next light
+        curve on) with tempRoots[10] = slots[k] (complex) instead of 0; None
+        switches it off."""
+        if slots is None:
+            self._slots = None
+            self._o.lib.tlo_set_slot_trace(self.h, None, 0)
+            return
+        self._slots = np.ascontiguousarray(slots, np.complex128)
+        self._o.lib.tlo_set_slot_trace(self.h, _ptr(self._slots), len(self._slots))
+
     def seeds(self, sx, sy, ex, ey, max_seeds=64):
         n = self.lc_len + 1
         xy = np.zeros((n, max_seeds, 2))
@@ -270,6 +301,9 @@ class Reference:
         lib.tlref_point_amp.argtypes = [_D] * 5 + [ctypes.c_int, _P, _P]
         lib.tlref_laguerre_solve.argtypes = [ctypes.c_int, _P, _P, _P]
         lib.tlref_coeffs.argtypes = [_D] * 7 + [_P]
+        lib.tlref_lc_irs_slots.argtypes = [_D] * 6 + [ctypes.c_uint, ctypes.c_long, ctypes.c_int,
+                                                      _D, _D, _D, _D, _D, _P, _P, _P, ctypes.c_long]
+        lib.tlref_lc_irs_slots.restype = ctypes.c_long
         lib.tlref_lc_irs_print.argtypes = [_D] * 6 + [ctypes.c_uint, ctypes.c_long, ctypes.c_int,
                                                       _D, _D, _D, _D, _D, ctypes.c_char_p,
                                                       ctypes.c_char_p]
@@ -301,6 +335,23 @@ class Reference:
         if parsed != n:
             raise RuntimeError(f"parsed {parsed} ray counts, expected {n}")
         return amp, rays
+
+    def lc_irs_slots(self, a, b, th, m2, m3, rho, lc_len, ppr, ld_v, sx, sy, ex, ey,
+                     max_slots=1 << 22):
+        """lc_irs, plus the value the chained solve of every getImages call
+        read one element past the previous roots (complex; NaN where there
+        was no such element)."""
+        n = lc_len + 1
+        amp = np.zeros(n)
+        rays = np.zeros(n, np.uint64)
+        slots = np.zeros(2 * max_slots)
+        calls = self.lib.tlref_lc_irs_slots(a, b, th, m2, m3, rho, lc_len, ppr,
+                                            0 if ld_v is None else 1,
+                                            0.0 if ld_v is None else ld_v, sx, sy, ex, ey,
+                                            _ptr(amp), _ptr(rays), _ptr(slots), max_slots)
+        if calls < 0 or calls > max_slots:
+            raise RuntimeError(f"slot recording failed ({calls} calls)")
+        return amp, rays, slots[:2 * calls].view(np.complex128).copy()
 
     def seeds(self, a, b, th, m2, m3, rho, lc_len, ppr, sx, sy, ex, ey, max_seeds=64):
         n = lc_len + 1

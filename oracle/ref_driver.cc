@@ -85,12 +85,58 @@ struct Probe : public LightCurveIRS {
   double rho() const { return _sourceRadius; }
 };
 
+// The slot recorder.  Solving from the previous roots, the reference reads
+// initRoots[10], one element past the ten roots of its previous solve
+// (laguerre.cc:111-128 via imgpoint.cc:187).  The roots vector grows by
+// push_back to a capacity of 16, so that element lies inside the buffer but
+// was never written: it holds whatever the allocator's recycled chunk held
+// before.  The library is linked with --wrap on ImgPoint::getImages(complex),
+// so that the calls getLCIRS makes (lcirs.cc:123,154) pass through
+// __wrap_..., which records that element before the real call.  The
+// recorder allocates nothing, so the heap history it observes is the
+// reference's own.
+//
+// _tempRoots is private; a pointer to it may still be named in an explicit
+// instantiation, which is how MemberPtr reaches it.
+struct TempRootsTag {
+  using type = vector<complex<double>> ImgPoint::*;
+  friend type member_of(TempRootsTag);
+};
+template <typename Tag, typename Tag::type M>
+struct MemberPtr {
+  friend typename Tag::type member_of(Tag) { return M; }
+};
+template struct MemberPtr<TempRootsTag, &ImgPoint::_tempRoots>;
+
+double* g_slot_out = nullptr;
+long g_slot_cap = 0;
+long g_slot_n = 0;
+
 void set_ld(Probe& lc, int model, double v) {
   if (model == 1) lc.setLimbDarkeningModel(LimbDarkeningModel(v));
   else lc.setLimbDarkeningModel(LimbDarkeningModel());
 }
 
 }  // namespace
+
+extern "C" vector<complex<double>> __real__ZN8ImgPoint9getImagesESt7complexIdE(ImgPoint* self,
+                                                                              complex<double> pos);
+extern "C" vector<complex<double>> __wrap__ZN8ImgPoint9getImagesESt7complexIdE(ImgPoint* self,
+                                                                              complex<double> pos) {
+  if (g_slot_out) {
+    const vector<complex<double>>& tr = self->*member_of(TempRootsTag());
+    if (g_slot_n < g_slot_cap) {
+      double* o = g_slot_out + 2 * g_slot_n;
+      if (tr.capacity() > 10) {
+        std::memcpy(o, tr.data() + 10, sizeof(complex<double>));
+      } else {
+        o[0] = o[1] = __builtin_nan("");
+      }
+    }
+    g_slot_n++;
+  }
+  return __real__ZN8ImgPoint9getImagesESt7complexIdE(self, pos);
+}
 
 extern "C" {
 
@@ -138,6 +184,38 @@ int tlref_lc_irs(double a, double b, double th, double m2, double m3, double rho
     for (int i = 0; i < parsed && i <= int(lcLength); ++i) raysOut[i] = sink.counts[i];
   });
   return parsed;
+}
+
+// tlref_lc_irs that also records, for every getImages call of getLCIRS (the
+// centre and caustic-trial solves, in order), the value of the slot the
+// chained solve reads past the end of the previous roots: slots[2k], [2k+1]
+// for call k (NaN while the vector holds no such element).  Returns the
+// number of calls (also when larger than maxSlots, whose excess is not
+// stored), or -1 when the ray counts could not all be parsed.
+long tlref_lc_irs_slots(double a, double b, double th, double m2, double m3, double rho,
+                        unsigned lcLength, long ppr, int ldModel, double v,
+                        double sx, double sy, double ex, double ey,
+                        double* ampOut, uint64_t* raysOut, double* slots, long maxSlots) {
+  int parsed = 0;
+  long calls = 0;
+  run_big_stack([&] {
+    CountParser sink;
+    std::streambuf* old = std::cout.rdbuf(&sink);
+    Probe lc(a, b, th, m2, m3, rho, lcLength, ppr);
+    set_ld(lc, ldModel, v);
+    lc.lcVec.resize(lcLength + 1);
+    g_slot_out = slots;
+    g_slot_cap = maxSlots;
+    g_slot_n = 0;
+    lc.getLCIRS({sx, sy}, {ex, ey});
+    calls = g_slot_n;
+    g_slot_out = nullptr;
+    std::cout.rdbuf(old);
+    for (unsigned i = 0; i <= lcLength; ++i) ampOut[i] = lc.lcVec[i];
+    parsed = int(sink.counts.size());
+    for (int i = 0; i < parsed && i <= int(lcLength); ++i) raysOut[i] = sink.counts[i];
+  });
+  return parsed == int(lcLength) + 1 ? calls : -1;
 }
 
 // getLCIRS with the amoeba and parameter print-out switched on

@@ -17,10 +17,12 @@
  *   Laguerre             src/Laguerre/laguerre.cc:18-198
  *   lens frame           src/LensCore/lens.cc:4-40
  *   critical curves      src/CCC/ccc.cc:42-98 (coefficients), 160-203
- *   image polynomial     own derivation (see lens_poly); the reference's
- *                        generated algebra (src/Images/imgpointcoeff.cc:223-447)
- *                        computes the same polynomial in another order, so
- *                        coefficients agree to rounding, not bit for bit.
+ *   image polynomial     src/Images/imgpoint.cc:160-176 with
+ *                        src/Images/imgpointcoeff.cc:223-447, evaluated in the
+ *                        reference's operation order (tl_imgcoeff.c), so the
+ *                        coefficients are the reference's to the bit; an
+ *                        independent derivation rounded once (lens_poly_exact)
+ *                        serves as a mathematical cross-check.
  *   images               src/Images/imgpoint.cc:72-90,160-274
  *   grid, boxes          src/LightCurves/lcirs.cc:39-92, src/Utils/utils.h:16-28
  *   light-curve loop     src/LightCurves/lcirs.cc:103-191
@@ -223,6 +225,13 @@ static const double lag_frac[8] = {0.5, 0.23, 0.73, 0.13, 0.37, 0.61, 0.89, 1.0}
  * (getImages calls), [1] Laguerre calls, [2] Laguerre iterations. */
 static __thread long g_work[3];
 
+/* Kick steps whose table index i/MT reached 8 (test instrumentation): the
+ * reference reads frac[i/MT] from an 8-entry vector (laguerre.cc:22,92), so
+ * from iteration 64 on it reads past the vector, into heap memory.  The
+ * oracle wraps the index modulo 8 there; a point whose solves took such a
+ * step has a result the reference does not define. */
+static __thread long g_frac_oob;
+
 /* Laguerre::laguerre (laguerre.cc:18-99).  poly has m+1 coefficients. */
 static int lag_iter(const cx* poly, int m, cx* xp) {
   cx x = *xp;
@@ -265,7 +274,10 @@ static int lag_iter(const cx* poly, int m, cx* xp) {
     cx x1 = csub(x, dx);
     if (x.re == x1.re && x.im == x1.im) { *xp = x; return 1; }
     if (i % LAG_MT) x = x1;
-    else x = csub(x, cmul(mk(lag_frac[(i / LAG_MT) & 7], 0.0), dx));
+    else {
+      if (i / LAG_MT >= 8) g_frac_oob++;
+      x = csub(x, cmul(mk(lag_frac[(i / LAG_MT) & 7], 0.0), dx));
+    }
     *xp = x;
   }
   *xp = x;
@@ -420,7 +432,7 @@ static void dpmul(const dcx* p, int np, const dcx* q, int nq, dcx* out) {
  * degree 10 with leading coefficient prod (conj(zeta) - conj(z_k)).  The
  * reference's polynomial (imgpointcoeff.cc:223-447) is this one times -1,
  * which is the sign kept here. */
-static void lens_poly(const lens_t* L, cx zeta, cx* c) {
+static void lens_poly_exact(const lens_t* L, cx zeta, cx* c) {
   const dcx one = {{1.0, 0.0}, {0.0, 0.0}};
   const dcx zero = {{0.0, 0.0}, {0.0, 0.0}};
   const dd m1 = dd_of(L->m1), m2 = dd_of(L->m2), m3 = dd_of(L->m3);
@@ -463,12 +475,35 @@ static void lens_poly(const lens_t* L, cx zeta, cx* c) {
   }
 }
 
+/* The polynomial as ImgPoint::getRootsPrecalculated forms it
+ * (imgpoint.cc:163-176): the position-free part (getCoeffsOptNoZ) plus the
+ * position part (getCoeffsOptJustZ), added coefficient by coefficient.  The
+ * reference caches the position-free part; recomputing it gives the same
+ * bits. */
+static void lens_poly(const lens_t* L, cx zeta, cx* c) {
+  double complex fz[11], pz[11];
+  const double complex z2 = CMPLX(L->z2.re, L->z2.im), z3 = CMPLX(L->z3.re, L->z3.im);
+  tlo_poly_nozeta(z2, z3, L->m2, L->m3, fz);
+  tlo_poly_zeta(z2, z3, L->m2, L->m3, CMPLX(zeta.re, zeta.im), pz);
+  for (int i = 0; i < 11; i++) c[i] = mk(creal(fz[i]) + creal(pz[i]), cimag(fz[i]) + cimag(pz[i]));
+}
+
 int tlo_coeffs(double a, double b, double th, double m2, double m3,
                double zr, double zi, double* out) {
   lens_t L;
   lens_init(&L, a, b, th, m2, m3);
   cx c[11];
   lens_poly(&L, mk(zr, zi), c);
+  for (int i = 0; i < 11; i++) { out[2 * i] = c[i].re; out[2 * i + 1] = c[i].im; }
+  return 0;
+}
+
+int tlo_coeffs_exact(double a, double b, double th, double m2, double m3,
+                     double zr, double zi, double* out) {
+  lens_t L;
+  lens_init(&L, a, b, th, m2, m3);
+  cx c[11];
+  lens_poly_exact(&L, mk(zr, zi), c);
   for (int i = 0; i < 11; i++) { out[2 * i] = c[i].re; out[2 * i + 1] = c[i].im; }
   return 0;
 }
@@ -484,6 +519,13 @@ typedef struct {
   int isImg[MAXDEG];
   cx imgs[MAXDEG];
   int nImg;
+  /* Optional values of the slot read past the end (test instrumentation):
+   * getImages call k of the chain starts its chained solve with
+   * tempRoots[10] = slotTrace[k] instead of 0.  The reference reads
+   * whatever its allocator left in that element of the roots buffer. */
+  const cx* slotTrace;
+  long slotCount;
+  long calls;
 } imgpoint_t;
 
 /* ImgPoint::ImgPoint (imgpoint.cc:57-67): a binary lens when a, b or m3 is 0 */
@@ -510,6 +552,7 @@ static void img_roots(imgpoint_t* ip, cx zeta, int force) {
   } else {
     cx in[MAXDEG + 1], out[MAXDEG];
     memcpy(in, ip->tempRoots, sizeof(in));
+    if (ip->slotTrace && ip->calls - 1 < ip->slotCount) in[10] = ip->slotTrace[ip->calls - 1];
     lag_solve(c, 10, in, out);
     if (!lag_check(c, 10, out)) lag_solve(c, 10, NULL, out);
     memcpy(ip->tempRoots, out, sizeof(out));
@@ -561,6 +604,7 @@ static void img_classify(imgpoint_t* ip, cx zeta) {
 /* ImgPoint::getImages(pos) for a triple lens (imgpoint.cc:208-274,281-286) */
 static int img_get(imgpoint_t* ip, cx zeta) {
   g_work[0]++;
+  ip->calls++;
   img_roots(ip, zeta, 0);
   img_classify(ip, zeta);
   if (!img_count_ok(ip)) {
@@ -816,6 +860,14 @@ tlo_lc* tlo_lcirs_new(double a, double b, double th, double m2, double m3,
   lc->ampScale = lc->intensity / M_PI / pow((double)ppr, 2.0);
   lc->step = lc->L / (double)((double)lc->N - 1.0);
   return lc;
+}
+
+int tlo_set_slot_trace(tlo_lc* lc, const double* xy, long n) {
+  if (!lc || n < 0) return -1;
+  lc->ip.slotTrace = (const cx*)xy;
+  lc->ip.slotCount = xy ? n : 0;
+  lc->ip.calls = 0;
+  return 0;
 }
 
 void tlo_lcirs_delete(tlo_lc* lc) {
@@ -1277,6 +1329,24 @@ int tlo_seeds_margin(tlo_lc* lc, double sx, double sy, double ex, double ey,
       seedXY[2 * ((size_t)i * maxSeeds + k)] = seeds[k].re;
       seedXY[2 * ((size_t)i * maxSeeds + k) + 1] = seeds[k].im;
     }
+  }
+  free(seeds);
+  return 0;
+}
+
+/* Per point of the seed sequence (tlo_seeds' solves): the number of
+ * Laguerre kick steps that read the reference's fractional-step table past
+ * its 8 entries (g_frac_oob). */
+int tlo_frac_oob(tlo_lc* lc, double sx, double sy, double ex, double ey, int* out) {
+  if (!lc || !out) return -1;
+  cx s = mk(sx, sy), e = mk(ex, ey);
+  cx* seeds = NULL;
+  int cap = 0;
+  for (unsigned i = 0; i <= lc->lcLength; i++) {
+    cx pos = traj_pos(s, e, i, lc->lcLength);
+    g_frac_oob = 0;
+    if (point_seeds(lc, pos, &seeds, &cap, NULL) < 0) { free(seeds); return -2; }
+    out[i] = (int)g_frac_oob;
   }
   free(seeds);
   return 0;

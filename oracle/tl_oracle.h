@@ -29,6 +29,22 @@ void tlo_lcirs_delete(tlo_lc* lc);
  * linear with parameter v), anything else = default A=0.6, B=0.4. */
 int tlo_set_limb_darkening(tlo_lc* lc, int model, double v);
 
+/* Test instrumentation.  The reference's chained solve reads initRoots[10],
+ * one element past the ten roots of its previous solve (laguerre.cc:111-128,
+ * imgpoint.cc:187): stale heap content, never written by the reference.  The
+ * oracle uses 0 there.  With a trace set, getImages call k (centre and
+ * caustic-trial solves, in order, counted from the next light curve on)
+ * starts its chained solve with tempRoots[10] = (xy[2k], xy[2k+1]).  The
+ * caller keeps xy alive; NULL switches the trace off. */
+int tlo_set_slot_trace(tlo_lc* lc, const double* xy, long n);
+
+/* Test instrumentation.  The reference's Laguerre iteration takes its
+ * fractional kick from an 8-entry table at frac[i/MT] (laguerre.cc:22,92);
+ * from iteration 64 on that index is past the table and the reference reads
+ * heap memory.  The oracle wraps the index modulo 8.  out[i] = the number of
+ * such kick steps in point i's solves (the seed sequence of tlo_seeds). */
+int tlo_frac_oob(tlo_lc* lc, double sx, double sy, double ex, double ey, int* out);
+
 /* Grid: out[0..7] = N, L, step, BLx, BLy, TRx, TRy, ampScale, then the 6
  * cumulative caustic boxes (minx, miny, maxx, maxy) at out[8..31], then
  * out[32], out[33] = limb-darkening A, B, out[34] = rho^2 (rho*rho),
@@ -73,10 +89,27 @@ int tlo_ccc(double a, double b, double th, double m2, double m3, int len,
             double* cc, double* ca);
 
 /* Degree-10 lens polynomial at source position (zr, zi): 11 complex
- * coefficients, lowest order first, in the sign convention of the reference
- * (leading coefficient -prod(conj(zeta) - conj(z_k))). */
+ * coefficients, lowest order first, as the reference rounds them
+ * (imgpoint.cc:163-176, imgpointcoeff.cc:223-447; leading coefficient
+ * -prod(conj(zeta) - conj(z_k))). */
 int tlo_coeffs(double a, double b, double th, double m2, double m3,
                double zr, double zi, double* out);
+
+/* The same polynomial derived independently (Q-product form, double-double
+ * accumulation) and rounded once per coefficient: the exact values of the
+ * coefficients for the double inputs, a cross-check of tlo_coeffs. */
+int tlo_coeffs_exact(double a, double b, double th, double m2, double m3,
+                     double zr, double zi, double* out);
+
+#ifndef __cplusplus
+#include <complex.h>
+/* ImgPoint::getCoeffsOptNoZ (imgpointcoeff.cc:223-278) and
+ * getCoeffsOptJustZ (imgpointcoeff.cc:280-447), in tl_imgcoeff.c. */
+void tlo_poly_nozeta(double complex z2, double complex z3, double m2, double m3,
+                     double complex* out);
+void tlo_poly_zeta(double complex z2, double complex z3, double m2, double m3,
+                   double complex zeta, double complex* out);
+#endif
 
 /* Laguerre::solveRoots (laguerre.cc:103-167).  init == NULL: fresh solve;
  * else deg+1 values (the last fills the slot the reference reads one past

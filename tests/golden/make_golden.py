@@ -7,12 +7,33 @@ oracle/_ref/libtlref.so, and this script drives that library through
 oracle/ref_driver.cc.  The fixtures are data only (inputs and the
 reference's outputs); the tests read them without the reference.
 
+Two runs of the reference per light curve.  Its chained root solve reads
+one element past the previous solve's roots (initRoots[10],
+src/Laguerre/laguerre.cc:111-128 via src/Images/imgpoint.cc:187), an element
+of the roots buffer that nothing writes: it holds whatever the allocator's
+recycled chunk held before.  So:
+  * the parity fixtures (rays, amp, seeds, ...) come from the reference run
+    with zero-filled allocations (GLIBC_TUNABLES=glibc.malloc.tcache_count=0
+    with MALLOC_PERTURB_=255: every allocation is handed out zeroed), where
+    that element reads 0, the value the oracle and the engine use;
+  * the light curves c1, c2_full and c3_full also hold the plain run
+    (default allocator), stored where it differs from the zero-filled one
+    (plain_idx, rays_plain_d, amp_plain_d), and for each curve with such
+    points the values that run read in that element, one per getImages call,
+    run-length encoded (slot_*): with them the oracle and the engine
+    reproduce the plain run as well.
+This script re-runs itself under the zero-filling allocator for the first
+kind and starts a plain worker process for the second.
+
 Fixtures:
   c1.npz          C1: grid, critical curves, per-point
                   ray counts, magnifications and seeds of all 200 points.
-  c2_ppr20.npz    C2 geometry, all 4096 points, pixel rho/20 (rays and A).
-  c2_window.npz   C2 geometry, 65 points around the central caustic at the
-                  full pixel rho/200.
+  c2_full.npz     the C2 bench curve (geometry seed 2): all 4096 points at
+                  the full pixel rho/200.
+  c2_ppr20.npz    the same curve at pixel rho/20.
+  c2_window.npz   a 65-point curve of its own around that curve's
+                  magnification peak, at the full pixel rho/200.
+  c3_full.npz     all 64 C3 curves at their pixel rho/100.
   c3_ppr20.npz    the first 16 C3 curves at pixel rho/20.
   laguerre.npz    solveRoots on random degree-6 and degree-10 polynomials,
                   fresh and from initial roots.
@@ -32,9 +53,13 @@ Usage: make_golden.py [name ...] writes only the named fixtures.
 """
 from __future__ import annotations
 
+import json
 import math
+import multiprocessing as mp
 import os
+import subprocess
 import sys
+import tempfile
 
 import numpy as np
 
@@ -52,6 +77,124 @@ def pack_seeds(xy, ns):
     off[1:] = np.cumsum(np.minimum(ns, xy.shape[1]))
     flat = np.concatenate([xy[i, :min(n, xy.shape[1])] for i, n in enumerate(ns)])
     return off, flat
+
+
+ZEROFILL_ENV = {"GLIBC_TUNABLES": "glibc.malloc.tcache_count=0", "MALLOC_PERTURB_": "255"}
+ZEROFILL = os.environ.get("TL_GOLDEN_ZEROFILL") == "1"
+WORKERS = min(8, os.cpu_count() or 1)
+
+
+def rle(v):
+    """Run-length encoding of a complex sequence (NaN runs included)."""
+    v = np.asarray(v, np.complex128)
+    if len(v) == 0:
+        return np.zeros(0, np.int64), np.zeros(0, np.complex128)
+    a = v.view(np.float64).reshape(-1, 2)
+    change = np.ones(len(v), bool)
+    change[1:] = np.any(a[1:].view(np.int64) != a[:-1].view(np.int64), axis=1)
+    start = np.nonzero(change)[0]
+    return start.astype(np.int64), v[start]
+
+
+def _curve_row(c, ppr):
+    return [*c.lens(), c.rho, ppr, c.u, *c.traj(), c.n_points]
+
+
+def _plain_one(args):
+    row = args
+    R = orc.Reference()
+    a, b, th, m2, m3, rho, ppr, u, sx, sy, ex, ey, n = row
+    amp, rays, slots = R.lc_irs_slots(a, b, th, m2, m3, rho, int(n) - 1, int(ppr), u,
+                                      sx, sy, ex, ey)
+    return amp, rays, slots
+
+
+def plain_runs(rows):
+    """The plain reference (default allocator) on each curve row, in a
+    worker process started without the zero-filling environment: per curve
+    (amp, rays, slot values read)."""
+    with tempfile.TemporaryDirectory() as d:
+        inp, outp = os.path.join(d, "rows.json"), os.path.join(d, "out.npz")
+        with open(inp, "w") as f:
+            json.dump([list(map(float, r)) for r in rows], f)
+        env = {k: v for k, v in os.environ.items()
+               if k not in ZEROFILL_ENV and k != "TL_GOLDEN_ZEROFILL"}
+        subprocess.run([sys.executable, os.path.abspath(__file__), "--plain-worker", inp, outp],
+                       env=env, check=True)
+        z = np.load(outp)
+        out = []
+        for k in range(len(rows)):
+            out.append((z[f"amp{k}"], z[f"rays{k}"], z[f"slots{k}"]))
+        return out
+
+
+def plain_worker(inp, outp):
+    rows = json.load(open(inp))
+    with mp.Pool(WORKERS) as pool:
+        res = pool.map(_plain_one, rows, chunksize=1)
+    arrs = {}
+    for k, (amp, rays, slots) in enumerate(res):
+        arrs[f"amp{k}"] = amp
+        arrs[f"rays{k}"] = rays
+        arrs[f"slots{k}"] = slots
+    np.savez(outp, **arrs)
+
+
+def _zero_one(row):
+    R = orc.Reference()
+    a, b, th, m2, m3, rho, ppr, u, sx, sy, ex, ey, n = row
+    return R.lc_irs(a, b, th, m2, m3, rho, int(n) - 1, int(ppr), u, sx, sy, ex, ey)
+
+
+def _small_uint(a):
+    a = np.asarray(a)
+    return a.astype(np.uint32) if a.size and int(a.max()) < (1 << 32) else a
+
+
+def plain_fields(rays, amp, plain):
+    """The plain runs (amp, rays, slot values per curve) stored against the
+    zero-filled ones: the points where either differs (plain_idx with their
+    rays_plain_d, amp_plain_d) and, for the curves with such points, the slot
+    values, run-length encoded (slot_curves, slot_off, slot_start, slot_val,
+    slot_calls)."""
+    amp_p = np.concatenate([p[0] for p in plain])
+    rays_p = np.concatenate([p[1] for p in plain])
+    idx = np.nonzero((rays_p != rays) | (amp_p != amp))[0]
+    off = np.zeros(len(plain) + 1, np.int64)
+    off[1:] = np.cumsum([len(p[0]) for p in plain])
+    curves = [k for k in range(len(plain)) if np.any((idx >= off[k]) & (idx < off[k + 1]))]
+    starts, vals, soff, ncalls = [], [], [0], []
+    for k in curves:
+        st, v = rle(plain[k][2])
+        starts.append(st)
+        vals.append(v)
+        soff.append(soff[-1] + len(st))
+        ncalls.append(len(plain[k][2]))
+    return dict(plain_idx=idx.astype(np.int64), rays_plain_d=_small_uint(rays_p[idx]),
+                amp_plain_d=amp_p[idx], slot_curves=np.array(curves, np.int64),
+                slot_off=np.array(soff, np.int64),
+                slot_start=np.concatenate(starts) if starts else np.zeros(0, np.int64),
+                slot_val=np.concatenate(vals) if vals else np.zeros(0, np.complex128),
+                slot_calls=np.array(ncalls, np.int64))
+
+
+def curves_fixture(name, curves, ppr=None, plain=False):
+    """Light curves (workloads.Curve list) at their pixel (or ppr): params,
+    offsets, rays and amp from the zero-filled reference; with plain, the
+    plain run's differences and slot values (plain_fields)."""
+    rows = [_curve_row(c, c.ppr if ppr is None else ppr) for c in curves]
+    with mp.Pool(WORKERS) as pool:
+        res = pool.map(_zero_one, rows, chunksize=1)
+    amps = [r[0] for r in res]
+    raysl = [r[1] for r in res]
+    off = np.zeros(len(rows) + 1, np.int64)
+    off[1:] = np.cumsum([len(a) for a in amps])
+    amp, rays = np.concatenate(amps), np.concatenate(raysl)
+    arrs = dict(params=np.array(rows), off=off, amp=amp, rays=_small_uint(rays))
+    if plain:
+        arrs.update(plain_fields(rays, amp, plain_runs(rows)))
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrs)
+    return int(rays.sum())
 
 
 def curve_fixture(R, c, ppr=None):
@@ -104,10 +247,32 @@ def binary_fixture(R):
     return [int(r.sum()) for r in raysl]
 
 
+def window_fixture(R, c2):
+    """A 65-point curve of its own around the C2 curve's magnification peak,
+    at the full pixel rho/200 (a short full-resolution case)."""
+    amp_full = np.load(os.path.join(HERE, "c2_full.npz"))["amp"]
+    i0 = int(np.argmax(amp_full)) - 32
+    i0 = max(0, min(i0, c2.n_points - 65))
+    t0, t1 = i0 / c2.lc_len, (i0 + 64) / c2.lc_len
+    win = (c2.sx + (c2.ex - c2.sx) * t0, c2.sy + (c2.ey - c2.sy) * t0,
+           c2.sx + (c2.ex - c2.sx) * t1, c2.sy + (c2.ey - c2.sy) * t1)
+    amp, rays = R.lc_irs(*c2.lens(), c2.rho, 64, c2.ppr, c2.u, *win)
+    np.savez_compressed(os.path.join(HERE, "c2_window.npz"), params=np.array(
+        [*c2.lens(), c2.rho, c2.ppr, c2.u, *win, 65]), amp=amp, rays=rays)
+    return int(rays.sum())
+
+
+def c2_curve(R):
+    return W.c2(seed=2, ca_fn=lambda a, b, th, m2, m3: R.ccc(a, b, th, m2, m3)[1])
+
+
 def main(only=()):
-    orc.build(ref=True)
+    if not ZEROFILL:
+        orc.build(ref=True)
+        env = dict(os.environ, TL_GOLDEN_ZEROFILL="1", **ZEROFILL_ENV)
+        subprocess.run([sys.executable, os.path.abspath(__file__), *only], env=env, check=True)
+        return
     R = orc.Reference()
-    O = orc.Oracle()
     rng = np.random.default_rng(12345)
     out = {}
     if only:
@@ -115,6 +280,10 @@ def main(only=()):
             out["amoeba_c1"] = amoeba_fixture(R)
         if "binary" in only:
             out["binary"] = binary_fixture(R)
+        if "c2_full" in only:
+            out["c2_full"] = curves_fixture("c2_full", [c2_curve(R)], plain=True)
+        if "c3_full" in only:
+            out["c3_full"] = curves_fixture("c3_full", W.c3(), plain=True)
         print(out)
         return
 
@@ -126,30 +295,21 @@ def main(only=()):
     cc, ca = R.ccc(*c.lens())
     np.savez_compressed(os.path.join(HERE, "c1.npz"), params=np.array(
         [*c.lens(), c.rho, c.ppr, c.u, *c.traj(), c.n_points]), amp=amp, rays=rays, grid=grid,
-        seed_off=off, seed_xy=flat, n_img=ni, cc=cc, ca=ca)
+        seed_off=off, seed_xy=flat, n_img=ni, cc=cc, ca=ca,
+        **plain_fields(rays, amp, plain_runs([_curve_row(c, c.ppr)])))
     out["c1"] = int(rays.sum())
 
     # C2 ------------------------------------------------------------------
-    def ca_fn(a, b, th, m2, m3):
-        return R.ccc(a, b, th, m2, m3)[1]
-
-    c2 = W.c2(seed=2, ca_fn=ca_fn)
+    c2 = c2_curve(R)
+    out["c2_full"] = curves_fixture("c2_full", [c2], plain=True)
     amp, rays, grid = curve_fixture(R, c2, ppr=20)
     np.savez_compressed(os.path.join(HERE, "c2_ppr20.npz"), params=np.array(
         [*c2.lens(), c2.rho, 20, c2.u, *c2.traj(), c2.n_points]), amp=amp, rays=rays, grid=grid)
     out["c2_ppr20"] = int(rays.sum())
-    # a 65-point window where the magnification peaks, at full resolution
-    i0 = int(np.argmax(amp)) - 32
-    i0 = max(0, min(i0, c2.n_points - 65))
-    t0, t1 = i0 / c2.lc_len, (i0 + 64) / c2.lc_len
-    win = (c2.sx + (c2.ex - c2.sx) * t0, c2.sy + (c2.ey - c2.sy) * t0,
-           c2.sx + (c2.ex - c2.sx) * t1, c2.sy + (c2.ey - c2.sy) * t1)
-    amp, rays = R.lc_irs(*c2.lens(), c2.rho, 64, c2.ppr, c2.u, *win)
-    np.savez_compressed(os.path.join(HERE, "c2_window.npz"), params=np.array(
-        [*c2.lens(), c2.rho, c2.ppr, c2.u, *win, 65]), amp=amp, rays=rays)
-    out["c2_window"] = int(rays.sum())
+    out["c2_window"] = window_fixture(R, c2)
 
     # C3 ------------------------------------------------------------------
+    out["c3_full"] = curves_fixture("c3_full", W.c3(), plain=True)
     rows = []
     amps, raysl = [], []
     for k, cc3 in enumerate(W.c3()[:16]):
@@ -208,4 +368,7 @@ def main(only=()):
 
 
 if __name__ == "__main__":
-    main(tuple(sys.argv[1:]))
+    if len(sys.argv) == 4 and sys.argv[1] == "--plain-worker":
+        plain_worker(sys.argv[2], sys.argv[3])
+    else:
+        main(tuple(sys.argv[1:]))

@@ -139,3 +139,40 @@ def test_new_entry_points_reject_bad_arguments(lib):
     assert L.lcirs_clone(None) is None
     lc.grid()
     assert E.last_error() == ""
+
+
+def test_host_image_polynomial_is_the_references(lib):
+    """tl_image_poly on the host (the header the kernels compile) gives the
+    reference's coefficients (imgpointcoeff.cc:223-447) to the bit."""
+    from triplelens_amd import engine as E
+    rows = golden("coeffs.npz")["rows"]
+    for r in rows:
+        a, b, th, m2, m3, zr, zi = r[:7]
+        ref = r[7:].view(np.complex128)
+        got = E.image_poly(a, b, th, m2, m3, [complex(zr, zi)])[0]
+        assert np.array_equal(got.view(np.float64), ref.view(np.float64))
+
+
+def test_host_image_polynomial_equals_oracle(lib, oracle):
+    from triplelens_amd import engine as E
+    rng = np.random.default_rng(19)
+    for _ in range(500):
+        a, b = np.exp(rng.uniform(np.log(0.3), np.log(3.0), 2))
+        th = rng.uniform(0, 2 * np.pi)
+        m2, m3 = np.exp(rng.uniform(np.log(1e-4), np.log(0.5), 2))
+        z = rng.uniform(-2, 2, 8) + 1j * rng.uniform(-2, 2, 8)
+        got = E.image_poly(a, b, th, m2, m3, z)
+        for k in range(len(z)):
+            want = oracle.coeffs(a, b, th, m2, m3, z[k])
+            assert np.array_equal(got[k].view(np.float64), want.view(np.float64))
+
+
+def test_slot_trace_arguments_are_checked(lib):
+    from triplelens_amd import engine as E, workloads as W
+    c = W.c1()
+    lc = E.LightCurve(*c.lens(), c.rho, c.lc_len, c.ppr)
+    lc.set_slot_trace(np.zeros(4, np.complex128))
+    lc.set_slot_trace(None)
+    lib.lcirs_set_slot_trace.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long]
+    assert lib.lcirs_set_slot_trace(lc.h, None, 5) == -1
+    assert lib.lcirs_set_slot_trace(None, None, 0) == -1

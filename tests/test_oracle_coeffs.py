@@ -1,11 +1,16 @@
 """Degree-10 image polynomial of the oracle.
 
-The oracle derives the polynomial itself and evaluates it in double-double,
-so each coefficient is the exact value for the double inputs, rounded once.
-Pinned by (1) exact rational arithmetic on a few cases, (2) the invariant
+The oracle evaluates the reference's generated expressions in their
+operation order (getCoeffsOptNoZ + getCoeffsOptJustZ, imgpointcoeff.cc:
+223-447, summed as imgpoint.cc:172-176 does), so its coefficients are the
+reference's to the bit.  Pinned by (1) the reference's coefficients (fixture
+coeffs.npz, and live comparisons with the reference build where it is
+present) bit for bit, (2) an independent derivation of the same polynomial
+(Q-product form, double-double, rounded once), which exact rational
+arithmetic pins in turn: the two agree within the reference's rounding
+error, which a dropped or misplaced term would exceed, and (3) the invariant
 P(z) = 0 at lens-equation solutions (images built by mapping random z
-forward), (3) the reference's coefficients (imgpointcoeff.cc:223-447, fixture
-coeffs.npz), which agree to the reference's own rounding error."""
+forward)."""
 import math
 from fractions import Fraction as F
 
@@ -58,13 +63,34 @@ def _exact(a, b, th, m2, m3, zeta):
                      [(right[i] if i < 10 else _C(0)) - left[i] for i in range(11)]])
 
 
-def test_coefficients_are_correctly_rounded(oracle):
+def test_exact_derivation_is_correctly_rounded(oracle):
     cases = [(1.0, 0.9433981132056605, 1.0121970114513341, 0.1, 0.1, 0.3 + 0.1j),
              (0.56, 0.69, 5.03, 0.0277, 0.0017, 0.05 - 0.02j),
              (2.2, 0.41, 2.5, 3e-4, 0.2, -0.7 + 0.4j)]
     for a, b, th, m2, m3, z in cases:
-        got = oracle.coeffs(a, b, th, m2, m3, z)
+        got = oracle.coeffs_exact(a, b, th, m2, m3, z)
         assert np.array_equal(got, _exact(a, b, th, m2, m3, z))
+
+
+def _random_cases(n, seed):
+    rng = np.random.default_rng(seed)
+    for _ in range(n):
+        a, b = np.exp(rng.uniform(np.log(0.3), np.log(3.0), 2))
+        th = rng.uniform(0, 2 * np.pi)
+        m2, m3 = np.exp(rng.uniform(np.log(1e-4), np.log(0.5), 2))
+        if rng.random() < 0.1:
+            m3 = 0.0  # the binary lens keeps the same polynomial with m3 = 0
+        z = complex(rng.uniform(-2, 2), rng.uniform(-2, 2))
+        yield float(a), float(b), float(th), float(m2), float(m3), z
+
+
+def test_reference_rounding_agrees_with_the_exact_polynomial(oracle):
+    """Same polynomial: the reference-order evaluation lies within its
+    rounding error (1e-13 of the largest coefficient) of the exact values."""
+    for a, b, th, m2, m3, z in _random_cases(2000, 7):
+        got = oracle.coeffs(a, b, th, m2, m3, z)
+        ex = oracle.coeffs_exact(a, b, th, m2, m3, z)
+        assert np.max(np.abs(got - ex)) <= 1e-13 * np.max(np.abs(ex))
 
 
 def test_polynomial_vanishes_at_images(oracle):
@@ -77,16 +103,23 @@ def test_polynomial_vanishes_at_images(oracle):
         zk = np.array([0.0, a, b * np.exp(1j * th)])
         z = complex(rng.uniform(-2, 2), rng.uniform(-2, 2))
         zeta = z - np.sum(np.array([m1, m2, m3]) / np.conj(z - zk))
-        c = oracle.coeffs(a, b, th, m2, m3, zeta)
-        val = np.polyval(c[::-1], z)
-        scale = np.sum(np.abs(c) * np.abs(z) ** np.arange(11))
-        assert abs(val) <= 1e-13 * scale
+        for c in (oracle.coeffs(a, b, th, m2, m3, zeta), oracle.coeffs_exact(a, b, th, m2, m3, zeta)):
+            val = np.polyval(c[::-1], z)
+            scale = np.sum(np.abs(c) * np.abs(z) ** np.arange(11))
+            assert abs(val) <= 1e-13 * scale
 
 
-def test_agree_with_reference_coefficients(oracle):
+def test_equal_to_reference_coefficients(oracle):
     rows = golden("coeffs.npz")["rows"]
     for r in rows:
         a, b, th, m2, m3, zr, zi = r[:7]
         ref = r[7:].view(np.complex128)
         got = oracle.coeffs(a, b, th, m2, m3, complex(zr, zi))
-        assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))
+        assert np.array_equal(got.view(np.float64), ref.view(np.float64))
+
+
+def test_equal_to_live_reference(oracle, reference):
+    for a, b, th, m2, m3, z in _random_cases(3000, 11):
+        got = oracle.coeffs(a, b, th, m2, m3, z)
+        ref = reference.coeffs(a, b, th, m2, m3, z)
+        assert np.array_equal(got.view(np.float64), ref.view(np.float64)), (a, b, th, m2, m3, z)

@@ -60,7 +60,7 @@ struct Curve {
   unsigned lcLength;
   long ppr;
   Lens L;
-  PolyBase pb;
+  cx fixedPoly[11];            // ImgPoint::getCoeffsOptNoZ (the reference caches it per lens)
   double rho2, rho2pow, ldA, ldB, intensity, ampScale;
   long long N;
   double Ldbl, step, blx, bly, trx, try_;
@@ -68,6 +68,7 @@ struct Curve {
   bool binImg = false, binIrs = false;  // the binary lens's image-count rule and ray map
   std::vector<cx> cc, ca;      // 6 x kCccLen, root-major
   RootChain chain;             // root-reuse state carried between calls
+  std::vector<cx> slotTrace;   // recorded tempRoots[10] values per getImages call (may be empty)
   std::vector<double> lcVec;   // lcLength values (copy_lc)
   std::vector<double> full;    // lcLength + 1 values of the last IRS curve
   std::vector<uint64_t> rays;
@@ -137,7 +138,7 @@ Curve* make_curve(double a, double b, double th, double m2, double m3, double rh
   c->lcLength = lcLength;
   c->ppr = ppr;
   c->L = make_lens(a, b, th, m2, m3);
-  poly_base(c->L, &c->pb);
+  image_poly_static(c->L, c->fixedPoly);
   c->rho2 = rho * rho;
   c->rho2pow = std::pow(rho, 2.0);
   ld_default(*c);
@@ -177,6 +178,7 @@ Curve* make_curve(double a, double b, double th, double m2, double m3, double rh
   c->step = c->Ldbl / (double)((double)c->N - 1.0);
   for (int k = 0; k < 11; k++) c->chain.tempRoots[k] = mk(0, 0);
   c->chain.have = 0;
+  c->chain.calls = 0;
   c->lcVec.assign(lcLength, 0.0);
   return c;
 }
@@ -271,6 +273,10 @@ struct Engine {
   DBuf<CurveDev> dCurves;
   DBuf<cx> dCaus;
   DBuf<int2> dSeeds;
+  DBuf<int> dPoolBase, dPoolCount;
+  DBuf<cx> dPoolZeta, dPoolCoef, dSlots;
+  DBuf<double> dPtCausD2;
+  DBuf<unsigned long long> dPoolCursor;
   DBuf<int> dNSeeds, dPtCurve, dBatchPts, dLParent, dLPoint;
   DBuf<double2> dPtPos;
   DBuf<float> dCost;
@@ -281,7 +287,7 @@ struct Engine {
   DBuf<uint32_t> dFlags;
   DBuf<int2> dSeedTasks;
   DBuf<unsigned long long> dHtab, dLCount, dQSeq, dSync, dPtCount;
-  DBuf<long long> dLSum, dPtSum;
+  DBuf<unsigned long long> dLSum, dPtSum;  // 128-bit sums: 2 words per label / point
   DBuf<Task> dQBuf;
   DBuf<RootChain> dChain;
   DBuf<double> dPointLC;
@@ -365,12 +371,13 @@ CurveDev to_dev(const Curve& c, cx s, cx e, int pointBase, int causticBase) {
   CurveDev d;
   std::memset(&d, 0, sizeof(d));
   d.L = c.L;
-  d.pb = c.pb;
+  for (int k = 0; k < 11; k++) d.fixedPoly[k] = c.fixedPoly[k];
   d.rho = c.rho;
   d.rho2 = c.rho2;
   d.rho2pow = c.rho2pow;
   d.ldA = c.ldA;
   d.ldB = c.ldB;
+  d.sumScale = sum_scale_for(c.ldA, c.ldB);
   d.blx = c.blx;
   d.bly = c.bly;
   d.step = c.step;
@@ -383,6 +390,9 @@ CurveDev to_dev(const Curve& c, cx s, cx e, int pointBase, int causticBase) {
   d.pointBase = pointBase;
   d.causticBase = causticBase;
   d.chainHave = c.chain.have;
+  d.chainCalls = c.chain.calls;
+  d.slotBase = 0;
+  d.slotCount = 0;
   d.ring = c.ring ? 1 : 0;
   d.split = c.split;
   d.binImg = c.binImg ? 1 : 0;
@@ -416,6 +426,20 @@ void Engine::run(Job& job) {
   for (int k = 0; k < nc; k++) {
     hc[k] = to_dev(*job.curves[k], job.starts[k], job.ends[k], pointBase[k], k * 6 * kCccLen);
     std::copy(job.curves[k]->ca.begin(), job.curves[k]->ca.end(), caus.begin() + (size_t)k * 6 * kCccLen);
+  }
+  // recorded tempRoots[10] values of the curves that carry them
+  std::vector<cx> slots;
+  for (int k = 0; k < nc && job.runSeeds; k++) {
+    const std::vector<cx>& tr = job.curves[k]->slotTrace;
+    if (tr.empty()) continue;
+    hc[k].slotBase = (int)slots.size();
+    hc[k].slotCount = (int)tr.size();
+    slots.insert(slots.end(), tr.begin(), tr.end());
+  }
+  if (!slots.empty()) {
+    dSlots.ensure(slots.size());
+    HIPCHK(hipMemcpyAsync(dSlots.p, slots.data(), slots.size() * sizeof(cx), hipMemcpyHostToDevice,
+                          stream));
   }
   dCurves.ensure(nc);
   dCaus.ensure(caus.size());
@@ -465,14 +489,52 @@ void Engine::run(Job& job) {
       evSeed.push_back(ev);
     }
     dChain.ensure(nc);
+    // the seed stream starts after the set-up copies on the engine stream
+    HIPCHK(hipEventRecord(ev0, stream));
+    HIPCHK(hipStreamWaitEvent(sstream, ev0, 0));
+    HIPCHK(hipEventRecord(evSeed[nChunks], sstream));
+    // every point's solve list and polynomials, before the chains need them
+    TrialArgs ta;
+    ta.curves = dCurves.p;
+    ta.ptCurve = dPtCurve.p;
+    ta.nPts = nPts;
+    ta.caustics = dCaus.p;
+    ta.ptPos = dPtPos.p;
+    dPtCausD2.ensure(nPts);
+    ta.ptCausD2 = dPtCausD2.p;
+    dPoolBase.ensure(nPts);
+    dPoolCount.ensure(nPts);
+    dPoolCursor.ensure(1);
+    size_t cap = std::max<size_t>(dPoolZeta.n, 2 * (size_t)nPts + 1024);
+    for (int attempt = 0;; attempt++) {
+      if (cap > (size_t)INT32_MAX) throw std::runtime_error("solve pool exceeds 2^31 entries");
+      dPoolZeta.ensure(cap);
+      dPoolCoef.ensure(cap * 11);
+      ta.pool.base = dPoolBase.p;
+      ta.pool.count = dPoolCount.p;
+      ta.pool.zeta = dPoolZeta.p;
+      ta.pool.coef = dPoolCoef.p;
+      ta.pool.cursor = dPoolCursor.p;
+      ta.pool.cap = (int)cap;
+      HIPCHK((hipError_t)launch_clear_u64(dPoolCursor.p, 1, 0, sstream));
+      HIPCHK((hipError_t)launch_trials(ta, sstream));
+      unsigned long long used = 0;
+      HIPCHK(hipMemcpyAsync(&used, dPoolCursor.p, sizeof(used), hipMemcpyDeviceToHost, sstream));
+      HIPCHK(hipStreamSynchronize(sstream));
+      if (used <= cap) break;
+      if (attempt >= 2) throw std::runtime_error("solve pool sizing did not converge");
+      cap = (size_t)used + 1024;
+    }
     SeedArgs sa;
     sa.curves = dCurves.p;
     sa.nCurves = nc;
     sa.caustics = dCaus.p;
     sa.trig = dTrig.p;
+    sa.pool = ta.pool;
+    sa.ptCausD2 = dPtCausD2.p;
+    sa.slotTrace = slots.empty() ? nullptr : dSlots.p;
     sa.seeds = dSeeds.p;
     sa.nSeeds = dNSeeds.p;
-    sa.ptPos = dPtPos.p;
     sa.cost = dCost.p;
     sa.ptAmp = dPtAmp.p;
     sa.ptCen = dPtCen.p;
@@ -483,10 +545,6 @@ void Engine::run(Job& job) {
     for (auto* c : job.curves) sa.maxSplit = std::max(sa.maxSplit, c->split);
     sa.flags = dFlags.p;
     sa.chainOut = dChain.p;
-    // the seed stream starts after the set-up copies on the engine stream
-    HIPCHK(hipEventRecord(ev0, stream));
-    HIPCHK(hipStreamWaitEvent(sstream, ev0, 0));
-    HIPCHK(hipEventRecord(evSeed[nChunks], sstream));
     for (int ch = 0; ch < nChunks; ch++) {
       sa.chunk = ch;
       sa.nChunks = nChunks;
@@ -512,7 +570,7 @@ void Engine::run(Job& job) {
   }
 
   std::vector<uint64_t> ptRays(nPts, 0);
-  std::vector<long long> ptSumAll(nPts, 0);
+  std::vector<double> ptSumAll(nPts, 0.0);  // brightness sums in fixed-point units
   // amoeba print-out: the runs of every point of the curves that asked for it
   bool wantAmoebae = false;
   for (auto* c : job.curves) wantAmoebae |= c->amoebaOut && job.runSeeds;
@@ -577,7 +635,7 @@ void Engine::run(Job& job) {
     dBatchPts.ensure(nb);
     dSeedTasks.ensure(seedTasks.size());
     dPtCount.ensure(nb);
-    dPtSum.ensure(nb);
+    dPtSum.ensure(2 * (size_t)nb);
     HIPCHK(hipMemcpyAsync(dBatchPts.p, batchPts.data(), nb * sizeof(int), hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(dSeedTasks.p, seedTasks.data(), seedTasks.size() * sizeof(int2),
                           hipMemcpyHostToDevice, stream));
@@ -589,7 +647,7 @@ void Engine::run(Job& job) {
       dLParent.ensure(labelCap);
       dLPoint.ensure(labelCap);
       dLCount.ensure(labelCap);
-      dLSum.ensure(labelCap);
+      dLSum.ensure(2 * (size_t)labelCap);
       FillArgs fa;
       std::memset(&fa, 0, sizeof(fa));
       fa.curves = dCurves.p;
@@ -641,7 +699,7 @@ void Engine::run(Job& job) {
         HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         int rc = launch_fill_reset(fa, stream);
         if (!rc) rc = launch_clear_u64(dPtCount.p, nb, 0, stream);
-        if (!rc) rc = launch_clear_u64((unsigned long long*)dPtSum.p, nb, 0, stream);
+        if (!rc) rc = launch_clear_u64(dPtSum.p, 2 * (unsigned long long)nb, 0, stream);
         if (!rc) rc = launch_fill(fa, nWg, stream);
         if (!rc) rc = launch_resolve(fa, labelCap, dPtCount.p, dPtSum.p, stream);
         const hipError_t ec = hipStreamEndCapture(stream, &graph);
@@ -661,7 +719,7 @@ void Engine::run(Job& job) {
       } else {
         HIPCHK((hipError_t)launch_fill_reset(fa, stream));
         HIPCHK((hipError_t)launch_clear_u64(dPtCount.p, nb, 0, stream));
-        HIPCHK((hipError_t)launch_clear_u64((unsigned long long*)dPtSum.p, nb, 0, stream));
+        HIPCHK((hipError_t)launch_clear_u64(dPtSum.p, 2 * (unsigned long long)nb, 0, stream));
         HIPCHK(hipEventRecord(ev0, stream));
         HIPCHK((hipError_t)launch_fill(fa, nWg, stream));
         HIPCHK(hipEventRecord(ev1, stream));
@@ -708,12 +766,15 @@ void Engine::run(Job& job) {
         continue;
       }
       std::vector<unsigned long long> cnt(nb);
-      std::vector<long long> sum(nb);
+      std::vector<unsigned long long> sum(2 * (size_t)nb);
       HIPCHK(hipMemcpy(cnt.data(), dPtCount.p, nb * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-      HIPCHK(hipMemcpy(sum.data(), dPtSum.p, nb * sizeof(long long), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(sum.data(), dPtSum.p, sum.size() * sizeof(unsigned long long),
+                       hipMemcpyDeviceToHost));
       for (int b = 0; b < nb; b++) {
         ptRays[batchPts[b]] = cnt[b];
-        ptSumAll[batchPts[b]] = sum[b];
+        // the signed 128-bit total: high word * 2^64 + unsigned low word
+        ptSumAll[batchPts[b]] = (double)(long long)sum[2 * (size_t)b + 1] * 18446744073709551616.0 +
+                                (double)sum[2 * (size_t)b];
       }
       if (wantAmoebae) collect_amoebae(job, ptCurve, batchPts, hcap, labelCap, amoebae);
       break;
@@ -771,7 +832,7 @@ void Engine::run(Job& job) {
     for (int i = 0; i < n; i++) {
       const int p = pointBase[k] + i;
       c.rays[i] = ptRays[p];
-      c.full[i] = ((double)ptSumAll[p] / kSumScale) * c.ampScale;
+      c.full[i] = (ptSumAll[p] / hc[k].sumScale) * c.ampScale;
       c.flags[i] = flags[p];
     }
     if (job.runSeeds) {
@@ -1337,6 +1398,65 @@ int tl_ccc_batch(int n, const double* params, double* cc, double* ca) {
     HIPCHK(hipStreamSynchronize(E.stream));
     HIPCHK(hipMemcpy(cc, E.dCcOut.p, (size_t)n * 6 * kCccLen * sizeof(cx), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(ca, E.dCaOut.p, (size_t)n * 6 * kCccLen * sizeof(cx), hipMemcpyDeviceToHost));
+    return 0;
+  });
+}
+
+int lcirs_set_slot_trace(void* lc, const double* xy, long n) {
+  return guarded([&] {
+    if (!lc) throw std::invalid_argument("null light-curve object");
+    if (n < 0 || (n > 0 && !xy)) throw std::invalid_argument("bad slot trace");
+    if (n > INT32_MAX) throw std::invalid_argument("slot trace too long");
+    Curve* c = static_cast<Curve*>(lc);
+    c->slotTrace.resize((size_t)n);
+    for (long k = 0; k < n; k++) c->slotTrace[k] = mk(xy[2 * k], xy[2 * k + 1]);
+    c->chain.calls = 0;
+    return 0;
+  });
+}
+
+int tl_image_poly(double a, double b, double th, double m2, double m3, int n, const double* zeta,
+                  double* out, int onDevice) {
+  return guarded([&] {
+    if (n < 0 || (n > 0 && (!zeta || !out))) throw std::invalid_argument("bad arguments");
+    const Lens L = make_lens(a, b, th, m2, m3);
+    cx fixed[11];
+    image_poly_static(L, fixed);
+    if (!onDevice) {
+      for (int g = 0; g < n; g++) {
+        cx c[11];
+        image_poly(L, fixed, mk(zeta[2 * g], zeta[2 * g + 1]), c);
+        for (int k = 0; k < 11; k++) {
+          out[22 * (size_t)g + 2 * k] = c[k].re;
+          out[22 * (size_t)g + 2 * k + 1] = c[k].im;
+        }
+      }
+      return 0;
+    }
+    Engine& E = engine();
+    std::lock_guard<std::mutex> lock(E.mu);
+    E.init();
+    if (n == 0) return 0;
+    cx *dz = nullptr, *dc = nullptr, *df = nullptr;
+    auto release = [&] {
+      if (dz) hipFree(dz);
+      if (dc) hipFree(dc);
+      if (df) hipFree(df);
+    };
+    try {
+      HIPCHK(hipMalloc(&dz, (size_t)n * sizeof(cx)));
+      HIPCHK(hipMalloc(&dc, (size_t)n * 11 * sizeof(cx)));
+      HIPCHK(hipMalloc(&df, 11 * sizeof(cx)));
+      HIPCHK(hipMemcpy(dz, zeta, (size_t)n * sizeof(cx), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(df, fixed, 11 * sizeof(cx), hipMemcpyHostToDevice));
+      HIPCHK((hipError_t)launch_image_poly(L, df, dz, n, dc, E.stream));
+      HIPCHK(hipStreamSynchronize(E.stream));
+      HIPCHK(hipMemcpy(out, dc, (size_t)n * 11 * sizeof(cx), hipMemcpyDeviceToHost));
+    } catch (...) {
+      release();
+      throw;
+    }
+    release();
     return 0;
   });
 }

@@ -4,6 +4,7 @@
 
 #include <stdint.h>
 #include "tl_math.h"
+#include "tl_coeffs.h"
 
 namespace tl {
 
@@ -25,14 +26,19 @@ enum : uint32_t {
   kFlagQueueFull = 16u,    // task queue full
   kFlagTimeout = 32u,      // the fill launch hit its time limit
   kFlagRowOverflow = 64u,  // more than kMaxCover runs in one row (work repeated, result exact)
+  kFlagKickTable = 128u,   // a Laguerre iteration of the point's solves reached step 64, where
+                           // the reference reads its 8-entry fractional-step table past the
+                           // end (laguerre.cc:22,92): heap memory, so the reference's result
+                           // there is undefined; the engine wraps the index modulo 8
 };
 
 // One light curve (a LightCurveIRS object) as the kernels see it.
 struct CurveDev {
   Lens L;
-  PolyBase pb;
+  cx fixedPoly[11];            // ImgPoint::getCoeffsOptNoZ of the lens (image_poly_static)
   double rho, rho2, rho2pow;   // rho, rho*rho (ray test), pow(rho, 2.0) (intersection test)
   double ldA, ldB;             // limb-darkening profile A + B sqrt(1 - r^2)
+  double sumScale;             // fixed-point scale of the brightness sum (sum_scale_for)
   double blx, bly, step, Ldbl, Nd;
   long long N;
   double sx, sy, ex, ey;       // trajectory
@@ -45,6 +51,10 @@ struct CurveDev {
   int split;                   // independent pieces of the seed search (1: one chain)
   int binImg;                  // binary image-count rule (a, b or m3 zero)
   int binIrs;                  // binary ray map (m3 = 0)
+  int chainCalls;              // getImages calls made by the chain before this run
+  int slotBase, slotCount;     // recorded values of tempRoots[10] (SeedArgs::slotTrace):
+                               // getImages call k of the chain (k < slotCount) starts its
+                               // chained solve with slotTrace[slotBase + k]; slotCount 0: none
   cx chain[11];
   cx boxMin[6], boxMax[6];
 };
@@ -59,14 +69,39 @@ struct alignas(16) Task {
   int b, row, xa, xb, label, kd, f1, f2;  // kd = kind | (dir < 0 ? 4 : 0)
 };
 
+// The source positions each light-curve point solves for, in the order the
+// reference solves them (lcirs.cc:123-165): its centre, then the caustic
+// trial points.  Written by trial_kernel for every point at once (they do
+// not depend on the root chain), with the image polynomial of every solve.
+struct SolvePool {
+  int* base;          // [point] first solve in the pool
+  int* count;         // [point] solves (1 + caustic trials)
+  cx* zeta;           // [solve] source position
+  cx* coef;           // [solve * 11 + k] coefficient of z^k (image_poly)
+  unsigned long long* cursor;  // solves handed out (may exceed cap: then rerun bigger)
+  int cap;            // pool capacity in solves
+};
+
+struct TrialArgs {
+  const CurveDev* curves;
+  const int* ptCurve;  // global point -> curve
+  int nPts;
+  const cx* caustics;
+  double2* ptPos;      // source position of every point
+  double* ptCausD2;    // squared distance to the nearest caustic sample
+  SolvePool pool;
+};
+
 struct SeedArgs {
   const CurveDev* curves;
   int nCurves;
   const cx* caustics;
   const TrigTable* trig;
+  SolvePool pool;      // filled by trial_kernel
+  const double* ptCausD2;
+  const cx* slotTrace; // recorded tempRoots[10] values (CurveDev::slotBase), may be null
   int2* seeds;        // [point][kMaxSeeds] pixel (nx, ny)
   int* nSeeds;
-  double2* ptPos;     // source position of every point
   float* cost;        // predicted rays (point-source magnification x pi ppr^2)
   double* ptAmp;      // point-source magnification at every position
   double2* ptCen;     // magnification-weighted centroid of the point images
@@ -99,7 +134,7 @@ struct FillArgs {
   int* lparent;
   int* lpoint;
   unsigned long long* lcount;
-  long long* lsum;
+  unsigned long long* lsum;   // 2 words per label: 128-bit brightness sum (low, high)
   int* labelNext;
   int labelCap;
   // shared queue (tasks handed between waves) and termination
@@ -132,7 +167,12 @@ struct FillArgs {
 };
 
 // Host-side launchers (tl_kernels.hip).  All return hipError_t as int.
+int launch_trials(const TrialArgs& a, hipStream_t s);
 int launch_seed(const SeedArgs& a, hipStream_t s);
+// image_poly for n source positions of one lens on the GPU (test access):
+// zeta[n] in, coef[11 n] out (device pointers).
+int launch_image_poly(const Lens& L, const cx* fixedPoly, const cx* zeta, int n, cx* coef,
+                      hipStream_t s);
 // Point magnification at the kRingSamples disk samples around each position
 // of the curves with `ring` set: ring[kRingSamples p + s], and cross[p], the
 // number of samples whose count of significant images (at least 1e-4 of the
@@ -148,16 +188,28 @@ int fill_resident_workgroups();
 int launch_clear_flag_bits(uint32_t* flags, const int* pts, int n, uint32_t bits, hipStream_t s);
 int launch_clear_u64(unsigned long long* p, unsigned long long n, unsigned long long v, hipStream_t s);
 int launch_fill_reset(const FillArgs& a, hipStream_t s);
-int launch_resolve(const FillArgs& a, int nLabels, unsigned long long* ptCount, long long* ptSum,
-                   hipStream_t s);
+// ptSum: 2 words per batch point, the 128-bit brightness sum (low, high).
+int launch_resolve(const FillArgs& a, int nLabels, unsigned long long* ptCount,
+                   unsigned long long* ptSum, hipStream_t s);
 // Point-source light curves (LightCurveBase::getLC): curve k writes
 // lcLength values at out[curves[k].pointBase ...].
 int launch_point_lc(const CurveDev* curves, int nCurves, const TrigTable* trig, double* out,
                     RootChain* chainOut, hipStream_t s);
 
-// Fixed-point scale of the brightness sum: each pixel's A + B sqrt(1 - r^2)
-// is rounded to a multiple of 2^-34 and summed as an integer, which makes
-// the sum independent of the order in which pixels are visited.
+// Fixed-point brightness sum: each pixel's A + B sqrt(1 - r^2) is rounded to
+// a multiple of 1/scale and summed as an integer, which makes the sum
+// independent of the order in which pixels are visited.  The scale is 2^34
+// (rounding below 3e-11 of a pixel's value) for profiles whose brightness
+// stays within 2 (every linear profile with 0 <= u <= 1, and the default),
+// halved per doubling beyond, so that one pixel never exceeds 2^35 units and
+// one run (at most 2^25 pixels) 2^60.  Wave-local totals are flushed before
+// they pass 2^61; label and point totals are 128-bit.
 constexpr double kSumScale = 17179869184.0;  // 2^34
+inline double sum_scale_for(double ldA, double ldB) {
+  const double peak = fabs(ldA) + fabs(ldB);
+  double s = kSumScale;
+  while (peak * s > 2.0 * kSumScale && s > 1.0) s *= 0.5;
+  return s;
+}
 
 }  // namespace tl

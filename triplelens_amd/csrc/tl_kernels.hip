@@ -1,5 +1,8 @@
 // HIP kernels of the IRS engine (gfx950).
 //
+//  trial_kernel   per light-curve point, all at once: source position, caustic
+//                 trial points (src/LightCurves/lcirs.cc:131-165) and the
+//                 image polynomial of every solve (tl_coeffs.h).
 //  seed_kernel    per light curve, the reference's seed sequence: point images
 //                 with the root-reuse chain, then the images of caustic trial
 //                 points (src/LightCurves/lcirs.cc:116-165), as pixel seeds.
@@ -120,97 +123,20 @@ __device__ __forceinline__ double rl(double v, int k) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 __device__ __forceinline__ cx rlc(const cx& v, int k) { return mk(rl(v.re, k), rl(v.im, k)); }
-__device__ __forceinline__ dd rld(const dd& v, int k) {
-  dd r;
-  r.hi = rl(v.hi, k);
-  r.lo = rl(v.lo, k);
-  return r;
-}
-__device__ __forceinline__ dcx rldc(const dcx& v, int k) {
-  dcx r;
-  r.re = rld(v.re, k);
-  r.im = rld(v.im, k);
-  return r;
-}
-
-// Lane l receives lane l-N's value within its row of 16 lanes (DPP row_shr).
-template <int N>
-__device__ __forceinline__ double shr(double v) {
-  if constexpr (N == 0) {
-    return v;
-  } else {
-    const long long b = __double_as_longlong(v);
-    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)b, 0x110 + N, 0xF, 0xF, true);
-    const unsigned hi =
-        (unsigned)__builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x110 + N, 0xF, 0xF, true);
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-  }
-}
-template <int N>
-__device__ __forceinline__ dcx shr_dc(const dcx& v) {
-  dcx r;
-  r.re.hi = shr<N>(v.re.hi);
-  r.re.lo = shr<N>(v.re.lo);
-  r.im.hi = shr<N>(v.im.hi);
-  r.im.lo = shr<N>(v.im.lo);
-  return r;
-}
-
-// Coefficient k (this lane's) of the product p q as dpmul forms it: the terms
-// p[i] q[k-i] with 0 <= k-i < NQ added in increasing i.  p[i] is lane i of pv
-// and q[j] lane j of qv.
-template <int NP, int NQ, int I = 0>
-__device__ __forceinline__ void dp_lane(const dcx& pv, const dcx& qv, int k, dcx& out) {
-  if constexpr (I < NP) {
-    const dcx p = rldc(pv, I);
-    const dcx q = shr_dc<I>(qv);
-    if (k - I >= 0 && k - I < NQ) out = dc_add(out, dc_mul(p, q));
-    dp_lane<NP, NQ, I + 1>(pv, qv, k, out);
-  }
-}
-
-// lens_poly (tl_math.h) with the coefficients of every stage computed on
-// separate lanes: lane k returns c_k (k <= 10; 0 on the other lanes).  Dv and
-// Sv hold the zeta-free factors D[k] (k < 4) and S[k] (k < 3) on lane k.
-__device__ __forceinline__ cx wl_poly(const Lens& L, const dcx& Dv, const dcx& Sv, cx zeta) {
-  const int k = __lane_id();
-  const dd m1 = dd_of(L.m1), m2 = dd_of(L.m2), m3 = dd_of(L.m3);
-  const dcx zc = dc_of(cconj(zeta));
-  const dcx w1 = dc_sub(zc, dc_of(cconj(L.z1))), w2 = dc_sub(zc, dc_of(cconj(L.z2))),
-            w3 = dc_sub(zc, dc_of(cconj(L.z3)));
-  dcx Q1 = dc_zero(), Q2 = dc_zero(), Q3 = dc_zero();
-  if (k < 4) {
-    const dcx sk = k < 3 ? Sv : dc_zero();
-    Q1 = dc_add(dc_mul(w1, Dv), sk);
-    Q2 = dc_add(dc_mul(w2, Dv), sk);
-    Q3 = dc_add(dc_mul(w3, Dv), sk);
-  }
-  dcx q12 = dc_zero(), q13 = dc_zero(), q23 = dc_zero();
-  dp_lane<4, 4>(Q1, Q2, k, q12);
-  dp_lane<4, 4>(Q1, Q3, k, q13);
-  dp_lane<4, 4>(Q2, Q3, k, q23);
-  if (k >= 7) {
-    q12 = dc_zero();
-    q13 = dc_zero();
-    q23 = dc_zero();
-  }
-  dcx q123 = dc_zero();
-  dp_lane<7, 4>(q12, Q3, k, q123);
-  if (k >= 10) q123 = dc_zero();
-  dcx mix = dc_zero();
-  if (k < 7) mix = dc_add(dc_add(dc_scale(q23, m1), dc_scale(q13, m2)), dc_scale(q12, m3));
-  // left = (z - zeta) q123: lin[0] = -zeta, lin[1] = 1
-  dcx left = dc_zero();
-  {
-    dcx lin = dc_zero();
-    if (k == 0) lin = dc_of(cneg(zeta));
-    if (k == 1) lin = dc_of(mk(1.0, 0.0));
-    dp_lane<2, 10>(lin, q123, k, left);
-  }
-  dcx right = dc_zero();
-  dp_lane<4, 7>(Dv, mix, k, right);
-  if (k >= 10) right = dc_zero();
-  return k <= 10 ? dc_round(dc_sub(right, left)) : mk(0.0, 0.0);
+// The image polynomial at zeta with the reference's rounding (tl_coeffs.h),
+// computed by every lane; lane k returns the coefficient of z^k (k <= 10; 0
+// on the other lanes).  Used where one position is solved at a time (the
+// point-source curve, the disk samples); the light curve's solves take their
+// coefficients from trial_kernel's pool instead.
+__device__ __forceinline__ cx wl_coeffs(const Lens& L, const cx* fixedPoly, cx zeta) {
+  cx c[11];
+  image_poly_zeta(L, zeta, c);
+  const int ln = __lane_id();
+  cx mine = mk(0.0, 0.0);
+#pragma unroll
+  for (int k = 0; k < 11; k++)
+    if (ln == k) mine = cadd(fixedPoly[k], c[k]);
+  return mine;
 }
 
 // One part of the quotient n / d as cdiv_sel forms it (part 0: real, 1:
@@ -236,7 +162,8 @@ __device__ __forceinline__ double smith_part(cx n, cx d, int part) {
 // |p_m| and |x|, the four parts of f/b and 2s/b, the moduli of the two
 // candidate denominators and the two parts of m/g.  Every value is the one
 // the serial code computes.
-__device__ __forceinline__ bool wl_lag(const cx& pv, int m, cx* xp, const TrigTable* trig) {
+__device__ __forceinline__ bool wl_lag(const cx& pv, int m, cx* xp, const TrigTable* trig,
+                                       bool& oob) {
   const int ln = __lane_id();
   cx x = *xp;
   const double md = (double)m;
@@ -292,8 +219,12 @@ __device__ __forceinline__ bool wl_lag(const cx& pv, int m, cx* xp, const TrigTa
     }
     cx x1 = csub(x, dx);
     if (x.re == x1.re && x.im == x1.im) { *xp = x; return true; }
-    if (i % kLagMT) x = x1;
-    else x = csub(x, cmul(mk(kLagFrac[(i / kLagMT) & 7], 0.0), dx));
+    if (i % kLagMT) {
+      x = x1;
+    } else {
+      oob |= i / kLagMT >= 8;
+      x = csub(x, cmul(mk(kLagFrac[(i / kLagMT) & 7], 0.0), dx));
+    }
   }
   *xp = x;
   return false;
@@ -302,7 +233,7 @@ __device__ __forceinline__ bool wl_lag(const cx& pv, int m, cx* xp, const TrigTa
 // Laguerre::laguerre with a different x on every lane (the polishing step):
 // the coefficients are the lanes of pv, read in the same order by all lanes.
 __device__ __forceinline__ bool wl_lag_lanes(const cx& pv, int m, cx* xp,
-                                             const TrigTable* trig) {
+                                             const TrigTable* trig, bool& oob) {
   cx x = *xp;
   const double md = (double)m;
   for (int i = 1; i <= kLagMaxIt; i++) {
@@ -330,8 +261,12 @@ __device__ __forceinline__ bool wl_lag_lanes(const cx& pv, int m, cx* xp,
     else dx = cscale(mk(trig->c[i], trig->s[i]), 1.0 + cabs_sel(x));
     cx x1 = csub(x, dx);
     if (x.re == x1.re && x.im == x1.im) { *xp = x; return true; }
-    if (i % kLagMT) x = x1;
-    else x = csub(x, cmul(mk(kLagFrac[(i / kLagMT) & 7], 0.0), dx));
+    if (i % kLagMT) {
+      x = x1;
+    } else {
+      oob |= i / kLagMT >= 8;
+      x = csub(x, cmul(mk(kLagFrac[(i / kLagMT) & 7], 0.0), dx));
+    }
   }
   *xp = x;
   return false;
@@ -343,13 +278,13 @@ __device__ __forceinline__ bool wl_lag_lanes(const cx& pv, int m, cx* xp,
 // the full polynomial.  Returns root r on lane r (r < 10; 0 elsewhere).
 // With init, initv's lane j seeds the search of step j (j = 10..1).
 __device__ __forceinline__ cx wl_solve10(const cx& cv, bool init, const cx& initv,
-                                         const TrigTable* trig) {
+                                         const TrigTable* trig, bool& oob) {
   const int ln = __lane_id();
   cx tmp = cv;
   cx rootv = mk(0.0, 0.0);
   for (int j = 10; j >= 1; j--) {
     cx x = init ? rlc(initv, j) : mk(kLagEps, kLagEps);
-    if (!wl_lag(tmp, j, &x, trig) && init) x = mk(kLagEps, kLagEps);
+    if (!wl_lag(tmp, j, &x, trig, oob) && init) x = mk(kLagEps, kLagEps);
     if (ln == 10 - j) rootv = x;
     cx b = rlc(tmp, j);
     for (int jj = j - 1; jj >= 0; jj--) {
@@ -358,7 +293,9 @@ __device__ __forceinline__ cx wl_solve10(const cx& cv, bool init, const cx& init
       b = cadd(cmul(x, b), c);
     }
   }
-  if (ln < 10) wl_lag_lanes(cv, 10, &rootv, trig);
+  bool laneOob = false;
+  if (ln < 10) wl_lag_lanes(cv, 10, &rootv, trig, laneOob);
+  oob |= __ballot(laneOob) != 0;
   return rootv;
 }
 
@@ -371,15 +308,16 @@ __device__ __forceinline__ bool wl_check(const cx& cv, const cx& rootv) {
   return cabs_sel(csub(c0, a0)) < 1.0e-10;
 }
 
-// getRootsPrecalculated + getImages (imgpoint.cc:160-274) on a wavefront.
-// chainv holds tempRoots[k] on lane k (lane 10 is the slot read past the end
-// of the reference's vector and stays 0).  Returns the mask of the roots
-// accepted as images (bit k: root k on lane k of chainv).
-__device__ __forceinline__ unsigned long long wl_get_images(const Lens& L, const dcx& Dv,
-                                                            const dcx& Sv, cx& chainv, int& have,
-                                                            cx zeta, const TrigTable* trig,
-                                                            bool binImg) {
-  const cx cv = wl_poly(L, Dv, Sv, zeta);
+// getRootsPrecalculated + getImages (imgpoint.cc:160-274) on a wavefront,
+// for the polynomial whose coefficient k is lane k of cv.  chainv holds
+// tempRoots[k] on lane k (lane 10 is the slot the reference reads past the
+// end of its vector: 0 here, or a recorded value the caller put there).
+// Returns the mask of the roots accepted as images (bit k: root k on lane k
+// of chainv).
+__device__ __forceinline__ unsigned long long wl_get_images(const Lens& L, const cx& cv,
+                                                            cx& chainv, int& have, cx zeta,
+                                                            const TrigTable* trig, bool binImg,
+                                                            bool& oob) {
   const int ln = __lane_id();
   // The reference's sequence (imgpoint.cc:181-206,262-270): a solve from the
   // chain's roots (when it has some), replaced by a fresh solve when the
@@ -389,7 +327,7 @@ __device__ __forceinline__ unsigned long long wl_get_images(const Lens& L, const
   bool withInit = have != 0;
   bool second = false;
   while (true) {
-    const cx r = wl_solve10(cv, withInit, chainv, trig);
+    const cx r = wl_solve10(cv, withInit, chainv, trig, oob);
     if (withInit && !wl_check(cv, r)) {
       withInit = false;
       continue;
@@ -429,11 +367,130 @@ __device__ __forceinline__ void emit_seeds(const CurveDev& C, const cx& chainv,
   cnt = cnt + n > kMaxSeeds ? kMaxSeeds : cnt + n;
 }
 
+// The caustic trial points of one source position in the reference's order
+// (lcirs.cc:131-165): for each caustic branch r whose cumulative box meets
+// the source square, the segments k = 0..kCccLen-2 are tested 64 at a time;
+// the first hit of every run of consecutive hits is a trial point (the
+// reference solves only those; `pointTaken` restarts at each branch).  For
+// each step that has trial points, onStep(f, trial) receives their mask f
+// (bit j: the segment on lane j, in segment order) and every lane's trial
+// position.
+template <typename OnStep>
+__device__ __forceinline__ void scan_trials(const CurveDev& C, cx pos, const cx* caus,
+                                            OnStep&& onStep) {
+  const int ln = __lane_id();
+  for (int r = 0; r < 6; r++) {
+    if (!(pos.re + C.rho >= C.boxMin[r].re && pos.re - C.rho <= C.boxMax[r].re &&
+          pos.im + C.rho >= C.boxMin[r].im && pos.im - C.rho <= C.boxMax[r].im))
+      continue;
+    const cx* cr = caus + r * kCccLen;
+    bool prevHit = false;
+    for (int k0 = 0; k0 < kCccLen - 1; k0 += 64) {
+      const int k = k0 + ln;
+      cx trial = mk(0.0, 0.0);
+      const bool hit = k < kCccLen - 1 &&
+                       intersection_check(pos, cr[k], cr[k + 1], C.rho, C.rho2pow, &trial);
+      const unsigned long long hm = __ballot(hit);
+      const unsigned long long f = hm & ~((hm << 1) | (prevHit ? 1ull : 0ull));
+      prevHit = (hm >> 63) & 1ull;
+      if (f) onStep(f, trial);
+    }
+  }
+}
+
+// Everything of a light-curve point that does not depend on the root chain,
+// one wavefront per point, all points at once: its source position, the
+// distance to the nearest caustic sample, and the list of positions the
+// chain will solve for (the centre, then the caustic trial points) with the
+// image polynomial of each (tl_coeffs.h: the reference's rounding).  The
+// list goes to the pool at a place taken with one atomic add; when the pool
+// is too small, the cursor still advances and nothing is written, and the
+// host runs the kernel again with a bigger pool.
+__global__ void __launch_bounds__(64) trial_kernel(TrialArgs a) {
+  const int p = blockIdx.x;
+  if (p >= a.nPts) return;
+  const int ln = __lane_id();
+  const CurveDev& C = a.curves[a.ptCurve[p]];
+  const Lens L = C.L;
+  const unsigned i = (unsigned)(p - C.pointBase);
+  const cx pos = traj_pos(mk(C.sx, C.sy), mk(C.ex, C.ey), i, C.lcLength);
+  const cx* caus = a.caustics + C.causticBase;
+  // distance to the nearest of the curve's caustic samples
+  double d2 = 1e300;
+  for (int q = ln; q < 6 * kCccLen; q += 64) {
+    const double v = cnorm(csub(caus[q], pos));
+    d2 = v < d2 ? v : d2;
+  }
+  for (int o = 32; o >= 1; o >>= 1) {
+    const double w2 = __shfl_xor(d2, o, 64);
+    d2 = w2 < d2 ? w2 : d2;
+  }
+  if (ln == 0) {
+    a.ptPos[p] = make_double2(pos.re, pos.im);
+    a.ptCausD2[p] = d2;
+  }
+  int n = 1;
+  scan_trials(C, pos, caus, [&](unsigned long long f, const cx&) { n += __builtin_popcountll(f); });
+  int base = 0;
+  if (ln == 0) base = (int)atomicAdd(a.pool.cursor, (unsigned long long)n);
+  base = __shfl(base, 0, 64);
+  if (ln == 0) {
+    a.pool.base[p] = base;
+    a.pool.count[p] = n;
+  }
+  if ((long long)base + n > a.pool.cap) return;
+  // the centre on lane 0, then each step's trial points on their own lanes
+  if (ln == 0) {
+    a.pool.zeta[base] = pos;
+    cx c[11];
+    image_poly(L, C.fixedPoly, pos, c);
+    for (int k = 0; k < 11; k++) a.pool.coef[(size_t)base * 11 + k] = c[k];
+  }
+  int next = base + 1;
+  scan_trials(C, pos, caus, [&](unsigned long long f, const cx& trial) {
+    if ((f >> ln) & 1ull) {
+      const int sid = next + __builtin_popcountll(f & ((1ull << ln) - 1ull));
+      a.pool.zeta[sid] = trial;
+      cx c[11];
+      image_poly(L, C.fixedPoly, trial, c);
+      for (int k = 0; k < 11; k++) a.pool.coef[(size_t)sid * 11 + k] = c[k];
+    }
+    next += __builtin_popcountll(f);
+  });
+}
+
+int launch_trials(const TrialArgs& a, hipStream_t s) {
+  if (a.nPts <= 0) return 0;
+  hipLaunchKernelGGL(trial_kernel, dim3(a.nPts), dim3(64), 0, s, a);
+  return (int)hipGetLastError();
+}
+
+// image_poly for n positions of one lens, one lane each (test access to the
+// device evaluation of the polynomial).
+__global__ void __launch_bounds__(64) image_poly_kernel(Lens L, const cx* fixedPoly,
+                                                        const cx* zeta, int n, cx* coef) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  cx c[11];
+  image_poly(L, fixedPoly, zeta[g], c);
+  for (int k = 0; k < 11; k++) coef[(size_t)g * 11 + k] = c[k];
+}
+
+int launch_image_poly(const Lens& L, const cx* fixedPoly, const cx* zeta, int n, cx* coef,
+                      hipStream_t s) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(image_poly_kernel, dim3((n + 63) / 64), dim3(64), 0, s, L, fixedPoly, zeta,
+                     n, coef);
+  return (int)hipGetLastError();
+}
+
 // Workgroup (curve c, piece q): a curve with split = 1 walks the chunk's
 // positions as one chain, continuing the curve's root-reuse state.  With
 // split = S > 1 the chunk's positions are cut into S contiguous pieces that
 // run in parallel; piece 0 continues the chain, the others start from a
-// fresh solve.
+// fresh solve.  Each position's solves (centre, then caustic trial points)
+// and their polynomials come from trial_kernel's pool, so the chain itself
+// only runs the root solves, the image tests and the seed output.
 __global__ void __launch_bounds__(64) seed_kernel(SeedArgs a) {
   const int c = blockIdx.x / a.maxSplit, q = blockIdx.x % a.maxSplit;
   if (c >= a.nCurves) return;
@@ -441,84 +498,43 @@ __global__ void __launch_bounds__(64) seed_kernel(SeedArgs a) {
   const CurveDev& C = a.curves[c];
   if (q >= C.split) return;
   const Lens L = C.L;
-  const dcx Dv = ln < 4 ? C.pb.D[ln < 4 ? ln : 0] : dc_zero();
-  const dcx Sv = ln < 3 ? C.pb.S[ln < 3 ? ln : 0] : dc_zero();
   cx chainv = mk(0.0, 0.0);
-  int have = 0;
+  int have = 0, calls = 0;
   if (q > 0) {
     // fresh solve at the piece's first position
   } else if (a.chainIn) {
     if (ln <= 10) chainv = a.chainIn[c].tempRoots[ln <= 10 ? ln : 0];
     have = a.chainIn[c].have;
+    calls = a.chainIn[c].calls;
   } else {
     if (ln <= 10) chainv = C.chain[ln <= 10 ? ln : 0];
     have = C.chainHave;
+    calls = C.chainCalls;
   }
-  const cx s = mk(C.sx, C.sy), e = mk(C.ex, C.ey);
-  const cx* caus = a.caustics + C.causticBase;
+  const bool traced = q == 0 && a.slotTrace && C.slotCount > 0;
   const long long n = (long long)C.lcLength + 1;
   const long long cBegin = n * a.chunk / a.nChunks, cEnd = n * (a.chunk + 1) / a.nChunks;
   const unsigned iBegin = (unsigned)(cBegin + (cEnd - cBegin) * q / C.split);
   const unsigned iEnd = (unsigned)(cBegin + (cEnd - cBegin) * (q + 1) / C.split);
   for (unsigned i = iBegin; i < iEnd; i++) {
     const int p = C.pointBase + (int)i;
-    const cx pos = traj_pos(s, e, i, C.lcLength);
-    if (ln == 0) a.ptPos[p] = make_double2(pos.re, pos.im);
+    const int base = a.pool.base[p], nSolve = a.pool.count[p];
     int cnt = 0;
     uint32_t fl = 0;
     int2* out = a.seeds + (size_t)p * kMaxSeeds;
     double amp = 0.0;
-    // The source positions solved for this point, in the reference's order
-    // (lcirs.cc:131-165): the centre, then on each caustic branch whose
-    // cumulative box meets the source square the first segment of every run
-    // of consecutive segments that meet the source (64 segments at a time).
-    int r = -1;              // caustic branch being scanned (-1: centre next)
-    bool findBranch = true;  // r has to advance to a branch whose box meets
-    int k0 = 0;              // next chunk of branch r
-    bool prevHit = false;    // hit of the segment before that chunk
-    unsigned long long f = 0;  // first-hit segments of the chunk not yet solved
-    cx trial = mk(0.0, 0.0);   // this lane's segment's trial position
-    const cx* cr = caus;
-    while (true) {
-      cx zeta;
-      const bool centre = r < 0;
-      if (centre) {
-        zeta = pos;
-        r = 0;
-      } else {
-        while (f == 0) {
-          if (findBranch) {
-            while (r < 6 && !(pos.re + C.rho >= C.boxMin[r].re && pos.re - C.rho <= C.boxMax[r].re &&
-                              pos.im + C.rho >= C.boxMin[r].im && pos.im - C.rho <= C.boxMax[r].im))
-              r++;
-            if (r >= 6) break;
-            cr = caus + r * kCccLen;
-            k0 = 0;
-            prevHit = false;
-            findBranch = false;
-          }
-          if (k0 >= kCccLen - 1) {
-            r++;
-            findBranch = true;
-            continue;
-          }
-          const int k = k0 + ln;
-          trial = mk(0.0, 0.0);
-          const bool hit = k < kCccLen - 1 && intersection_check(pos, cr[k], cr[k + 1], C.rho,
-                                                                 C.rho2pow, &trial);
-          const unsigned long long hm = __ballot(hit);
-          f = hm & ~((hm << 1) | (prevHit ? 1ull : 0ull));
-          prevHit = (hm >> 63) & 1ull;
-          k0 += 64;
-        }
-        if (f == 0) break;
-        const int j = __builtin_ctzll(f);
-        f &= f - 1;
-        zeta = rlc(trial, j);
-      }
-      const unsigned long long m = wl_get_images(L, Dv, Sv, chainv, have, zeta, a.trig, C.binImg);
+    for (int sIdx = 0; sIdx < nSolve; sIdx++) {
+      const size_t sid = (size_t)base + sIdx;
+      const cx cv = ln <= 10 ? a.pool.coef[sid * 11 + (ln <= 10 ? ln : 0)] : mk(0.0, 0.0);
+      const cx zeta = a.pool.zeta[sid];
+      if (traced && have && calls < C.slotCount && ln == 10)
+        chainv = a.slotTrace[C.slotBase + calls];
+      calls++;
+      bool oob = false;
+      const unsigned long long m = wl_get_images(L, cv, chainv, have, zeta, a.trig, C.binImg, oob);
+      if (oob) fl |= kFlagKickTable;
       emit_seeds(C, chainv, m, out, cnt, fl);
-      if (centre) {
+      if (sIdx == 0) {
         // point-source magnification (getPointAmp, lcbase.cc:17-38) and the
         // magnification-weighted image centroid, summed in root order
         const double term = ((m >> ln) & 1ull) ? point_amp_term(L, chainv) : 0.0;
@@ -531,17 +547,6 @@ __global__ void __launch_bounds__(64) seed_kernel(SeedArgs a) {
           amp += rl(term, k);
           sx += rl(wx, k);
           sy += rl(wy, k);
-        }
-        // distance to the nearest of the curve's caustic samples
-        double d2 = 1e300;
-        for (int q = ln; q < 6 * kCccLen; q += 64) {
-          const cx dq = csub(caus[q], pos);
-          const double v = cnorm(dq);
-          d2 = v < d2 ? v : d2;
-        }
-        for (int o = 32; o >= 1; o >>= 1) {
-          const double w2 = __shfl_xor(d2, o, 64);
-          d2 = w2 < d2 ? w2 : d2;
         }
         // the two largest image magnifications
         double t1 = term;
@@ -560,7 +565,7 @@ __global__ void __launch_bounds__(64) seed_kernel(SeedArgs a) {
         if (ln == 0) {
           a.ptAmp[p] = amp;
           a.ptCen[p] = make_double2(sx / amp, sy / amp);
-          const double dist = sqrt(d2);
+          const double dist = sqrt(a.ptCausD2[p]);
           // inside a caustic: more images than the 4 (binary: 3) outside
           a.ptCaus[p] = __builtin_popcountll(m) > (C.binImg ? 3 : 4) ? -dist : dist;
           a.ptPair[p] = amp > 0.0 ? (t1 + t2) / amp : 0.0;
@@ -575,7 +580,10 @@ __global__ void __launch_bounds__(64) seed_kernel(SeedArgs a) {
   }
   if (q == C.split - 1) {
     if (ln <= 10) a.chainOut[c].tempRoots[ln] = chainv;
-    if (ln == 0) a.chainOut[c].have = have;
+    if (ln == 0) {
+      a.chainOut[c].have = have;
+      a.chainOut[c].calls = calls;
+    }
   }
 }
 
@@ -591,14 +599,14 @@ __global__ void __launch_bounds__(64) point_lc_kernel(const CurveDev* curves, in
   const int ln = __lane_id();
   const CurveDev& C = curves[c];
   const Lens L = C.L;
-  const dcx Dv = ln < 4 ? C.pb.D[ln < 4 ? ln : 0] : dc_zero();
-  const dcx Sv = ln < 3 ? C.pb.S[ln < 3 ? ln : 0] : dc_zero();
   cx chainv = ln <= 10 ? C.chain[ln <= 10 ? ln : 0] : mk(0.0, 0.0);
   int have = C.chainHave;
   const cx s = mk(C.sx, C.sy), e = mk(C.ex, C.ey);
   for (unsigned i = 0; i < C.lcLength; i++) {
     const cx pos = cadd(s, cscale(csub(e, s), i / (C.lcLength - 1.0)));
-    unsigned long long m = wl_get_images(L, Dv, Sv, chainv, have, pos, trig, C.binImg);
+    const cx cv = wl_coeffs(L, C.fixedPoly, pos);
+    bool oob = false;
+    unsigned long long m = wl_get_images(L, cv, chainv, have, pos, trig, C.binImg, oob);
     const double term = ((m >> ln) & 1ull) ? point_amp_term(L, chainv) : 0.0;
     double amp = 0.0;
     while (m) {
@@ -609,7 +617,10 @@ __global__ void __launch_bounds__(64) point_lc_kernel(const CurveDev* curves, in
     if (ln == 0) out[C.pointBase + (int)i] = amp;
   }
   if (ln <= 10) chainOut[c].tempRoots[ln] = chainv;
-  if (ln == 0) chainOut[c].have = have;
+  if (ln == 0) {
+    chainOut[c].have = have;
+    chainOut[c].calls = C.chainCalls + (int)C.lcLength;
+  }
 }
 
 int launch_point_lc(const CurveDev* curves, int nCurves, const TrigTable* trig, double* out,
@@ -651,8 +662,6 @@ __global__ void __launch_bounds__(64) ring_amp_kernel(const CurveDev* curves, co
   if (!C.ring) return;
   const int ln = __lane_id();
   const Lens L = C.L;
-  const dcx Dv = ln < 4 ? C.pb.D[ln < 4 ? ln : 0] : dc_zero();
-  const dcx Sv = ln < 3 ? C.pb.S[ln < 3 ? ln : 0] : dc_zero();
   const cx centre = ln < 10 ? ptRoots[(size_t)p * 10 + (ln < 10 ? ln : 0)] : mk(0.0, 0.0);
   const double2 pp = ptPos[p];
   const bool img0 = ln < 10 && img_check(L, centre, mk(pp.x, pp.y), 1.0e-6);
@@ -664,7 +673,9 @@ __global__ void __launch_bounds__(64) ring_amp_kernel(const CurveDev* curves, co
     cx chainv = centre;
     int have = 1;
     const cx zeta = mk(pp.x + C.rho * kRingOff[s][0], pp.y + C.rho * kRingOff[s][1]);
-    unsigned long long m = wl_get_images(L, Dv, Sv, chainv, have, zeta, trig, C.binImg);
+    const cx cv = wl_coeffs(L, C.fixedPoly, zeta);
+    bool oob = false;
+    unsigned long long m = wl_get_images(L, cv, chainv, have, zeta, trig, C.binImg, oob);
     const bool img = (m >> ln) & 1ull;
     const double term = img ? point_amp_term(L, chainv) : 0.0;
     double amp = 0.0;
@@ -947,7 +958,7 @@ __device__ __forceinline__ void afaddi(int* p, int v) {
 // in registers across the scan.
 struct RowCtx {
   Lens L;
-  double blx, bly, step, rho2, ldA, ldB;
+  double blx, bly, step, rho2, ldA, ldB, sumScale;
   double y, px, py;
   int N;
   int binIrs;  // binary ray map (irsBinary)
@@ -955,9 +966,10 @@ struct RowCtx {
 
 // Ray test of pixel x of the row (lcirs.cc:400-402) and its brightness
 // A + B sqrt(1 - r^2) (lcirs.cc:406-407, limbdarkeningmodel.cc:45-49) as a
-// multiple of 2^-34.  The test is the reference's to the bit (irsSIMD, or
-// irsBinary for m3 = 0); the brightness is computed with IEEE division and
-// square root and rounded to fixed point by adding 2^52.
+// multiple of 1/sumScale.  The test is the reference's to the bit (irsSIMD,
+// or irsBinary for m3 = 0); the brightness is computed with IEEE division
+// and square root and rounded to the nearest integer multiple by adding
+// 1.5 * 2^52 (exact for |value| below 2^51, either sign).
 __device__ __forceinline__ bool eval_px(const RowCtx& R, int x, long long& val) {
   const double xx = R.blx + R.step * (double)x;
   const double rsq = R.binIrs ? irs_binary(R.L, xx, R.y, R.px, R.py)
@@ -966,8 +978,8 @@ __device__ __forceinline__ bool eval_px(const RowCtx& R, int x, long long& val) 
   // sourceBrightnessRSq(rsq / rho^2) (lcirs.cc:406-407,
   // limbdarkeningmodel.cc:45-49): IEEE division and square root
   const double v = R.ldA + R.ldB * sqrt(1.0 - rsq / R.rho2);
-  const double t = fma(v, kSumScale, 4503599627370496.0);  // 2^52: integer in the low bits
-  val = __double_as_longlong(t) - 0x4330000000000000ll;
+  const double t = fma(v, R.sumScale, 6755399441055744.0);  // 1.5 * 2^52
+  val = __double_as_longlong(t) - 0x4338000000000000ll;
   return true;
 }
 
@@ -1067,10 +1079,20 @@ __device__ __forceinline__ Task ring_pop(TaskState& S) {
   return S.ring[(S.lhead + S.lcount) & (kLocalCap - 1)];
 }
 
+// Adds the signed value v to the 128-bit integer (p[0] low, p[1] high).
+// Each fetch-add on the low word reports its own carry, so concurrent adds
+// sum exactly.
+__device__ __forceinline__ void add128(unsigned long long* p, long long v) {
+  const unsigned long long u = (unsigned long long)v;
+  const unsigned long long old = atomicAdd(p, u);
+  const unsigned long long hi = (v < 0 ? ~0ull : 0ull) + (old + u < old ? 1ull : 0ull);
+  if (hi) atomicAdd(p + 1, hi);
+}
+
 __device__ __forceinline__ void flush_acc(TaskState& S) {
   if (S.accCnt && __lane_id() == 0) {
     atomicAdd(&S.A->lcount[S.accLabel], S.accCnt);
-    atomicAdd((unsigned long long*)&S.A->lsum[S.accLabel], (unsigned long long)S.accSum);
+    add128(&S.A->lsum[2 * (size_t)S.accLabel], S.accSum);
   }
   S.accCnt = 0;
   S.accSum = 0;
@@ -1154,7 +1176,9 @@ __device__ __forceinline__ void handle_run(TaskState& S, int xl, int xr, long lo
   S.nIns++;
   S.nScoutRuns += S.kind == 2 ? 1 : 0;
   if (xl <= 1 || xr >= S.N - 2) flag_point(A, S.gp, kFlagEdge);
-  if (label != S.accLabel) {
+  // a run adds at most 2^60 units: flushing from 2^61 on keeps the total
+  // within 2^62
+  if (label != S.accLabel || S.accSum > (1ll << 61) || S.accSum < -(1ll << 61)) {
     flush_acc(S);
     S.accLabel = label;
   }
@@ -1400,6 +1424,7 @@ __device__ __forceinline__ void process_task(TaskState& S) {
     S.R.rho2 = unid(C.rho2);
     S.R.ldA = unid(C.ldA);
     S.R.ldB = unid(C.ldB);
+    S.R.sumScale = unid(C.sumScale);
     S.R.binIrs = uni(C.binIrs);
     S.heavy = A.shareRays > 0.0f && A.ptCost[S.gp] >= A.shareRays;
     S.R.N = S.N;
@@ -1671,7 +1696,8 @@ __global__ void fill_reset_kernel(FillArgs A) {
     A.lparent[i] = (int)i;
     A.lpoint[i] = i < (unsigned long long)A.nBatch ? (int)i : -1;
     A.lcount[i] = 0;
-    A.lsum[i] = 0;
+    A.lsum[2 * i] = 0;
+    A.lsum[2 * i + 1] = 0;
   }
   for (unsigned long long i = tid; i < A.qcap; i += stride) A.qseq[i] = i;
   if (tid == 0) {
@@ -1693,7 +1719,7 @@ int launch_fill_reset(const FillArgs& a, hipStream_t s) {
 // Adds every label whose class holds the reference-seed label of its point
 // to that point's totals.
 __global__ void resolve_kernel(FillArgs A, int nLabels, unsigned long long* ptCount,
-                               long long* ptSum) {
+                               unsigned long long* ptSum) {
   const int L = blockIdx.x * blockDim.x + threadIdx.x;
   if (L >= nLabels) return;
   const int b = A.lpoint[L];
@@ -1703,10 +1729,16 @@ __global__ void resolve_kernel(FillArgs A, int nLabels, unsigned long long* ptCo
   const int rb = find_root(A.lparent, b);
   if (r != rb) return;
   atomicAdd(&ptCount[b], A.lcount[L]);
-  atomicAdd((unsigned long long*)&ptSum[b], (unsigned long long)A.lsum[L]);
+  // the label's 128-bit total into the point's: low word with its carry,
+  // then the high word
+  const unsigned long long lo = A.lsum[2 * (size_t)L], hi = A.lsum[2 * (size_t)L + 1];
+  const unsigned long long old = atomicAdd(&ptSum[2 * (size_t)b], lo);
+  const unsigned long long carry = old + lo < old ? 1ull : 0ull;
+  if (hi + carry) atomicAdd(&ptSum[2 * (size_t)b + 1], hi + carry);
 }
 
-int launch_resolve(const FillArgs& a, int nLabels, unsigned long long* ptCount, long long* ptSum,
+int launch_resolve(const FillArgs& a, int nLabels, unsigned long long* ptCount,
+                   unsigned long long* ptSum,
                    hipStream_t s) {
   if (nLabels <= 0) return 0;
   hipLaunchKernelGGL(resolve_kernel, dim3((nLabels + 255) / 256), dim3(256), 0, s, a, nLabels,

@@ -15,8 +15,8 @@
 //   * image test: src/Images/imgpoint.cc:72-90;
 //   * caustic intersection test: src/LightCurves/lcirs.cc:195-255;
 //   * ray map: src/LightCurves/lcirs.cc:290-341 (irsSIMD lane order).
-// The degree-10 image polynomial is this project's own derivation, evaluated
-// in double-double and rounded once (see lens_poly).
+// The degree-10 image polynomial (src/Images/imgpointcoeff.cc:223-447, in the
+// reference's operation order) is in tl_coeffs.h.
 #pragma once
 
 #include <math.h>
@@ -389,115 +389,6 @@ TL_HD void critical_curves_into(const Lens& L, const CccTable& tab, const TrigTa
     }
 }
 
-// ---- double-double polynomial arithmetic for the image polynomial ----------
-struct dd { double hi, lo; };
-struct dcx { dd re, im; };
-
-TL_HD dd two_sum(double a, double b) {
-  double s = a + b, bb = s - a;
-  dd r; r.hi = s; r.lo = (a - (s - bb)) + (b - bb);
-  return r;
-}
-TL_HD dd quick_two_sum(double a, double b) {
-  double s = a + b;
-  dd r; r.hi = s; r.lo = b - (s - a);
-  return r;
-}
-TL_HD dd dd_add(dd a, dd b) {
-  dd s = two_sum(a.hi, b.hi), t = two_sum(a.lo, b.lo);
-  s.lo += t.hi;
-  s = quick_two_sum(s.hi, s.lo);
-  s.lo += t.lo;
-  return quick_two_sum(s.hi, s.lo);
-}
-TL_HD dd dd_neg(dd a) { dd r; r.hi = -a.hi; r.lo = -a.lo; return r; }
-TL_HD dd dd_mul(dd a, dd b) {
-  double p = a.hi * b.hi;
-  double e = fma(a.hi, b.hi, -p);
-  e += a.hi * b.lo + a.lo * b.hi;
-  return quick_two_sum(p, e);
-}
-TL_HD dd dd_of(double x) { dd r; r.hi = x; r.lo = 0.0; return r; }
-TL_HD dcx dc_of(cx z) { dcx r; r.re = dd_of(z.re); r.im = dd_of(z.im); return r; }
-TL_HD dcx dc_add(dcx a, dcx b) { dcx r; r.re = dd_add(a.re, b.re); r.im = dd_add(a.im, b.im); return r; }
-TL_HD dcx dc_sub(dcx a, dcx b) {
-  dcx r; r.re = dd_add(a.re, dd_neg(b.re)); r.im = dd_add(a.im, dd_neg(b.im));
-  return r;
-}
-TL_HD dcx dc_mul(dcx a, dcx b) {
-  dcx r;
-  r.re = dd_add(dd_mul(a.re, b.re), dd_neg(dd_mul(a.im, b.im)));
-  r.im = dd_add(dd_mul(a.re, b.im), dd_mul(a.im, b.re));
-  return r;
-}
-TL_HD dcx dc_scale(dcx a, dd t) { dcx r; r.re = dd_mul(a.re, t); r.im = dd_mul(a.im, t); return r; }
-TL_HD cx dc_round(dcx a) { return mk(a.re.hi + a.re.lo, a.im.hi + a.im.lo); }
-TL_HD dcx dc_zero() { dcx r; r.re = dd_of(0.0); r.im = dd_of(0.0); return r; }
-
-template <int NP, int NQ>
-TL_HD void dpmul(const dcx* p, const dcx* q, dcx* out) {
-  for (int i = 0; i < NP + NQ - 1; i++) out[i] = dc_zero();
-  for (int i = 0; i < NP; i++)
-    for (int j = 0; j < NQ; j++) out[i + j] = dc_add(out[i + j], dc_mul(p[i], q[j]));
-}
-
-// Zeta-independent factors of the image polynomial: D = prod (z - z_k) and
-// S = sum m_k D/(z - z_k), kept in double-double.
-struct PolyBase { dcx D[4]; dcx S[3]; };
-
-TL_HD void poly_base(const Lens& L, PolyBase* pb) {
-  dcx one = dc_of(mk(1.0, 0.0));
-  dd m1 = dd_of(L.m1), m2 = dd_of(L.m2), m3 = dd_of(L.m3);
-  dcx f1[2] = {dc_of(cneg(L.z1)), one}, f2[2] = {dc_of(cneg(L.z2)), one},
-      f3[2] = {dc_of(cneg(L.z3)), one};
-  dcx d12[3], d13[3], d23[3];
-  dpmul<2, 2>(f1, f2, d12);
-  dpmul<2, 2>(f1, f3, d13);
-  dpmul<2, 2>(f2, f3, d23);
-  dpmul<3, 2>(d12, f3, pb->D);
-  for (int i = 0; i < 3; i++)
-    pb->S[i] = dc_add(dc_add(dc_scale(d23[i], m1), dc_scale(d13[i], m2)), dc_scale(d12[i], m3));
-}
-
-// Image polynomial at source position zeta.  Conjugating the lens equation
-// gives conj(z) = conj(zeta) + sum m_k/(z - z_k) = Nn/D; with
-// Q_k = Nn - conj(z_k) D = (conj(zeta) - conj(z_k)) D + S, substitution gives
-//   (z - zeta) Q1 Q2 Q3 - D (m1 Q2 Q3 + m2 Q1 Q3 + m3 Q1 Q2) = 0.
-// Coefficients (lowest order first) are returned with the sign of the
-// reference's polynomial (imgpointcoeff.cc:223-447), i.e. this one times -1,
-// each the double-double value rounded once.
-TL_HD void lens_poly(const Lens& L, const PolyBase& pb, cx zeta, cx* c) {
-  dcx one = dc_of(mk(1.0, 0.0));
-  dd m1 = dd_of(L.m1), m2 = dd_of(L.m2), m3 = dd_of(L.m3);
-  dcx zc = dc_of(cconj(zeta));
-  dcx w1 = dc_sub(zc, dc_of(cconj(L.z1))), w2 = dc_sub(zc, dc_of(cconj(L.z2))),
-      w3 = dc_sub(zc, dc_of(cconj(L.z3)));
-  dcx Q1[4], Q2[4], Q3[4];
-  for (int i = 0; i < 4; i++) {
-    dcx s = i < 3 ? pb.S[i] : dc_zero();
-    Q1[i] = dc_add(dc_mul(w1, pb.D[i]), s);
-    Q2[i] = dc_add(dc_mul(w2, pb.D[i]), s);
-    Q3[i] = dc_add(dc_mul(w3, pb.D[i]), s);
-  }
-  dcx q12[7], q13[7], q23[7], q123[10];
-  dpmul<4, 4>(Q1, Q2, q12);
-  dpmul<4, 4>(Q1, Q3, q13);
-  dpmul<4, 4>(Q2, Q3, q23);
-  dpmul<7, 4>(q12, Q3, q123);
-  dcx lin[2] = {dc_of(cneg(zeta)), one};
-  dcx left[11];
-  dpmul<2, 10>(lin, q123, left);
-  dcx mix[7];
-  for (int i = 0; i < 7; i++)
-    mix[i] = dc_add(dc_add(dc_scale(q23[i], m1), dc_scale(q13[i], m2)), dc_scale(q12[i], m3));
-  dcx right[10];
-  dpmul<4, 7>(pb.D, mix, right);
-  for (int i = 0; i < 11; i++) {
-    dcx r = i < 10 ? right[i] : dc_zero();
-    c[i] = dc_round(dc_sub(r, left[i]));
-  }
-}
-
 // ImgPoint::imgCheck (imgpoint.cc:72-90)
 TL_HD bool img_check(const Lens& L, cx img, cx zeta, double tol) {
   cx t = csub(img, cdiv_rc(L.m1, cconj(csub(img, L.z1))));
@@ -507,37 +398,13 @@ TL_HD bool img_check(const Lens& L, cx img, cx zeta, double tol) {
 }
 
 // Root-reuse state of one ImgPoint (imgpoint.cc:160-205): tempRoots[10] is
-// the slot the reference reads one past the end of its vector; it stays 0.
+// the slot the reference reads one past the end of its vector (0 here unless
+// a recorded value is supplied); calls counts the getImages calls made.
 struct RootChain {
   cx tempRoots[11];
   int have;
+  int calls;
 };
-
-// getRootsPrecalculated + getImages for a triple lens (imgpoint.cc:160-274).
-// imgs receives the accepted roots in root order; returns their number.
-TL_HD int get_images(const Lens& L, const PolyBase& pb, RootChain* rc, cx zeta, cx* imgs,
-                     const TrigTable* trig) {
-  cx c[11];
-  lens_poly(L, pb, zeta, c);
-  for (int attempt = 0; attempt < 2; attempt++) {
-    if (!rc->have || attempt == 1) {
-      lag_solve<10>(c, nullptr, rc->tempRoots, trig);
-      rc->have = 1;
-    } else {
-      cx out[10];
-      lag_solve<10>(c, rc->tempRoots, out, trig);
-      if (!lag_check<10>(c, out)) lag_solve<10>(c, nullptr, out, trig);
-      for (int k = 0; k < 10; k++) rc->tempRoots[k] = out[k];
-    }
-    rc->tempRoots[10] = mk(0.0, 0.0);
-    int n = 0;
-    for (int k = 0; k < 10; k++)
-      if (img_check(L, rc->tempRoots[k], zeta, 1.0e-6)) imgs[n++] = rc->tempRoots[k];
-    if (!(n % 2 != 0 || n < 4)) return n;
-    if (attempt == 1) return n;
-  }
-  return 0;
-}
 
 // LightCurveIRS::intersectionCheck (lcirs.cc:195-255)
 TL_HD bool intersection_check(cx s, cx A, cx B, double rho, double rho2pow, cx* trial) {

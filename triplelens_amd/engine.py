@@ -23,6 +23,9 @@ FLAG_LABEL_FULL = 8
 FLAG_QUEUE_FULL = 16
 FLAG_TIMEOUT = 32
 FLAG_ROW_OVERFLOW = 64
+FLAG_KICK_TABLE = 128   # the reference's result at the point is undefined (see tl_lcirs.h)
+# flags that report on a point without making its result incomplete
+FLAGS_INFO = FLAG_EDGE | FLAG_KICK_TABLE
 
 _D = ctypes.c_double
 _P = ctypes.c_void_p
@@ -128,6 +131,17 @@ class LightCurve:
         self._lib.set_limb_darkening(self.h, model.encode(), float(v))
         _check()
 
+    def set_slot_trace(self, slots):
+        """lcirs_set_slot_trace: getImages call k of this object's chain (from
+        the next light curve on) starts its chained solve with tempRoots[10]
+        = slots[k] (complex) instead of 0; None or empty clears it."""
+        self._lib.lcirs_set_slot_trace.argtypes = [_P, _P, ctypes.c_long]
+        if slots is None or len(slots) == 0:
+            _check(self._lib.lcirs_set_slot_trace(self.h, None, 0))
+            return
+        xy = np.ascontiguousarray(slots, np.complex128)
+        _check(self._lib.lcirs_set_slot_trace(self.h, xy.ctypes.data, len(xy)))
+
     def lc_irs(self, sx, sy, ex, ey):
         """Runs get_lc_irs; returns (magnification, rays, flags), lcLength+1 each."""
         self._lib.get_lc_irs(self.h, sx, sy, ex, ey)
@@ -143,6 +157,13 @@ class LightCurve:
         _check(self._lib.lcirs_copy_ray_counts(self.h, rays.ctypes.data))
         _check(self._lib.lcirs_copy_flags(self.h, flags.ctypes.data))
         return amp, rays, flags
+
+    def flags(self):
+        """Status flags of every position of the last seed search or light
+        curve (TL_FLAG_* bits)."""
+        flags = np.zeros(self.lc_len + 1, np.uint32)
+        _check(self._lib.lcirs_copy_flags(self.h, flags.ctypes.data))
+        return flags
 
     def point_amp(self):
         """Point-source magnification, weighted image centroid and signed
@@ -265,6 +286,19 @@ class LightCurve:
                                                     pix.ctypes.data, rays.ctypes.data,
                                                     amp.ctypes.data, flags.ctypes.data))
         return amp, rays, flags
+
+
+def image_poly(a, b, th, m2, m3, zeta, on_device=False):
+    """tl_image_poly: the degree-10 image polynomial (coefficients of z^0..z^10)
+    at each source position of ``zeta``, as the reference rounds it; on the
+    host or with the engine's GPU kernel.  Returns an (n, 11) complex array."""
+    lib = load()
+    z = np.ascontiguousarray(np.atleast_1d(zeta), np.complex128)
+    out = np.zeros((len(z), 11), np.complex128)
+    lib.tl_image_poly.argtypes = [_D] * 5 + [ctypes.c_int, _P, _P, ctypes.c_int]
+    _check(lib.tl_image_poly(a, b, th, m2, m3, len(z), z.ctypes.data, out.ctypes.data,
+                             1 if on_device else 0))
+    return out
 
 
 def ccc_batch(params):

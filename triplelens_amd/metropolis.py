@@ -72,7 +72,7 @@ def exact_curves(curves: list[Curve]) -> list[np.ndarray]:
     res = E.batch_irs(lcs, np.array([c.traj() for c in curves]))
     out = []
     for amp, _rays, flags in res:
-        bad = int(np.bitwise_or.reduce(flags)) & ~E.FLAG_EDGE
+        bad = int(np.bitwise_or.reduce(flags)) & ~E.FLAGS_INFO
         if bad:
             raise RuntimeError(f"ray shooting reported status flags {bad:#x}")
         out.append(amp)

@@ -54,6 +54,12 @@ class SurrogateError(RuntimeError):
     pass
 
 
+def _check_flags(flags):
+    bad = int(np.bitwise_or.reduce(flags)) & ~_E.FLAGS_INFO
+    if bad:
+        raise SurrogateError(f"ray shooting reported status flags {bad:#x}")
+
+
 def _lib():
     lib = _E.load()
     if getattr(lib, "_tl_mlp_declared", False):
@@ -345,6 +351,9 @@ class Surrogate:
         self.sigma_max = np.broadcast_to(np.asarray(sigma_max, np.float64),
                                          (len(self.band_edges) + 1,)).copy()
         self.audit = audit
+        # an audited position further than this (relative) from its ray-shot
+        # value fails the audit: the whole curve is then ray-shot
+        self.audit_bound = 1e-3
         # shot positions are filled from the union of four split searches;
         # positions per piece of the three extra ones (None: seed_piece)
         self.seed_union = True
@@ -467,9 +476,13 @@ class Surrogate:
 
     def light_curve(self, lc, c, rng=None, stats=None):
         """Magnification at the lcLength+1 positions of curve c on engine object
-        lc.  Returns (A, info) with info = fallback, audited masks and the
-        audit's relative errors.  With a list `stats`, the engine statistics
-        (engine.last_stats) of each engine run are appended to it."""
+        lc.  Returns (A, info) with info = fallback, audited masks, the
+        audit's relative errors and audit_failed.  Audited positions carry
+        their ray-shot value.  When an audit misses the bound (audit_bound),
+        every remaining position of the curve is ray-shot as well (and
+        marked fallback), so the curve returned is plain ray shooting.  With
+        a list `stats`, the engine statistics (engine.last_stats) of each
+        engine run are appended to it."""
         return self.light_curves([lc], [c], rng, stats)[0]
 
     def light_curves(self, lcs, curves, rng=None, stats=None):
@@ -521,25 +534,53 @@ class Surrogate:
                 seeds_list.append(sd)
                 shot_k.append(k)
         out = [None] * len(lcs)
+        failed = []  # curves whose audit missed the bound
         if shots:
             res = _E.batch_fill_positions([lcs[k] for k in shot_k], shots,
                                           seeds_list if union else None)
             if stats is not None:
                 stats.append(_E.last_stats())
             for k, idx, (A_irs, rays, flags) in zip(shot_k, shots, res):
-                bad = int(np.bitwise_or.reduce(flags)) & ~_E.FLAG_EDGE
-                if bad:
-                    raise SurrogateError(f"ray shooting reported status flags {bad:#x}")
+                _check_flags(flags)
                 A, fallback, audited = parts[k]
                 full = np.zeros(len(A))
                 full[idx] = A_irs
                 errs = np.abs(A[audited] - full[audited]) / full[audited]
-                A = np.where(fallback, full, A)
+                A = np.where(fallback | audited, full, A)
                 out[k] = (A, {"fallback": fallback, "audited": audited, "audit_rel_err": errs,
-                              "rays": int(rays.sum())})
+                              "audit_failed": False, "rays": int(rays.sum())})
+                if errs.size and not errs.max() <= self.audit_bound:
+                    failed.append(k)
         for k in range(len(lcs)):
             if out[k] is None:
                 A, fallback, audited = parts[k]
                 out[k] = (A, {"fallback": fallback, "audited": audited,
-                              "audit_rel_err": np.zeros(0), "rays": 0})
+                              "audit_rel_err": np.zeros(0), "audit_failed": False, "rays": 0})
+        if failed:
+            # the network is off on these curves: ray-shoot everything it
+            # answered, from the same seeds as the other shot positions
+            rest, rest_seeds = [], []
+            for k in failed:
+                A, info = out[k]
+                idx = np.nonzero(~(info["fallback"] | info["audited"]))[0]
+                ns, pix = lcs[k].seeds()
+                sd = (ns[idx], pix[idx])
+                for cl in clones_of[k]:
+                    ns2, pix2 = cl.seeds()
+                    sd = union_seeds(sd[0], sd[1], ns2[idx], pix2[idx])
+                rest.append(idx)
+                rest_seeds.append(sd)
+            res = _E.batch_fill_positions([lcs[k] for k in failed], rest,
+                                          rest_seeds if union else None)
+            if stats is not None:
+                stats.append(_E.last_stats())
+            for k, idx, (A_irs, rays, flags) in zip(failed, rest, res):
+                _check_flags(flags)
+                A, info = out[k]
+                A = A.copy()
+                A[idx] = A_irs
+                fb = info["fallback"].copy()
+                fb[idx] = True
+                info.update(fallback=fb, audit_failed=True, rays=info["rays"] + int(rays.sum()))
+                out[k] = (A, info)
         return out
