@@ -42,29 +42,42 @@ def test_chains_with_surrogate_and_with_ray_shooting():
     assert rs["fallback"] + rs["audited"] > 0
 
 
-def test_seed_union_recovers_an_image_one_split_search_misses():
-    """The first proposal of the first Metropolis chain from the C2 geometry
-    has a position (515 of 2048, its source disk across the caustic) where
-    one split seed search misses the seed of a whole image: filled from that
-    search's seeds it shows 2.58 against the single chain's 19.84.  The
-    surrogate path fills its shot positions from the union of four split
-    searches and matches the chain there."""
+def test_seed_union_never_misses_an_image_of_the_single_chain():
+    """The surrogate path fills its shot positions from the union of four
+    split seed searches (different piece boundaries).  A split search starts
+    some positions' roots afresh, and near a caustic a fresh start can
+    converge without a whole image's seed; the union guards against that.
+    On the first proposals of four Metropolis chains from the C2 geometry
+    (2048 positions each), no shot position has fewer rays than the
+    reference's single chain gives it, and every curve is within the 1e-3
+    bound of the single chain's."""
     from triplelens_amd import engine as E, metropolis as M, surrogate as S, workloads as W
 
     def ca_fn(a, b, th, m2, m3):
         return E.LightCurve(a, b, th, m2, m3, 5e-4, 1, 200).cc_ca()[1]
 
     truth = dataclasses.replace(W.c2(seed=2, ca_fn=ca_fn), n_points=2048)
-    p = M.propose(M.starts_near(truth, 8)[0], np.random.default_rng([0, 0, 1]), 0.01)
     sur = S.Surrogate.load(MODEL)
-    ref = M._new(p)
-    A_ref, rays_ref, _ = ref.lc_irs(*p.traj())
-    one = M._new(p)
-    one.set_seed_split(sur.seed_parts(p.n_points))
-    one.find_seeds(*p.traj())
-    A_one, rays_one, _ = one.fill_positions(np.array([515]))
-    assert rays_one[0] < rays_ref[515] and A_one[0] < 0.5 * A_ref[515]
-    A, info = sur.light_curve(M._new(p), p, np.random.default_rng(7))
-    assert info["fallback"][515]
-    assert abs(A[515] / A_ref[515] - 1.0) < 1e-12
-    assert np.max(np.abs(A / A_ref - 1.0)) <= 1e-3
+    n_shot = 0
+    for k, start in enumerate(M.starts_near(truth, 4)):
+        p = M.propose(start, np.random.default_rng([0, k, 1]), 0.01)
+        A_ref, rays_ref, _ = M._new(p).lc_irs(*p.traj())
+        lc = M._new(p)
+        A, info = sur.light_curve(lc, p, np.random.default_rng(7))
+        shot = np.nonzero(info["fallback"] | info["audited"])[0]
+        n_shot += len(shot)
+        A_shot, rays_shot, _ = lc.fill_positions(shot, _union_seeds(sur, lc, p, shot))
+        assert np.all(rays_shot >= rays_ref[shot]), (k, shot[rays_shot < rays_ref[shot]])
+        assert np.max(np.abs(A / A_ref - 1.0)) <= 1e-3, k
+    assert n_shot > 0
+
+
+def _union_seeds(sur, lc, p, idx):
+    """The merged seeds the surrogate path filled positions idx with."""
+    from triplelens_amd import surrogate as S
+    ns, pix = lc.seeds()
+    sd = (ns[idx], pix[idx])
+    for cl in sur._extra_clones(lc, p):
+        ns2, pix2 = cl.seeds()
+        sd = S.union_seeds(sd[0], sd[1], ns2[idx], pix2[idx])
+    return sd
