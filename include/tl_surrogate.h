@@ -36,6 +36,17 @@ int tl_mlp_set_scaling(void* net, const float* mean, const float* scale);
 /* Inference: x is n rows of 14 features, out n rows of (mu, s). */
 int tl_mlp_predict(void* net, int n, const float* x, float* out);
 
+/* Inference that allocates nothing, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation).  No counterpart in the reference.  x (n x 14 features)
+ * and out (n x (mu, s)) are DEVICE pointers the caller owns; the
+ * activations live in a fixed array in the library, one per GPU, of
+ * 4.4 MiB, held until the process ends.  n is at most 4096 (larger n is
+ * rejected with an error).  Calls are serialised within a process, and the
+ * array keeps the last call's contents.  The result equals tl_mlp_predict's
+ * for the same rows.  Returns 0, or -1 with tl_mlp_last_error(). */
+int tl_mlp_predict_noalloc(void* net, int n, const float* x, float* out);
+
 /* One forward and backward pass over n rows (a multiple of 64): the gradient
  * of the mean loss in the logical parameter order, and the loss. */
 int tl_mlp_grads(void* net, int n, const float* x, const float* y, float* grads, float* loss);

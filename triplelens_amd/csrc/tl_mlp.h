@@ -66,4 +66,22 @@ int launch_pack_input(const float* x, int n, const float* mean, const float* sca
 // FP32 to BF16 (round to nearest even).
 int launch_to_bf16(const float* x, size_t n, uint16_t* y, hipStream_t s);
 
+// ---- inference without allocation ----------------------------------------
+// Largest batch of tl_mlp_predict_noalloc, and the byte layout of its
+// scratch array (tl_mlp.hip): packed input, two hidden layers used in turn,
+// padded output.
+constexpr size_t kNoAllocRows = 4096;
+constexpr size_t kNaH0 = 0;
+constexpr size_t kNaHA = kNaH0 + kNoAllocRows * kMlpInPad * sizeof(uint16_t);
+constexpr size_t kNaHB = kNaHA + kNoAllocRows * kMlpHidden * sizeof(uint16_t);
+constexpr size_t kNaOut = kNaHB + kNoAllocRows * kMlpHidden * sizeof(uint16_t);
+constexpr size_t kNoAllocScratchBytes = kNaOut + kNoAllocRows * kMlpOutPad * sizeof(float);
+static_assert(kNaHA % 256 == 0 && kNaHB % 256 == 0 && kNaOut % 256 == 0, "aligned parts");
+// Device address of the scratch array on the current device (resolved with
+// hipGetSymbolAddress once per device); 0 or a hipError_t.
+int noalloc_scratch(void** out);
+int launch_pack_input_padded(const float* x, int n, int nPad, const float* mean,
+                             const float* scale, uint16_t* xb, hipStream_t s);
+int launch_unpack_output(const float* c, int n, float* out, hipStream_t s);
+
 }  // namespace tl

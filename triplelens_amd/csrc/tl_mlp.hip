@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <mutex>
+
 #include "tl_mlp.h"
 
 namespace tl {
@@ -209,6 +211,66 @@ __global__ void to_bf16_kernel(const float* x, size_t n, uint16_t* y) {
 
 int launch_to_bf16(const float* x, size_t n, uint16_t* y, hipStream_t s) {
   hipLaunchKernelGGL(to_bf16_kernel, dim3(256), dim3(256), 0, s, x, n, y);
+  return (int)hipGetLastError();
+}
+
+// ---- inference without allocation (tl_mlp_predict_noalloc) ---------------
+// The activations of one inference of at most kNoAllocRows rows live in this
+// file-scope array: the packed input, two hidden layers used in turn, and the
+// padded output (byte offsets kNa*).  Every GPU the library runs on holds
+// its own copy until the process ends.  Host code reaches it only through
+// the address noalloc_scratch() resolves per device.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+int noalloc_scratch(void** out) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static void* addr[kMaxDevices] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return (int)e;
+  if (dev < 0 || dev >= kMaxDevices) return (int)hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return (int)e;
+    addr[dev] = p;
+  }
+  *out = addr[dev];
+  return 0;
+}
+
+// Rows [0, n) packed from x as pack_input_kernel does, rows [n, nPad) zero.
+__global__ void pack_input_padded_kernel(const float* x, int n, int nPad, const float* mean,
+                                         const float* scale, uint16_t* xb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nPad) return;
+  for (int f = 0; f < kMlpInPad; f++)
+    xb[(size_t)i * kMlpInPad + f] = (i < n && f < kMlpIn)
+                                        ? f2bf((x[(size_t)i * kMlpIn + f] - mean[f]) / scale[f])
+                                        : (uint16_t)0;
+}
+
+int launch_pack_input_padded(const float* x, int n, int nPad, const float* mean,
+                             const float* scale, uint16_t* xb, hipStream_t s) {
+  if (nPad <= 0) return 0;
+  hipLaunchKernelGGL(pack_input_padded_kernel, dim3((nPad + 255) / 256), dim3(256), 0, s, x, n,
+                     nPad, mean, scale, xb);
+  return (int)hipGetLastError();
+}
+
+// (mu, s) of rows [0, n) from the padded output (n x kMlpOutPad) into n x 2.
+__global__ void unpack_output_kernel(const float* c, int n, float* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[2 * (size_t)i] = c[(size_t)i * kMlpOutPad];
+  out[2 * (size_t)i + 1] = c[(size_t)i * kMlpOutPad + 1];
+}
+
+int launch_unpack_output(const float* c, int n, float* out, hipStream_t s) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(unpack_output_kernel, dim3((n + 255) / 256), dim3(256), 0, s, c, n, out);
   return (int)hipGetLastError();
 }
 

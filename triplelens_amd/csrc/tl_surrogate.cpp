@@ -130,6 +130,37 @@ struct Mlp {
     }
   }
 
+  // forward pass over `rows` packed rows in h0 with caller-given buffers:
+  // the hidden layers alternate between hA and hB, the output goes to out
+  // (no pre-activations kept)
+  void forward_into(int rows, const uint16_t* h0, uint16_t* hA, uint16_t* hB, float* o) {
+    const uint16_t* in = h0;
+    uint16_t* bufs[2] = {hA, hB};
+    for (int l = 0; l < kMlpLayers; l++) {
+      GemmArgs a;
+      std::memset(&a, 0, sizeof(a));
+      a.A = in;
+      a.lda = L.K[l];
+      a.B = pb.p + L.w[l];
+      a.ldb = L.N[l];
+      a.M = rows;
+      a.N = L.N[l];
+      a.K = L.K[l];
+      a.bias = p.p + L.b[l];
+      if (l < kMlpLayers - 1) {
+        a.epi = kEpiGelu;
+        a.H = bufs[l & 1];
+        a.ldh = kMlpHidden;
+        in = a.H;
+      } else {
+        a.epi = kEpiLinear;
+        a.C = o;
+        a.ldc = kMlpOutPad;
+      }
+      MCHK(launch_gemm(a, stream));
+    }
+  }
+
   // backward pass from the loss of targets y; gradients into g
   void backward(int rows) {
     MCHK(hipMemsetAsync(loss.p, 0, sizeof(float), stream));
@@ -232,6 +263,9 @@ void* tl_mlp_new(unsigned long long seed) {
       throw std::runtime_error("no HIP device available: the surrogate needs a gfx950 GPU");
     M = new Mlp();
     MCHK(hipStreamCreateWithFlags(&M->stream, hipStreamNonBlocking));
+    // the allocation-free inference's scratch on this device
+    void* scratchBase = nullptr;
+    MCHK(noalloc_scratch(&scratchBase));
     M->L = mlp_layout();
     const size_t n = M->L.total;
     M->p.ensure(n);
@@ -329,6 +363,36 @@ int tl_mlp_predict(void* h, int n, const float* x, float* out) {
         out[2 * ((size_t)r0 + i) + 1] = o[(size_t)i * kMlpOutPad + 1];
       }
     }
+    return 0;
+  });
+}
+
+int tl_mlp_predict_noalloc(void* h, int n, const float* x, float* out) {
+  // one inference at a time per process: every call uses this GPU's single
+  // scratch array (tl_mlp.hip)
+  static std::mutex noallocMu;
+  return mguard([&] {
+    Mlp* M = checked(h);
+    if (n < 0 || (size_t)n > kNoAllocRows)
+      throw std::invalid_argument("tl_mlp_predict_noalloc: n = " + std::to_string(n) +
+                                  " rows; at most " + std::to_string(kNoAllocRows) +
+                                  " fit its scratch array");
+    if (n > 0 && (!x || !out)) throw std::invalid_argument("bad arguments");
+    std::lock_guard<std::mutex> lock(noallocMu);
+    std::lock_guard<std::mutex> lockNet(M->mu);
+    if (n == 0) return 0;
+    void* base = nullptr;
+    MCHK(noalloc_scratch(&base));
+    unsigned char* b = static_cast<unsigned char*>(base);
+    uint16_t* h0 = reinterpret_cast<uint16_t*>(b + kNaH0);
+    uint16_t* hA = reinterpret_cast<uint16_t*>(b + kNaHA);
+    uint16_t* hB = reinterpret_cast<uint16_t*>(b + kNaHB);
+    float* o = reinterpret_cast<float*>(b + kNaOut);
+    const int padded = ((n + 15) / 16) * 16;
+    MCHK(launch_pack_input_padded(x, n, padded, M->mean.p, M->scale.p, h0, M->stream));
+    M->forward_into(padded, h0, hA, hB, o);
+    MCHK(launch_unpack_output(o, n, out, M->stream));
+    MCHK(hipStreamSynchronize(M->stream));
     return 0;
   });
 }
