@@ -184,8 +184,10 @@ int lcirs_batch_fill_positions(int n, void** lcs, const int* counts, const int* 
  * launches, out[7] total milliseconds of the last run, out[8] summed busy
  * time of the fill's wavefronts (ms), out[9] of that, time in run-table
  * inserts (ms), out[10] idle polls, out[11] wavefronts of the last launch,
- * out[12] scout tasks, out[13] runs found by scouts.  out must hold 14
- * values. */
+ * out[12] scout tasks, out[13] runs found by scouts, out[14] milliseconds
+ * of the solve lists and their polynomials (trial kernels), out[15] of the
+ * disk samples, out[16] solves (centre and caustic-trial positions).  out
+ * must hold 17 values. */
 int tl_last_stats(double* out);
 
 /* Critical curves and caustics of n geometries on the GPU (one lane each):

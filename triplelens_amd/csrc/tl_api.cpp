@@ -226,7 +226,7 @@ struct Options {
 struct Stats {
   double tasks = 0, evals = 0, ins = 0, dup = 0, seedMs = 0, fillMs = 0, fillLaunches = 0,
          totalMs = 0, busyMs = 0, insertMs = 0, idlePolls = 0, waves = 0, scouts = 0,
-         scoutRuns = 0;
+         scoutRuns = 0, trialMs = 0, ringMs = 0, solves = 0;
 };
 
 // One recorded run of the amoeba print-out: row, leftmost and rightmost pixel.
@@ -269,12 +269,14 @@ struct Engine {
   hipStream_t sstream = nullptr;  // seed search (overlaps the fills)
   std::vector<hipEvent_t> seedEvents;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t evTrial0 = nullptr, evTrial1 = nullptr, evRing0 = nullptr, evRing1 = nullptr;
   DBuf<TrigTable> dTrig;
   DBuf<CurveDev> dCurves;
   DBuf<cx> dCaus;
   DBuf<int2> dSeeds;
-  DBuf<int> dPoolBase, dPoolCount;
+  DBuf<int> dPoolBase, dPoolCount, dPoolOwner;
   DBuf<cx> dPoolZeta, dPoolCoef, dSlots;
+  DBuf<cx> dPolyScratch;  // polynomials of the disk samples / point-source positions
   DBuf<double> dPtCausD2;
   DBuf<unsigned long long> dPoolCursor;
   DBuf<int> dNSeeds, dPtCurve, dBatchPts, dLParent, dLPoint;
@@ -287,7 +289,7 @@ struct Engine {
   DBuf<uint32_t> dFlags;
   DBuf<int2> dSeedTasks;
   DBuf<unsigned long long> dHtab, dLCount, dQSeq, dSync, dPtCount;
-  DBuf<unsigned long long> dLSum, dPtSum;  // 128-bit sums: 2 words per label / point
+  DBuf<unsigned long long> dLSum, dPtSum;  // split sums (hi * 2^32 + lo): 2 words each
   DBuf<Task> dQBuf;
   DBuf<RootChain> dChain;
   DBuf<double> dPointLC;
@@ -312,6 +314,7 @@ struct Engine {
     HIPCHK(hipStreamCreateWithFlags(&sstream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&ev0));
     HIPCHK(hipEventCreate(&ev1));
+    for (hipEvent_t* e : {&evTrial0, &evTrial1, &evRing0, &evRing1}) HIPCHK(hipEventCreate(e));
     dTrig.ensure(1);
     HIPCHK(hipMemcpy(dTrig.p, &trig(), sizeof(TrigTable), hipMemcpyHostToDevice));
     ready = true;
@@ -457,6 +460,7 @@ void Engine::run(Job& job) {
   if (anyRing) {
     dRing.ensure((size_t)nPts * kRingSamples);
     dRingCross.ensure(nPts);
+    dPolyScratch.ensure((size_t)nPts * kRingSamples * 11);
   }
   dFlags.ensure(nPts);
   dPtCurve.ensure(nPts);
@@ -506,25 +510,31 @@ void Engine::run(Job& job) {
     dPoolCount.ensure(nPts);
     dPoolCursor.ensure(1);
     size_t cap = std::max<size_t>(dPoolZeta.n, 2 * (size_t)nPts + 1024);
+    HIPCHK(hipEventRecord(evTrial0, sstream));
+    unsigned long long used = 0;
     for (int attempt = 0;; attempt++) {
       if (cap > (size_t)INT32_MAX) throw std::runtime_error("solve pool exceeds 2^31 entries");
       dPoolZeta.ensure(cap);
+      dPoolOwner.ensure(cap);
       dPoolCoef.ensure(cap * 11);
       ta.pool.base = dPoolBase.p;
       ta.pool.count = dPoolCount.p;
       ta.pool.zeta = dPoolZeta.p;
+      ta.pool.owner = dPoolOwner.p;
       ta.pool.coef = dPoolCoef.p;
       ta.pool.cursor = dPoolCursor.p;
       ta.pool.cap = (int)cap;
       HIPCHK((hipError_t)launch_clear_u64(dPoolCursor.p, 1, 0, sstream));
       HIPCHK((hipError_t)launch_trials(ta, sstream));
-      unsigned long long used = 0;
       HIPCHK(hipMemcpyAsync(&used, dPoolCursor.p, sizeof(used), hipMemcpyDeviceToHost, sstream));
       HIPCHK(hipStreamSynchronize(sstream));
       if (used <= cap) break;
       if (attempt >= 2) throw std::runtime_error("solve pool sizing did not converge");
       cap = (size_t)used + 1024;
     }
+    HIPCHK((hipError_t)launch_trial_polys(ta, (int)used, sstream));
+    HIPCHK(hipEventRecord(evTrial1, sstream));
+    stats.solves = (double)used;
     SeedArgs sa;
     sa.curves = dCurves.p;
     sa.nCurves = nc;
@@ -554,9 +564,12 @@ void Engine::run(Job& job) {
     }
     // disk samples of the curves that asked for them, behind the seed search
     // (and beside the fills)
-    if (anyRing)
+    if (anyRing) {
+      HIPCHK(hipEventRecord(evRing0, sstream));
       HIPCHK((hipError_t)launch_ring_amp(dCurves.p, dPtCurve.p, dPtPos.p, dPtRoots.p, nPts,
-                                         dTrig.p, dRing.p, dRingCross.p, sstream));
+                                         dTrig.p, dRing.p, dRingCross.p, dPolyScratch.p, sstream));
+      HIPCHK(hipEventRecord(evRing1, sstream));
+    }
   } else {
     for (int p = 0; p < nPts; p++) {
       nSeeds[p] = std::min(job.nSeeds[p], kMaxSeeds);
@@ -772,8 +785,8 @@ void Engine::run(Job& job) {
                        hipMemcpyDeviceToHost));
       for (int b = 0; b < nb; b++) {
         ptRays[batchPts[b]] = cnt[b];
-        // the signed 128-bit total: high word * 2^64 + unsigned low word
-        ptSumAll[batchPts[b]] = (double)(long long)sum[2 * (size_t)b + 1] * 18446744073709551616.0 +
+        // the split total: signed high word * 2^32 + unsigned low word
+        ptSumAll[batchPts[b]] = (double)(long long)sum[2 * (size_t)b + 1] * 4294967296.0 +
                                 (double)sum[2 * (size_t)b];
       }
       if (wantAmoebae) collect_amoebae(job, ptCurve, batchPts, hcap, labelCap, amoebae);
@@ -784,8 +797,14 @@ void Engine::run(Job& job) {
   if (job.runSeeds) {
     HIPCHK(hipStreamSynchronize(sstream));
     float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, evSeed[nChunks], evSeed[nChunks - 1]));
+    HIPCHK(hipEventElapsedTime(&ms, evTrial1, evSeed[nChunks - 1]));
     stats.seedMs = ms;
+    HIPCHK(hipEventElapsedTime(&ms, evTrial0, evTrial1));
+    stats.trialMs = ms;
+    if (anyRing) {
+      HIPCHK(hipEventElapsedTime(&ms, evRing0, evRing1));
+      stats.ringMs = ms;
+    }
     std::vector<RootChain> chains(nc);
     HIPCHK(hipMemcpy(chains.data(), dChain.p, nc * sizeof(RootChain), hipMemcpyDeviceToHost));
     for (int k = 0; k < nc; k++) job.curves[k]->chain = chains[k];
@@ -886,8 +905,10 @@ void Engine::point_lc(Curve& c, cx s, cx e) {
   dCurves.ensure(1);
   dChain.ensure(1);
   dPointLC.ensure(c.lcLength);
+  dPolyScratch.ensure((size_t)c.lcLength * 11);
   HIPCHK(hipMemcpyAsync(dCurves.p, &d, sizeof(CurveDev), hipMemcpyHostToDevice, stream));
-  HIPCHK((hipError_t)launch_point_lc(dCurves.p, 1, dTrig.p, dPointLC.p, dChain.p, stream));
+  HIPCHK((hipError_t)launch_point_lc(dCurves.p, 1, (int)c.lcLength, dTrig.p, dPointLC.p, dChain.p,
+                                     dPolyScratch.p, stream));
   HIPCHK(hipStreamSynchronize(stream));
   HIPCHK(hipMemcpy(c.lcVec.data(), dPointLC.p, c.lcLength * sizeof(double), hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(&c.chain, dChain.p, sizeof(RootChain), hipMemcpyDeviceToHost));
@@ -1507,7 +1528,8 @@ int tl_last_stats(double* out) {
   out[0] = s.tasks; out[1] = s.evals; out[2] = s.ins; out[3] = s.dup;
   out[4] = s.seedMs; out[5] = s.fillMs; out[6] = s.fillLaunches; out[7] = s.totalMs;
   out[8] = s.busyMs; out[9] = s.insertMs; out[10] = s.idlePolls; out[11] = s.waves;
-  out[12] = s.scouts; out[13] = s.scoutRuns;
+  out[12] = s.scouts; out[13] = s.scoutRuns; out[14] = s.trialMs; out[15] = s.ringMs;
+  out[16] = s.solves;
   return 0;
 }
 

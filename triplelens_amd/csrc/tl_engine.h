@@ -77,6 +77,7 @@ struct SolvePool {
   int* base;          // [point] first solve in the pool
   int* count;         // [point] solves (1 + caustic trials)
   cx* zeta;           // [solve] source position
+  int* owner;         // [solve] the point it belongs to
   cx* coef;           // [solve * 11 + k] coefficient of z^k (image_poly)
   unsigned long long* cursor;  // solves handed out (may exceed cap: then rerun bigger)
   int cap;            // pool capacity in solves
@@ -134,7 +135,7 @@ struct FillArgs {
   int* lparent;
   int* lpoint;
   unsigned long long* lcount;
-  unsigned long long* lsum;   // 2 words per label: 128-bit brightness sum (low, high)
+  unsigned long long* lsum;   // 2 words per label: brightness total hi * 2^32 + lo (add_split)
   int* labelNext;
   int labelCap;
   // shared queue (tasks handed between waves) and termination
@@ -168,6 +169,8 @@ struct FillArgs {
 
 // Host-side launchers (tl_kernels.hip).  All return hipError_t as int.
 int launch_trials(const TrialArgs& a, hipStream_t s);
+// The polynomials of the first nSolves solves of the pool (after launch_trials).
+int launch_trial_polys(const TrialArgs& a, int nSolves, hipStream_t s);
 int launch_seed(const SeedArgs& a, hipStream_t s);
 // image_poly for n source positions of one lens on the GPU (test access):
 // zeta[n] in, coef[11 n] out (device pointers).
@@ -178,9 +181,10 @@ int launch_image_poly(const Lens& L, const cx* fixedPoly, const cx* zeta, int n,
 // number of samples whose count of significant images (at least 1e-4 of the
 // magnification each) differs from the centre's.
 constexpr int kRingSamples = 12;
+// coef: nPts * kRingSamples * 11 values of scratch for the samples' polynomials.
 int launch_ring_amp(const CurveDev* curves, const int* ptCurve, const double2* ptPos,
                     const cx* ptRoots, int nPts, const TrigTable* trig, double* ring, int* cross,
-                    hipStream_t s);
+                    cx* coef, hipStream_t s);
 int launch_ccc(const Lens* lenses, int n, const CccTable* tab, const TrigTable* trig, cx* cc,
                cx* ca, hipStream_t s);
 int launch_fill(const FillArgs& a, int nWorkgroups, hipStream_t s);
@@ -188,13 +192,16 @@ int fill_resident_workgroups();
 int launch_clear_flag_bits(uint32_t* flags, const int* pts, int n, uint32_t bits, hipStream_t s);
 int launch_clear_u64(unsigned long long* p, unsigned long long n, unsigned long long v, hipStream_t s);
 int launch_fill_reset(const FillArgs& a, hipStream_t s);
-// ptSum: 2 words per batch point, the 128-bit brightness sum (low, high).
+// ptSum: 2 words per batch point, the brightness total as hi * 2^32 + lo
+// (lo unsigned, hi signed).
 int launch_resolve(const FillArgs& a, int nLabels, unsigned long long* ptCount,
                    unsigned long long* ptSum, hipStream_t s);
 // Point-source light curves (LightCurveBase::getLC): curve k writes
 // lcLength values at out[curves[k].pointBase ...].
-int launch_point_lc(const CurveDev* curves, int nCurves, const TrigTable* trig, double* out,
-                    RootChain* chainOut, hipStream_t s);
+// nTotal: the largest pointBase + lcLength of the curves; coef: nTotal * 11
+// values of scratch for the positions' polynomials.
+int launch_point_lc(const CurveDev* curves, int nCurves, int nTotal, const TrigTable* trig,
+                    double* out, RootChain* chainOut, cx* coef, hipStream_t s);
 
 // Fixed-point brightness sum: each pixel's A + B sqrt(1 - r^2) is rounded to
 // a multiple of 1/scale and summed as an integer, which makes the sum
@@ -203,7 +210,8 @@ int launch_point_lc(const CurveDev* curves, int nCurves, const TrigTable* trig, 
 // stays within 2 (every linear profile with 0 <= u <= 1, and the default),
 // halved per doubling beyond, so that one pixel never exceeds 2^35 units and
 // one run (at most 2^25 pixels) 2^60.  Wave-local totals are flushed before
-// they pass 2^61; label and point totals are 128-bit.
+// they pass 2^61; label and point totals are kept in two 64-bit words, the
+// low 32 bits of every addition and the rest, so that they do not wrap.
 constexpr double kSumScale = 17179869184.0;  // 2^34
 inline double sum_scale_for(double ldA, double ldB) {
   const double peak = fabs(ldA) + fabs(ldB);

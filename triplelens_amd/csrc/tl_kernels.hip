@@ -123,22 +123,6 @@ __device__ __forceinline__ double rl(double v, int k) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 __device__ __forceinline__ cx rlc(const cx& v, int k) { return mk(rl(v.re, k), rl(v.im, k)); }
-// The image polynomial at zeta with the reference's rounding (tl_coeffs.h),
-// computed by every lane; lane k returns the coefficient of z^k (k <= 10; 0
-// on the other lanes).  Used where one position is solved at a time (the
-// point-source curve, the disk samples); the light curve's solves take their
-// coefficients from trial_kernel's pool instead.
-__device__ __forceinline__ cx wl_coeffs(const Lens& L, const cx* fixedPoly, cx zeta) {
-  cx c[11];
-  image_poly_zeta(L, zeta, c);
-  const int ln = __lane_id();
-  cx mine = mk(0.0, 0.0);
-#pragma unroll
-  for (int k = 0; k < 11; k++)
-    if (ln == k) mine = cadd(fixedPoly[k], c[k]);
-  return mine;
-}
-
 // One part of the quotient n / d as cdiv_sel forms it (part 0: real, 1:
 // imaginary), for this lane's numerator and part: lanes that need the
 // quotients of several numerators over one denominator share one division
@@ -401,17 +385,18 @@ __device__ __forceinline__ void scan_trials(const CurveDev& C, cx pos, const cx*
 // Everything of a light-curve point that does not depend on the root chain,
 // one wavefront per point, all points at once: its source position, the
 // distance to the nearest caustic sample, and the list of positions the
-// chain will solve for (the centre, then the caustic trial points) with the
-// image polynomial of each (tl_coeffs.h: the reference's rounding).  The
+// chain will solve for (the centre, then the caustic trial points).  The
 // list goes to the pool at a place taken with one atomic add; when the pool
 // is too small, the cursor still advances and nothing is written, and the
-// host runs the kernel again with a bigger pool.
+// host runs the kernel again with a bigger pool.  The solves' polynomials
+// are computed afterwards by trial_poly_kernel, one lane per solve (that
+// evaluation needs many registers; kept apart, this kernel stays small and
+// many of its wavefronts share each SIMD).
 __global__ void __launch_bounds__(64) trial_kernel(TrialArgs a) {
   const int p = blockIdx.x;
   if (p >= a.nPts) return;
   const int ln = __lane_id();
   const CurveDev& C = a.curves[a.ptCurve[p]];
-  const Lens L = C.L;
   const unsigned i = (unsigned)(p - C.pointBase);
   const cx pos = traj_pos(mk(C.sx, C.sy), mk(C.ex, C.ey), i, C.lcLength);
   const cx* caus = a.caustics + C.causticBase;
@@ -439,29 +424,41 @@ __global__ void __launch_bounds__(64) trial_kernel(TrialArgs a) {
     a.pool.count[p] = n;
   }
   if ((long long)base + n > a.pool.cap) return;
-  // the centre on lane 0, then each step's trial points on their own lanes
   if (ln == 0) {
     a.pool.zeta[base] = pos;
-    cx c[11];
-    image_poly(L, C.fixedPoly, pos, c);
-    for (int k = 0; k < 11; k++) a.pool.coef[(size_t)base * 11 + k] = c[k];
+    a.pool.owner[base] = p;
   }
   int next = base + 1;
   scan_trials(C, pos, caus, [&](unsigned long long f, const cx& trial) {
     if ((f >> ln) & 1ull) {
       const int sid = next + __builtin_popcountll(f & ((1ull << ln) - 1ull));
       a.pool.zeta[sid] = trial;
-      cx c[11];
-      image_poly(L, C.fixedPoly, trial, c);
-      for (int k = 0; k < 11; k++) a.pool.coef[(size_t)sid * 11 + k] = c[k];
+      a.pool.owner[sid] = p;
     }
     next += __builtin_popcountll(f);
   });
 }
 
+// The image polynomial of every solve in the pool (tl_coeffs.h: the
+// reference's rounding), one lane per solve.
+__global__ void __launch_bounds__(64) trial_poly_kernel(TrialArgs a, int nSolves) {
+  const int sid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sid >= nSolves) return;
+  const CurveDev& C = a.curves[a.ptCurve[a.pool.owner[sid]]];
+  cx c[11];
+  image_poly(C.L, C.fixedPoly, a.pool.zeta[sid], c);
+  for (int k = 0; k < 11; k++) a.pool.coef[(size_t)sid * 11 + k] = c[k];
+}
+
 int launch_trials(const TrialArgs& a, hipStream_t s) {
   if (a.nPts <= 0) return 0;
   hipLaunchKernelGGL(trial_kernel, dim3(a.nPts), dim3(64), 0, s, a);
+  return (int)hipGetLastError();
+}
+
+int launch_trial_polys(const TrialArgs& a, int nSolves, hipStream_t s) {
+  if (nSolves <= 0) return 0;
+  hipLaunchKernelGGL(trial_poly_kernel, dim3((nSolves + 63) / 64), dim3(64), 0, s, a, nSolves);
   return (int)hipGetLastError();
 }
 
@@ -591,9 +588,27 @@ __global__ void __launch_bounds__(64) seed_kernel(SeedArgs a) {
 // t = i/(lcLength-1), magnification sum over images (in root order) of
 // 1/|1 - |sum m/(z-z_k)^2|^2| (the squares as products; the reference uses
 // std::pow(z, 2.0)).  One wavefront per curve, as the seed search.
+// The image polynomial at every position of the point-source curves (one
+// lane per position), for point_lc_kernel: coef[(pointBase + i) * 11 + k].
+__global__ void __launch_bounds__(64) point_poly_kernel(const CurveDev* curves, int nCurves,
+                                                        int nTotal, cx* coef) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nTotal) return;
+  int c = 0;
+  while (c + 1 < nCurves && curves[c + 1].pointBase <= g) c++;
+  const CurveDev& C = curves[c];
+  const int i = g - C.pointBase;
+  if (i < 0 || i >= (int)C.lcLength) return;
+  const cx s = mk(C.sx, C.sy), e = mk(C.ex, C.ey);
+  const cx pos = cadd(s, cscale(csub(e, s), i / (C.lcLength - 1.0)));
+  cx cf[11];
+  image_poly(C.L, C.fixedPoly, pos, cf);
+  for (int k = 0; k < 11; k++) coef[(size_t)g * 11 + k] = cf[k];
+}
+
 __global__ void __launch_bounds__(64) point_lc_kernel(const CurveDev* curves, int nCurves,
                                                       const TrigTable* trig, double* out,
-                                                      RootChain* chainOut) {
+                                                      RootChain* chainOut, const cx* coef) {
   const int c = blockIdx.x;
   if (c >= nCurves) return;
   const int ln = __lane_id();
@@ -604,7 +619,8 @@ __global__ void __launch_bounds__(64) point_lc_kernel(const CurveDev* curves, in
   const cx s = mk(C.sx, C.sy), e = mk(C.ex, C.ey);
   for (unsigned i = 0; i < C.lcLength; i++) {
     const cx pos = cadd(s, cscale(csub(e, s), i / (C.lcLength - 1.0)));
-    const cx cv = wl_coeffs(L, C.fixedPoly, pos);
+    const size_t g = (size_t)C.pointBase + i;
+    const cx cv = ln <= 10 ? coef[g * 11 + (ln <= 10 ? ln : 0)] : mk(0.0, 0.0);
     bool oob = false;
     unsigned long long m = wl_get_images(L, cv, chainv, have, pos, trig, C.binImg, oob);
     const double term = ((m >> ln) & 1ull) ? point_amp_term(L, chainv) : 0.0;
@@ -623,11 +639,13 @@ __global__ void __launch_bounds__(64) point_lc_kernel(const CurveDev* curves, in
   }
 }
 
-int launch_point_lc(const CurveDev* curves, int nCurves, const TrigTable* trig, double* out,
-                    RootChain* chainOut, hipStream_t s) {
-  if (nCurves <= 0) return 0;
+int launch_point_lc(const CurveDev* curves, int nCurves, int nTotal, const TrigTable* trig,
+                    double* out, RootChain* chainOut, cx* coef, hipStream_t s) {
+  if (nCurves <= 0 || nTotal <= 0) return 0;
+  hipLaunchKernelGGL(point_poly_kernel, dim3((nTotal + 63) / 64), dim3(64), 0, s, curves, nCurves,
+                     nTotal, coef);
   hipLaunchKernelGGL(point_lc_kernel, dim3(nCurves), dim3(64), 0, s, curves, nCurves, trig, out,
-                     chainOut);
+                     chainOut, coef);
   return (int)hipGetLastError();
 }
 
@@ -652,10 +670,26 @@ __constant__ double kRingOff[kRingSamples][2] = {
 // the threshold keeps out the faint images next to a lens whose lens-equation
 // residual sits at the acceptance threshold and that come and go between
 // nearby points, while an image pair born on a fold is always bright.
+// The image polynomial at every disk sample (one lane per sample) for
+// ring_amp_kernel: coef[((p * kRingSamples) + s) * 11 + k].
+__global__ void __launch_bounds__(64) ring_poly_kernel(const CurveDev* curves, const int* ptCurve,
+                                                       const double2* ptPos, int nPts, cx* coef) {
+  const long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (g >= (long long)nPts * kRingSamples) return;
+  const int p = (int)(g / kRingSamples), smp = (int)(g % kRingSamples);
+  const CurveDev& C = curves[ptCurve[p]];
+  if (!C.ring) return;
+  const double2 pp = ptPos[p];
+  const cx zeta = mk(pp.x + C.rho * kRingOff[smp][0], pp.y + C.rho * kRingOff[smp][1]);
+  cx cf[11];
+  image_poly(C.L, C.fixedPoly, zeta, cf);
+  for (int k = 0; k < 11; k++) coef[(size_t)g * 11 + k] = cf[k];
+}
+
 __global__ void __launch_bounds__(64) ring_amp_kernel(const CurveDev* curves, const int* ptCurve,
                                                       const double2* ptPos, const cx* ptRoots,
                                                       int nPts, const TrigTable* trig,
-                                                      double* ring, int* cross) {
+                                                      double* ring, int* cross, const cx* coef) {
   const int p = blockIdx.x;
   if (p >= nPts) return;
   const CurveDev& C = curves[ptCurve[p]];
@@ -673,7 +707,8 @@ __global__ void __launch_bounds__(64) ring_amp_kernel(const CurveDev* curves, co
     cx chainv = centre;
     int have = 1;
     const cx zeta = mk(pp.x + C.rho * kRingOff[s][0], pp.y + C.rho * kRingOff[s][1]);
-    const cx cv = wl_coeffs(L, C.fixedPoly, zeta);
+    const size_t g = (size_t)p * kRingSamples + s;
+    const cx cv = ln <= 10 ? coef[g * 11 + (ln <= 10 ? ln : 0)] : mk(0.0, 0.0);
     bool oob = false;
     unsigned long long m = wl_get_images(L, cv, chainv, have, zeta, trig, C.binImg, oob);
     const bool img = (m >> ln) & 1ull;
@@ -692,10 +727,13 @@ __global__ void __launch_bounds__(64) ring_amp_kernel(const CurveDev* curves, co
 
 int launch_ring_amp(const CurveDev* curves, const int* ptCurve, const double2* ptPos,
                     const cx* ptRoots, int nPts, const TrigTable* trig, double* ring, int* cross,
-                    hipStream_t s) {
+                    cx* coef, hipStream_t s) {
   if (nPts <= 0) return 0;
+  const long long lanes = (long long)nPts * kRingSamples;
+  hipLaunchKernelGGL(ring_poly_kernel, dim3((unsigned)((lanes + 63) / 64)), dim3(64), 0, s, curves,
+                     ptCurve, ptPos, nPts, coef);
   hipLaunchKernelGGL(ring_amp_kernel, dim3(nPts), dim3(64), 0, s, curves, ptCurve, ptPos, ptRoots,
-                     nPts, trig, ring, cross);
+                     nPts, trig, ring, cross, coef);
   return (int)hipGetLastError();
 }
 
@@ -1079,20 +1117,22 @@ __device__ __forceinline__ Task ring_pop(TaskState& S) {
   return S.ring[(S.lhead + S.lcount) & (kLocalCap - 1)];
 }
 
-// Adds the signed value v to the 128-bit integer (p[0] low, p[1] high).
-// Each fetch-add on the low word reports its own carry, so concurrent adds
-// sum exactly.
-__device__ __forceinline__ void add128(unsigned long long* p, long long v) {
-  const unsigned long long u = (unsigned long long)v;
-  const unsigned long long old = atomicAdd(p, u);
-  const unsigned long long hi = (v < 0 ? ~0ull : 0ull) + (old + u < old ? 1ull : 0ull);
-  if (hi) atomicAdd(p + 1, hi);
+// Adds the signed value v to a split total (p[0], p[1]) that stands for
+// p[1] * 2^32 + p[0]: p[0] sums the low 32 bits of every value added (as an
+// unsigned number) and p[1] the rest (v >> 32, arithmetic).  Both adds need
+// no result, so they are issued and not waited for, and any number of
+// concurrent adds sums exactly as long as fewer than 2^32 values are added
+// to one total.
+__device__ __forceinline__ void add_split(unsigned long long* p, long long v) {
+  atomicAdd(p, (unsigned long long)(v & 0xFFFFFFFFll));
+  const long long hi = v >> 32;
+  if (hi) atomicAdd(p + 1, (unsigned long long)hi);
 }
 
 __device__ __forceinline__ void flush_acc(TaskState& S) {
   if (S.accCnt && __lane_id() == 0) {
     atomicAdd(&S.A->lcount[S.accLabel], S.accCnt);
-    add128(&S.A->lsum[2 * (size_t)S.accLabel], S.accSum);
+    add_split(&S.A->lsum[2 * (size_t)S.accLabel], S.accSum);
   }
   S.accCnt = 0;
   S.accSum = 0;
@@ -1393,10 +1433,10 @@ __device__ __forceinline__ void scan_window(const RowCtx& R, int s, int e, bool 
 // row each: the task's row (one interval, or two around a known run), then
 // each look-back (the same).  The follow-up kept for this wave (if any) is
 // left in S.cont / S.haveCont.
-__device__ __forceinline__ void process_task(TaskState& S) {
+// The task's fields as wave-uniform values, and the constants of its source
+// position (kept while the wave stays on that position).
+__device__ __forceinline__ void load_task(TaskState& S) {
   const FillArgs& A = *S.A;
-  S.haveCont = false;
-  S.nBack = 0;
   S.t.b = uni(S.t.b);
   S.t.row = uni(S.t.row);
   S.t.xa = uni(S.t.xa);
@@ -1406,7 +1446,6 @@ __device__ __forceinline__ void process_task(TaskState& S) {
   S.t.f1 = uni(S.t.f1);
   S.t.f2 = uni(S.t.f2);
   S.kind = S.t.kd & 3;
-  S.nScout += S.kind == 2 ? 1 : 0;
   S.dir = (S.t.kd & 4) ? -1 : 1;
   if (S.t.b != S.curB) {
     // constants of this source position, kept while the wave stays on it
@@ -1432,6 +1471,14 @@ __device__ __forceinline__ void process_task(TaskState& S) {
     S.R.px = unid(pos.x);
     S.R.py = unid(pos.y);
   }
+}
+
+__device__ __forceinline__ void process_task(TaskState& S) {
+  const FillArgs& A = *S.A;
+  S.haveCont = false;
+  S.nBack = 0;
+  load_task(S);
+  S.nScout += S.kind == 2 ? 1 : 0;
   if (S.t.row < 0 || S.t.row >= S.N) return;
   if (S.kind == 1 && (S.t.xa < 0 || S.t.xa > S.N - 1)) {  // seed column outside the grid
     flag_point(A, S.gp, kFlagEdge);
@@ -1729,12 +1776,12 @@ __global__ void resolve_kernel(FillArgs A, int nLabels, unsigned long long* ptCo
   const int rb = find_root(A.lparent, b);
   if (r != rb) return;
   atomicAdd(&ptCount[b], A.lcount[L]);
-  // the label's 128-bit total into the point's: low word with its carry,
-  // then the high word
+  // the label's split total into the point's, word by word (the point's
+  // total is hi * 2^32 + lo like the labels'; its low word may carry past
+  // 2^64 only after 2^32 labels)
   const unsigned long long lo = A.lsum[2 * (size_t)L], hi = A.lsum[2 * (size_t)L + 1];
-  const unsigned long long old = atomicAdd(&ptSum[2 * (size_t)b], lo);
-  const unsigned long long carry = old + lo < old ? 1ull : 0ull;
-  if (hi + carry) atomicAdd(&ptSum[2 * (size_t)b + 1], hi + carry);
+  atomicAdd(&ptSum[2 * (size_t)b], lo);
+  if (hi) atomicAdd(&ptSum[2 * (size_t)b + 1], hi);
 }
 
 int launch_resolve(const FillArgs& a, int nLabels, unsigned long long* ptCount,
