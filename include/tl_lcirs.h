@@ -231,8 +231,7 @@ int tl_math_selftest(int n, unsigned long long seed);
 /* Engine settings: name/value pairs ("fill_workgroups", "probe_limit",
  * "queue_log2", "scout_rows", "scout_jump", "scout_retry", "scout_width",
  * "scout_share", "share_rays", "use_graph", "leap_every", "leap_jumps", "poll_backoff",
- * "share_idle", "run_pixels",
- * "table_load", "seed_chunks",
+ * "run_pixels", "table_load", "seed_chunks",
  * "seed_chunk_min", "time_limit_s").  None changes a result.  Returns 0 or
  * -1 for an unknown name.  No counterpart in the reference. */
 int tl_set_option(const char* name, double value);

@@ -202,7 +202,7 @@ struct DBuf {
 };
 
 struct Options {
-  int fillWorkgroups = 0;  // 0: one per compute unit (4 waves), capped at the occupancy limit
+  int fillWorkgroups = 0;  // 0: one per compute unit (16 waves), capped at the occupancy limit
   int probeLimit = 4096;
   int queueLog2 = 21;
   int scoutRows = 8;
@@ -215,7 +215,6 @@ struct Options {
   int leapEvery = 4;        // every this many scouts a chain sends a leap scout (0: never)
   int leapJumps = 8;        // ... this many jumps ahead
   int pollBackoff = 6;
-  int shareIdle = 0;
   double runPixels = 128.0;  // predicted rays per run: sizes the run table
   double tableLoad = 4.0;   // run-table slots per predicted run
   double timeLimitS = 60.0;
@@ -595,12 +594,16 @@ void Engine::run(Job& job) {
   // that poll one of them do not slow the atomics on the others
   constexpr int kLine = 16;  // u64 per 128 bytes
   dSync.ensure(7 * kLine);
-  // every wave of the fill must be resident at once (its end test counts
-  // them all), so the launch is capped at the occupancy limit
+  // one fill workgroup per compute unit (it fills the unit's registers),
+  // capped at the occupancy limit; while the seed search runs beside the
+  // fills, one unit is left to its wavefront (a group that cannot start
+  // would only start, find nothing and leave once the seed search is done,
+  // but the launch would last until then)
   const int resident = fill_resident_workgroups();
   if (resident <= 0) throw std::runtime_error("fill kernel occupancy query failed");
   int nWg = opt.fillWorkgroups > 0 ? opt.fillWorkgroups : cus;
   if (nWg > resident) nWg = resident;
+  if (job.runSeeds && nChunks > 1 && opt.fillWorkgroups <= 0 && nWg > 1) nWg--;
 
   std::vector<int> order;
   for (int ch = 0; ch < nChunks; ch++) {
@@ -686,7 +689,7 @@ void Engine::run(Job& job) {
       fa.qtail = dSync.p + 1 * kLine;
       fa.qcap = qcap;
       fa.active = reinterpret_cast<int*>(dSync.p + 2 * kLine);
-      fa.nWaves = nWg * (kFillThreads / 64);
+      fa.nGroups = nWg;
       fa.seedNext = reinterpret_cast<int*>(dSync.p + 3 * kLine);
       fa.flags = dFlags.p;
       fa.abortFlag = reinterpret_cast<int*>(dSync.p + 5 * kLine);
@@ -702,7 +705,6 @@ void Engine::run(Job& job) {
       fa.leapEvery = opt.leapEvery;
       fa.leapJumps = opt.leapJumps;
       fa.pollBackoff = opt.pollBackoff;
-      fa.shareIdle = opt.shareIdle;
       if (opt.useGraph) {
         // the batch's reset, clears, fill and resolve as one graph (kernel
         // nodes only: the resets are kernels, not memsets); the events
@@ -1551,7 +1553,6 @@ int tl_set_option(const char* name, double value) {
   else if (n == "share_rays") o.shareRays = std::max(0.0, value);
   else if (n == "leap_jumps") o.leapJumps = std::min(64, std::max(2, (int)value));
   else if (n == "poll_backoff") o.pollBackoff = std::min(6, std::max(0, (int)value));
-  else if (n == "share_idle") o.shareIdle = value != 0.0 ? 1 : 0;
   else if (n == "run_pixels") o.runPixels = std::min(65536.0, std::max(1.0, value));
   else if (n == "table_load") o.tableLoad = std::min(64.0, std::max(1.5, value));
   else if (n == "time_limit_s") o.timeLimitS = std::max(0.1, value);

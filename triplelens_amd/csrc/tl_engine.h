@@ -12,8 +12,11 @@ constexpr int kCccLen = 500;       // critical-curve samples per branch (lcirs.c
 constexpr int kMaxSeeds = 64;      // seeds kept per source position
 constexpr int kMaxBatch = 16384;   // source positions per fill launch (14-bit key field)
 constexpr int kMaxCover = 16;      // known runs read per row lookup
-constexpr int kFillThreads = 256;  // threads per fill workgroup (4 waves)
-constexpr int kLocalCap = 64;      // tasks in each wave's own ring (LDS)
+constexpr int kFillWaves = 16;     // waves per fill workgroup: 1 manager + 15 workers, four
+                                   // waves per SIMD
+constexpr int kFillThreads = 64 * kFillWaves;
+constexpr int kLocalCap = 64;      // tasks in each worker's own ring (LDS)
+constexpr int kPoolCap = 512;      // tasks in the workgroup's shared pool (LDS)
 
 // Per-point status bits (OR-ed into flags[p]).
 enum : uint32_t {
@@ -144,8 +147,9 @@ struct FillArgs {
   unsigned long long* qhead;
   unsigned long long* qtail;
   unsigned long long qcap;
-  int* active;                // waves holding work (the fill ends when 0 and no queued task)
-  int nWaves;                 // waves of the launch
+  int* active;                // started workgroups holding work (the fill ends when 0 and no
+                              // task is queued)
+  int nGroups;                // workgroups of the launch
   int* seedNext;
   // results and status
   uint32_t* flags;            // per global point
@@ -158,13 +162,13 @@ struct FillArgs {
   int scoutJump;              // rows a scout jumps ahead
   int scoutWidth;             // runs wider than this shorten the jump (0: never)
   int scoutRetry;             // a scout that finds nothing is sent again half as far
-  int scoutShare;             // sweeps queue their scouts in the shared queue at once
+  int scoutShare;             // sweeps hand their scouts to the workgroup's pool at once
   const float* ptCost;        // predicted rays per position
-  float shareRays;            // ... from which its sweeps' scouts are queued at once (0: never)
+  float shareRays;            // ... from which its sweeps' scouts go to the pool at once (0: never)
   int leapEvery;              // every this many scouts a chain sends a leap scout (0: never)
   int leapJumps;              // ... this many jumps ahead
-  int pollBackoff;            // idle polls sleep up to 2^pollBackoff x 1024 cycles
-  int shareIdle;              // while some wave is idle, queue follow-ups in the shared ring
+  int pollBackoff;            // idle polls of the global queue sleep up to 2^pollBackoff x 1024
+                              // cycles
 };
 
 // Host-side launchers (tl_kernels.hip).  All return hipError_t as int.

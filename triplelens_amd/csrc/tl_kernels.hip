@@ -885,10 +885,11 @@ __device__ __forceinline__ void unite(const FillArgs& A, int a, int b) {
 }
 
 // ---------------------------------------------------------------------------
-// Task queue: bounded multi-producer multi-consumer ring with per-slot
-// sequence numbers.  The payload is complete before the sequence number that
-// publishes it is written (mem_order between them), and a slot is handed back
-// only after its payload has been read.  Lane 0 operates.
+// Global task ring (tasks handed between workgroups): bounded
+// multi-producer multi-consumer ring with per-slot sequence numbers.  The
+// payload is complete before the sequence number that publishes it is
+// written (mem_order between them), and a slot is handed back only after its
+// payload has been read.
 // ---------------------------------------------------------------------------
 // A task as four 64-bit words (two ints each, low half first), and back:
 // built from the fields, so no task is copied through memory.
@@ -940,45 +941,6 @@ __device__ __forceinline__ bool q_push(const FillArgs& A, const Task& t) {
     }
   }
   return bcast0(ok) != 0;
-}
-
-__device__ __forceinline__ bool q_pop(const FillArgs& A, Task& t) {
-  int ok = 0;
-  unsigned long long w[4] = {0, 0, 0, 0};
-  if (__lane_id() == 0) {
-    unsigned long long pos = ald(A.qhead);
-    if (pos < ald(A.qtail)) {
-      for (int it = 0; it < 64; it++) {
-        const unsigned long long idx = pos & (A.qcap - 1);
-        const unsigned long long seq = ald(&A.qseq[idx]);
-        const long long dif = (long long)(seq - (pos + 1));
-        if (dif == 0) {
-          if (acas(A.qhead, &pos, pos + 1)) {
-            unsigned long long* src = reinterpret_cast<unsigned long long*>(&A.qbuf[idx]);
-            w[0] = ald(&src[0]);
-            w[1] = ald(&src[1]);
-            w[2] = ald(&src[2]);
-            w[3] = ald(&src[3]);
-            mem_order();
-            ast(&A.qseq[idx], pos + A.qcap);
-            ok = 1;
-            break;
-          }
-        } else if (dif < 0) {
-          break;  // empty, or the slot is still being written
-        } else {
-          pos = ald(A.qhead);
-        }
-      }
-    }
-  }
-  ok = bcast0(ok);
-  w[0] = bcast0(w[0]);
-  w[1] = bcast0(w[1]);
-  w[2] = bcast0(w[2]);
-  w[3] = bcast0(w[3]);
-  if (ok) t = words_task(w);
-  return ok != 0;
 }
 
 __device__ __forceinline__ void flag_point(const FillArgs& A, int gp, uint32_t f) {
@@ -1064,6 +1026,279 @@ struct LookBack {
 };
 constexpr int kMaxLookBacks = 2;  // kept in two named slots (no indexed array)
 
+// ---------------------------------------------------------------------------
+// Workgroup task pool (LDS)
+// ---------------------------------------------------------------------------
+// A fill workgroup has one manager wave and kWorkers worker waves.  The
+// workers share a pool of waiting tasks in LDS, guarded by a lock word that
+// lane 0 of a wave takes.  `pending` counts the pool's tasks plus the
+// workers that hold work (a task in hand or in their own ring); it changes
+// only under the lock, so when it reads 0 no worker of the group holds work
+// or can make any, and only the manager brings tasks in (seed tasks, or
+// tasks that other groups queued in the global ring).  LDS operations of one
+// wave are performed in program order; the unlock waits for the wave's LDS
+// operations before it releases the word.
+constexpr int kWorkers = kFillWaves - 1;
+constexpr int kFeedMax = 16;     // tasks the manager brings in at a time
+constexpr int kDonateMax = 8;    // tasks the manager queues for other groups at a time
+constexpr int kPoolShare = 16;   // while other groups are idle, workers keep the pool at
+                                 // this many tasks
+
+struct GroupPool {
+  int lock;
+  int pending;     // pool tasks + workers holding work
+  int head, count; // the pool ring: count tasks from head on
+  int done;        // set by the manager when the fill is over
+  int hungry;      // set by the manager while other groups are idle: workers then
+                   // move waiting tasks to the pool, and the manager on to the
+                   // global ring
+  int pad[2];
+  Task slot[kPoolCap];
+};
+
+#define WGS __HIP_MEMORY_SCOPE_WORKGROUP
+__device__ __forceinline__ int lds_ld(int* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, WGS); }
+__device__ __forceinline__ void lds_st(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, WGS);
+}
+// Waits until the wave's LDS (and scalar) memory operations are done.
+__device__ __forceinline__ void lds_wait() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0), vmcnt and expcnt not waited for
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+// Lane 0 takes and releases the pool lock.
+__device__ __forceinline__ void pool_lock(GroupPool* G) {
+  while (true) {
+    int expected = 0;
+    if (__hip_atomic_compare_exchange_strong(&G->lock, &expected, 1, __ATOMIC_RELAXED,
+                                             __ATOMIC_RELAXED, WGS))
+      break;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+__device__ __forceinline__ void pool_unlock(GroupPool* G) {
+  lds_wait();
+  lds_st(&G->lock, 0);
+}
+
+__device__ __forceinline__ Task uni_task(const Task& t) {
+  Task u;
+  u.b = uni(t.b); u.row = uni(t.row); u.xa = uni(t.xa); u.xb = uni(t.xb);
+  u.label = uni(t.label); u.kd = uni(t.kd); u.f1 = uni(t.f1); u.f2 = uni(t.f2);
+  return u;
+}
+
+// A worker that holds work adds the (wave-uniform) task t; false when the
+// pool is full.
+__device__ __forceinline__ bool pool_push(GroupPool* G, const Task& t) {
+  int ok = 0;
+  if (__lane_id() == 0) {
+    pool_lock(G);
+    const int c = lds_ld(&G->count);
+    if (c < kPoolCap) {
+      G->slot[(lds_ld(&G->head) + c) & (kPoolCap - 1)] = t;
+      lds_st(&G->count, c + 1);
+      lds_st(&G->pending, lds_ld(&G->pending) + 1);
+      ok = 1;
+    }
+    pool_unlock(G);
+  }
+  return uni(ok) != 0;
+}
+
+// A worker takes the oldest task of the pool.  busy: the worker holds work
+// now (its last task is done and its ring is empty): it takes a task
+// (pending - 1) or, with the pool empty, gives up its count (pending - 1)
+// and is idle from then on.  !busy: an idle worker takes a task and is
+// counted as holding work (pending unchanged), or stays idle.
+__device__ __forceinline__ bool pool_take(GroupPool* G, Task& t, bool busy) {
+  int ok = 0;
+  Task got = t;
+  if (__lane_id() == 0) {
+    pool_lock(G);
+    const int c = lds_ld(&G->count);
+    if (c > 0) {
+      const int h = lds_ld(&G->head);
+      got = G->slot[h];
+      lds_st(&G->head, (h + 1) & (kPoolCap - 1));
+      lds_st(&G->count, c - 1);
+      ok = 1;
+    }
+    if (busy) lds_st(&G->pending, lds_ld(&G->pending) - 1);
+    pool_unlock(G);
+  }
+  ok = uni(ok);
+  if (ok) t = uni_task(got);
+  return ok != 0;
+}
+
+// Manager: adds the tasks held by lanes [lo, hi) (pending + their number).
+// Returns how many fitted; the caller makes room first.
+__device__ __forceinline__ int pool_add_lanes(GroupPool* G, const Task& t, int lo, int hi) {
+  const int ln = __lane_id();
+  int base = 0, fit = 0;
+  if (ln == 0) {
+    pool_lock(G);
+    const int c = lds_ld(&G->count);
+    fit = hi - lo < kPoolCap - c ? hi - lo : kPoolCap - c;
+    base = lds_ld(&G->head) + c;
+  }
+  fit = uni(fit);
+  base = uni(base);
+  if (ln >= lo && ln < lo + fit) G->slot[(base + ln - lo) & (kPoolCap - 1)] = t;
+  if (ln == 0) {
+    lds_st(&G->count, lds_ld(&G->count) + fit);
+    lds_st(&G->pending, lds_ld(&G->pending) + fit);
+    pool_unlock(G);
+  }
+  return fit;
+}
+
+// Manager: takes up to n of the pool's oldest tasks into lanes [0, k)
+// (pending - k) and returns k.
+__device__ __forceinline__ int pool_remove_lanes(GroupPool* G, Task& t, int n) {
+  const int ln = __lane_id();
+  int k = 0, h = 0;
+  if (ln == 0) {
+    pool_lock(G);
+    const int c = lds_ld(&G->count);
+    k = n < c ? n : c;
+    h = lds_ld(&G->head);
+  }
+  k = uni(k);
+  h = uni(h);
+  if (ln < k) t = G->slot[(h + ln) & (kPoolCap - 1)];
+  if (ln == 0) {
+    lds_st(&G->head, (h + k) & (kPoolCap - 1));
+    lds_st(&G->count, lds_ld(&G->count) - k);
+    lds_st(&G->pending, lds_ld(&G->pending) - k);
+    pool_unlock(G);
+  }
+  return k;
+}
+
+// Global ring, many tasks per call (manager): lanes [0, n) offer one task
+// each; the leading k slots that are free at the tail are claimed with one
+// compare-and-swap and written in parallel.  Returns k (the tasks of lanes
+// [0, k) are queued).
+__device__ __forceinline__ int q_push_lanes(const FillArgs& A, const Task& t, int n) {
+  const int ln = __lane_id();
+  unsigned long long pos = 0;
+  int k = 0;
+  for (int it = 0; it < 64; it++) {
+    if (ln == 0) pos = ald(A.qtail);
+    pos = uni64(pos);
+    const bool fr = ln < n && ald(&A.qseq[(pos + ln) & (A.qcap - 1)]) == pos + ln;
+    const unsigned long long m = __ballot(fr);
+    k = ~m == 0 ? 64 : __builtin_ctzll(~m);
+    if (k == 0) return 0;  // full, or another producer moved the tail: try again later
+    int ok = 0;
+    if (ln == 0) {
+      unsigned long long exp = pos;
+      ok = acas(A.qtail, &exp, pos + k);
+    }
+    if (uni(ok)) break;
+    k = 0;
+  }
+  if (ln < k) {
+    const unsigned long long idx = (pos + ln) & (A.qcap - 1);
+    unsigned long long w[4];
+    task_words(t, w);
+    unsigned long long* dst = reinterpret_cast<unsigned long long*>(&A.qbuf[idx]);
+    ast(&dst[0], w[0]);
+    ast(&dst[1], w[1]);
+    ast(&dst[2], w[2]);
+    ast(&dst[3], w[3]);
+    mem_order();
+    ast(&A.qseq[idx], pos + ln + 1);
+  }
+  return k;
+}
+
+// Global ring, many tasks per call (manager): takes the leading k <= n
+// ready slots at the head into lanes [0, k); returns k.
+__device__ __forceinline__ int q_pop_lanes(const FillArgs& A, Task& t, int n) {
+  const int ln = __lane_id();
+  unsigned long long pos = 0;
+  int k = 0;
+  for (int it = 0; it < 64; it++) {
+    if (ln == 0) pos = ald(A.qhead);
+    pos = uni64(pos);
+    const bool rd = ln < n && ald(&A.qseq[(pos + ln) & (A.qcap - 1)]) == pos + ln + 1;
+    const unsigned long long m = __ballot(rd);
+    k = ~m == 0 ? 64 : __builtin_ctzll(~m);
+    if (k == 0) return 0;
+    int ok = 0;
+    if (ln == 0) {
+      unsigned long long exp = pos;
+      ok = acas(A.qhead, &exp, pos + k);
+    }
+    if (uni(ok)) break;
+    k = 0;
+  }
+  if (ln < k) {
+    const unsigned long long idx = (pos + ln) & (A.qcap - 1);
+    unsigned long long w[4];
+    const unsigned long long* src = reinterpret_cast<unsigned long long*>(&A.qbuf[idx]);
+    w[0] = ald(const_cast<unsigned long long*>(&src[0]));
+    w[1] = ald(const_cast<unsigned long long*>(&src[1]));
+    w[2] = ald(const_cast<unsigned long long*>(&src[2]));
+    w[3] = ald(const_cast<unsigned long long*>(&src[3]));
+    mem_order();
+    ast(&A.qseq[idx], pos + ln + A.qcap);
+    t = words_task(w);
+  }
+  return k;
+}
+
+// Manager: puts the tasks of lanes [lo, hi) back into the pool, or, where it
+// is full, into the global ring; waits while both are full (workers keep
+// draining the pool), up to the deadline (s_memrealtime ticks).
+__device__ __forceinline__ void pool_add_all(const FillArgs& A, GroupPool* G, const Task& t,
+                                             int lo, int hi, long long deadline) {
+  for (int it = 0; lo < hi; it++) {
+    lo += pool_add_lanes(G, t, lo, hi);
+    if (lo >= hi) break;
+    // lanes [lo, hi) still hold tasks: offer them to the ring from lane 0 on
+    Task u = t;
+    const int src = __lane_id() + lo;
+    u.b = __shfl(t.b, src, 64); u.row = __shfl(t.row, src, 64);
+    u.xa = __shfl(t.xa, src, 64); u.xb = __shfl(t.xb, src, 64);
+    u.label = __shfl(t.label, src, 64); u.kd = __shfl(t.kd, src, 64);
+    u.f1 = __shfl(t.f1, src, 64); u.f2 = __shfl(t.f2, src, 64);
+    lo += q_push_lanes(A, u, hi - lo);
+    if (lo >= hi) break;
+    if ((long long)__builtin_amdgcn_s_memrealtime() > deadline) {
+      if (__lane_id() == 0) atomicExch(A.abortFlag, 1);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(16);
+  }
+}
+
+// Manager: seed tasks [st, st + k) of the batch into lanes [0, k); returns k.
+__device__ __forceinline__ int claim_seeds(const FillArgs& A, Task& t, int n) {
+  const int ln = __lane_id();
+  int st = A.nSeedTasks;
+  if (ln == 0 && aldi(A.seedNext) < A.nSeedTasks) st = atomicAdd(A.seedNext, n);
+  st = uni(st);
+  int k = A.nSeedTasks - st;
+  k = k < 0 ? 0 : (k > n ? n : k);
+  if (ln < k) {
+    const int2 sk = A.seedTasks[st + ln];
+    const int gp = A.batchPts[sk.x];
+    const int2 px = A.seeds[(size_t)gp * kMaxSeeds + sk.y];
+    const long long N = A.curves[A.ptCurve[gp]].N;
+    // the reference skips seeds outside rows 1..N-1 (lcirs.cc:386-389)
+    const bool rowOk = px.y > 0 && (long long)px.y < N;
+    t.b = sk.x; t.row = rowOk ? px.y : -1; t.xa = px.x; t.xb = px.x; t.label = sk.x;
+    t.kd = 1; t.f1 = 0; t.f2 = -1;
+  }
+  return k;
+}
+
 // State of the task being processed (uniform across the wave).
 struct TaskState {
   const FillArgs* A;
@@ -1076,7 +1311,7 @@ struct TaskState {
                             // the parent, kind 3: the child)
   Task cont;                // follow-up kept by this wave (the next sweep step)
   bool haveCont;
-  bool share;               // some wave was idle at the last look: queue follow-ups
+  GroupPool* G;             // the workgroup's shared pool
   int nBack;
   LookBack back0, back1;
   int accLabel;             // totals of new runs, credited to accLabel when
@@ -1090,13 +1325,14 @@ struct TaskState {
   int lhead, lcount;
 };
 
-// Task hand-off.  Follow-ups a wave does not continue with go to its own ring
-// in LDS (no atomics: one owner); the newest is taken next (depth first).
-// Waves that run dry take tasks from the shared queue, and waves with a
-// backlog move their oldest tasks there while some wave is idle.
+// Task hand-off.  Follow-ups a worker does not continue with go to its own
+// ring in LDS (no atomics: one owner); the newest is taken next (depth
+// first).  While other workers of the group are idle, a worker moves its
+// oldest waiting tasks to the group's pool; a full ring spills its oldest
+// task there too, and a full pool into the global ring.
 __device__ __forceinline__ bool ring_donate(TaskState& S) {
   const Task t = S.ring[S.lhead];
-  if (!q_push(*S.A, t)) return false;
+  if (!pool_push(S.G, t) && !q_push(*S.A, t)) return false;
   S.lhead = (S.lhead + 1) & (kLocalCap - 1);
   S.lcount--;
   return true;
@@ -1104,7 +1340,7 @@ __device__ __forceinline__ bool ring_donate(TaskState& S) {
 
 __device__ __forceinline__ void ring_push(TaskState& S, const Task& f) {
   if (S.lcount == kLocalCap && !ring_donate(S)) {
-    flag_point(*S.A, S.gp, kFlagQueueFull);  // both full: the task is lost
+    flag_point(*S.A, S.gp, kFlagQueueFull);  // everything full: the task is lost
     return;
   }
   const int slot = (S.lhead + S.lcount) & (kLocalCap - 1);
@@ -1148,21 +1384,20 @@ __device__ __forceinline__ Task make_task(int b, int row, int xa, int xb, int la
 }
 
 constexpr int kQueueOnly = 8;  // kd flag: never kept by the wave
-constexpr int kShare = 16;     // kd flag: to the shared queue at once (the wave's
-                               // ring when the queue is full)
+constexpr int kShare = 16;     // kd flag: to the group's pool at once (the wave's
+                               // ring when the pool is full)
 
 // A follow-up of a run, dispatched as soon as it is made: the first one that
 // may stay with the wave becomes its next step, tasks marked kShare go to the
-// shared queue (as do all while some wave is idle and sharing is on), the
-// others to the wave's ring.  (Collecting them first made the compiler keep
-// them in a scratch array.)
+// group's pool, the others to the wave's ring.  (Collecting them first made
+// the compiler keep them in a scratch array.)
 __device__ __forceinline__ void follow(TaskState& S, Task f) {
   const bool queueOnly = (f.kd & kQueueOnly) != 0, share = (f.kd & kShare) != 0;
   f.kd &= ~(kQueueOnly | kShare);
   if (!queueOnly && !share && !S.haveCont) {
     S.cont = f;
     S.haveCont = true;
-  } else if (!((share || S.share) && q_push(*S.A, f))) {
+  } else if (!(share && pool_push(S.G, f))) {
     ring_push(S, f);
   }
 }
@@ -1549,17 +1784,165 @@ __device__ __forceinline__ void process_task(TaskState& S) {
   }
 }
 
-// Work comes from the seed list, then from this wave's ring, then from the
-// shared queue.  A wave that finds none of them leaves the active count and
-// polls; it rejoins before taking a task from the queue, so a task is never
-// queued while the count reads 0.  The fill is over when the count is 0 and
-// the queue is empty.
-__global__ void __launch_bounds__(kFillThreads) fill_kernel(FillArgs A) {
-  __shared__ Task rings[kFillThreads / 64][kLocalCap];
+// The manager wave of a fill workgroup.  It keeps the group's pool fed while
+// workers are idle (seed tasks first, then tasks that other groups queued in
+// the global ring), queues some of a large pool for other groups while any
+// group is idle, and runs the end test.  A group joins the global active
+// count when it starts, before it takes any task; a group whose `pending`
+// count reads 0 with the seed list used up leaves the count and polls the
+// global ring, and rejoins the count before it takes tasks from the ring, so
+// a task is never queued while the count reads 0.  The fill is over when the
+// count is 0 and the ring is empty: no group then holds work, and a group
+// that starts later (its compute unit was busy with other kernels) finds
+// none and leaves.  So the groups need not all be resident at once.
+__device__ void fill_manager(const FillArgs& A, GroupPool* G, long long t0,
+                             unsigned long long& polls) {
   const int ln = __lane_id();
+  if (ln == 0) {
+    afaddi(A.active, 1);
+    mem_order();
+  }
+  bool counted = true;  // the group is in A.active
+  int backoff = 0;
+  unsigned long long loops = 0;
+  long long nextShareCheck = 0, nextQueueCheck = 0;
+  bool othersIdle = false;
+  bool seedsLeft = true;  // the batch's seed list is handed out front to back, once
+  Task t = make_task(0, 0, 0, 0, 0, 0, 1, 0, 0);
+  while (true) {
+    loops++;
+    const long long now = (long long)__builtin_amdgcn_s_memrealtime();
+    if (now - t0 > A.timeLimitTicks) {
+      if (ln == 0) atomicExch(A.abortFlag, 1);
+      break;
+    }
+    int pend = 0, cnt = 0;
+    if (ln == 0) {
+      pend = lds_ld(&G->pending);
+      cnt = lds_ld(&G->count);
+    }
+    pend = uni(pend);
+    cnt = uni(cnt);
+    const int idle = kWorkers - (pend - cnt);  // workers without work
+    if (pend == 0) {
+      // the group is idle: seed tasks, else the global ring
+      int k = seedsLeft ? claim_seeds(A, t, kFeedMax) : 0;
+      seedsLeft = k > 0;
+      if (k > 0) {
+        pool_add_all(A, G, t, 0, k, t0 + A.timeLimitTicks);
+        continue;
+      }
+      if (counted) {
+        if (ln == 0) {
+          mem_order();
+          afaddi(A.active, -1);
+        }
+        counted = false;
+      }
+      polls++;
+      if (backoff < A.pollBackoff) backoff++;
+      for (int s = 0; s < (1 << backoff); s++) __builtin_amdgcn_s_sleep(16);
+      int abortNow = 0, act = 1;
+      unsigned long long qh = 0, qt = 0;
+      if (ln == 0) {
+        if ((polls & 3) == 0) {
+          act = aldi(A.active);
+          abortNow = aldi(A.abortFlag);
+          mem_order();
+        }
+        qh = ald(A.qhead);
+        qt = ald(A.qtail);
+      }
+      if (uni(abortNow)) break;
+      act = uni(act);
+      qh = uni64(qh);
+      qt = uni64(qt);
+      if (qt > qh) {
+        // rejoin before taking tasks
+        if (ln == 0) {
+          afaddi(A.active, 1);
+          mem_order();
+        }
+        counted = true;
+        k = q_pop_lanes(A, t, kFeedMax);
+        if (k > 0) {
+          pool_add_all(A, G, t, 0, k, t0 + A.timeLimitTicks);
+          backoff = 0;
+        } else {
+          if (ln == 0) afaddi(A.active, -1);
+          counted = false;
+        }
+      } else if (act <= 0) {
+        break;  // no group holds work and nothing is queued
+      }
+      continue;
+    }
+    backoff = 0;
+    if (idle > cnt) {
+      // workers are waiting: seed tasks, else (at most every ~2 us) the ring
+      const int want = idle - cnt + 2 < kFeedMax ? idle - cnt + 2 : kFeedMax;
+      int k = seedsLeft ? claim_seeds(A, t, want) : 0;
+      seedsLeft = k > 0;
+      if (k == 0 && now >= nextQueueCheck) {
+        nextQueueCheck = now + 200;
+        k = q_pop_lanes(A, t, want);
+      }
+      if (k > 0) {
+        pool_add_all(A, G, t, 0, k, t0 + A.timeLimitTicks);
+        continue;
+      }
+    } else {
+      // while some other group is idle (the global count is read at most
+      // every ~5 us), the pool's tasks go on to the global ring, and the
+      // workers keep the pool topped up
+      if (now >= nextShareCheck) {
+        nextShareCheck = now + 500;
+        int act = 0;
+        if (ln == 0) act = aldi(A.active);
+        othersIdle = uni(act) < A.nGroups;
+        if (ln == 0) lds_st(&G->hungry, othersIdle ? 1 : 0);
+      }
+      if (othersIdle && cnt >= 2) {
+        const int n = cnt / 2 < kDonateMax ? cnt / 2 : kDonateMax;
+        const int k = pool_remove_lanes(G, t, n);
+        const int q = q_push_lanes(A, t, k);
+        if (q < k) pool_add_all(A, G, t, q, k, t0 + A.timeLimitTicks);  // ring full: keep the rest
+        if (q > 0) continue;
+      }
+    }
+    __builtin_amdgcn_s_sleep(4);
+  }
+  if (ln == 0) lds_st(&G->done, 1);
+}
+
+// Worker waves take work from their own ring, then from the group's pool;
+// a worker that finds neither is idle and waits on the pool until the
+// manager ends the fill.
+__global__ void __launch_bounds__(kFillThreads) fill_kernel(FillArgs A) {
+  __shared__ Task rings[kWorkers][kLocalCap];
+  __shared__ GroupPool pool;
+  const int ln = __lane_id();
+  const int wave = threadIdx.x / 64;
   const long long t0 = (long long)__builtin_amdgcn_s_memrealtime();
+  GroupPool* G = &pool;
+  if (threadIdx.x == 0) {
+    G->lock = 0;
+    G->pending = 0;
+    G->head = 0;
+    G->count = 0;
+    G->done = 0;
+    G->hungry = 0;
+  }
+  __syncthreads();
+  unsigned long long nTasks = 0, busyTicks = 0, idlePolls = 0;
+  if (wave == 0) {
+    fill_manager(A, G, t0, idlePolls);
+    if (ln == 0) atomicAdd(&A.stats[6], idlePolls);
+    return;
+  }
   TaskState S;
   S.A = &A;
+  S.G = G;
   S.curB = -1;
   S.evals = 0;
   S.nIns = 0;
@@ -1570,87 +1953,44 @@ __global__ void __launch_bounds__(kFillThreads) fill_kernel(FillArgs A) {
   S.accLabel = -1;
   S.accSum = 0;
   S.accCnt = 0;
-  S.ring = rings[threadIdx.x / 64];
+  S.ring = rings[wave - 1];
   S.lhead = 0;
   S.lcount = 0;
-  S.share = false;
-  unsigned long long nTasks = 0, busyTicks = 0, idlePolls = 0;
-  bool have = false;
+  bool have = false, busy = false;
   while (true) {
     if (!have && S.lcount > 0) {
       S.t = ring_pop(S);
       have = true;
     }
+    if (!have && busy) {
+      // own work done: the pool's oldest task, or idle
+      flush_acc(S);
+      have = pool_take(G, S.t, true);
+      busy = have;
+    }
     if (!have) {
-      if (aldi(A.abortFlag)) break;
-      int st = A.nSeedTasks;
-      if (ln == 0 && aldi(A.seedNext) < A.nSeedTasks) st = atomicAdd(A.seedNext, 1);
-      st = bcast0(st);
-      if (st < A.nSeedTasks) {
-        const int2 sk = A.seedTasks[st];
-        const int gp = A.batchPts[sk.x];
-        const int2 px = A.seeds[(size_t)gp * kMaxSeeds + sk.y];
-        const CurveDev& C = A.curves[A.ptCurve[gp]];
-        // the reference skips seeds outside rows 1..N-1 (lcirs.cc:386-389)
-        const bool rowOk = px.y > 0 && (long long)px.y < C.N;
-        S.t = make_task(sk.x, rowOk ? px.y : -1, px.x, px.x, sk.x, 1, 1, 0, -1);
-        have = true;
-      } else if (q_pop(A, S.t)) {
-        have = true;
-      } else {
-        // idle: leave the active count, then poll
-        flush_acc(S);
+      bool over = false;
+      while (true) {
+        idlePolls++;
+        __builtin_amdgcn_s_sleep(24);
+        int d = 0, c = 0;
         if (ln == 0) {
-          mem_order();
-          afaddi(A.active, -1);
+          d = lds_ld(&G->done);
+          c = lds_ld(&G->count);
         }
-        bool done = false;
-        int backoff = 0;
-        while (!have) {
-          // poll at most every ~30 us once idle for a while, reading the
-          // active count and the abort flag only every fourth time
-          idlePolls++;
-          if (backoff < A.pollBackoff) backoff++;
-          for (int k = 0; k < (1 << backoff); k++) __builtin_amdgcn_s_sleep(16);
-          const bool full = (idlePolls & 3) == 0;
-          int act = 1, abortNow = 0;
-          unsigned long long qh = 0, qt = 0;
-          if (ln == 0) {
-            if (full) {
-              act = aldi(A.active);
-              abortNow = aldi(A.abortFlag);
-              mem_order();
-            }
-            qh = ald(A.qhead);
-            qt = ald(A.qtail);
-          }
-          act = bcast0(act);
-          qh = bcast0(qh);
-          qt = bcast0(qt);
-          if (bcast0(abortNow)) { done = true; break; }
-          if ((long long)__builtin_amdgcn_s_memrealtime() - t0 > A.timeLimitTicks) {
-            if (ln == 0) atomicExch(A.abortFlag, 1);
-            done = true;
-            break;
-          }
-          if (qt > qh) {
-            // rejoin before taking the task
-            if (ln == 0) {
-              afaddi(A.active, 1);
-              mem_order();
-            }
-            if (q_pop(A, S.t)) {
-              have = true;
-            } else if (ln == 0) {
-              afaddi(A.active, -1);
-            }
-          } else if (act <= 0) {
-            done = true;  // no wave holds work and nothing is queued
-            break;
-          }
+        if (uni(d)) { over = true; break; }
+        if (uni(c) > 0 && pool_take(G, S.t, false)) break;
+        if ((idlePolls & 255) == 0 &&
+            (long long)__builtin_amdgcn_s_memrealtime() - t0 > A.timeLimitTicks + 100000000ll) {
+          // the manager ends the fill at the time limit; this bound only
+          // guards against a manager that never does
+          over = true;
+          break;
         }
-        if (done) break;
       }
+      if (over) break;
+      have = true;
+      busy = true;
     }
     nTasks++;
     const unsigned long long tb0 = __builtin_amdgcn_s_memrealtime();
@@ -1659,19 +1999,23 @@ __global__ void __launch_bounds__(kFillThreads) fill_kernel(FillArgs A) {
     if (S.haveCont) {
       S.t = S.cont;
     } else {
-      flush_acc(S);
       have = false;
     }
-    if ((nTasks & 7) == 0 && (S.lcount > 0 || A.shareIdle)) {
-      // hand the oldest waiting tasks to idle waves
-      int act = 0;
-      if (ln == 0) act = aldi(A.active);
-      act = bcast0(act);
-      int give = A.nWaves - act;
-      S.share = A.shareIdle && give > 0;
-      if (give > 4) give = 4;
-      for (int k = 0; k < give && S.lcount > 0; k++)
-        if (!ring_donate(S)) break;
+    if ((nTasks & 3) == 0 && S.lcount > 0) {
+      // hand waiting tasks to idle workers of the group, and while other
+      // groups are idle keep the pool topped up for them
+      int pend = 0, cnt = 0, hungry = 0;
+      if (ln == 0) {
+        pend = lds_ld(&G->pending);
+        cnt = lds_ld(&G->count);
+        hungry = lds_ld(&G->hungry);
+      }
+      cnt = uni(cnt);
+      const int idle = kWorkers - (uni(pend) - cnt);
+      if (idle > cnt || (uni(hungry) && cnt < kPoolShare)) {
+        ring_donate(S);
+        if (S.lcount > 2) ring_donate(S);
+      }
     }
     if ((nTasks & 255) == 0) {
       const long long now = (long long)__builtin_amdgcn_s_memrealtime();
@@ -1679,6 +2023,9 @@ __global__ void __launch_bounds__(kFillThreads) fill_kernel(FillArgs A) {
         if (ln == 0) atomicExch(A.abortFlag, 1);
         break;
       }
+      int ab = 0;
+      if (ln == 0) ab = aldi(A.abortFlag);
+      if (uni(ab)) break;
     }
   }
   flush_acc(S);
@@ -1696,9 +2043,8 @@ __global__ void __launch_bounds__(kFillThreads) fill_kernel(FillArgs A) {
   }
 }
 
-// Workgroups of fill_kernel that can be resident on the device at once.  The
-// fill's end test counts every wave of the launch as working until it has
-// started and found nothing to do, so the launch must not exceed this.
+// Workgroups of fill_kernel that can be resident on the device at once (a
+// larger launch is correct, its surplus groups run after the others).
 int fill_resident_workgroups() {
   int perCu = 0, dev = 0, cus = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
@@ -1750,7 +2096,7 @@ __global__ void fill_reset_kernel(FillArgs A) {
   if (tid == 0) {
     *A.qhead = 0;
     *A.qtail = 0;
-    *A.active = A.nWaves;
+    *A.active = 0;
     *A.seedNext = 0;
     *A.labelNext = A.nBatch;
     *A.abortFlag = 0;
