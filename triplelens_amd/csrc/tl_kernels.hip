@@ -958,63 +958,67 @@ __device__ __forceinline__ void afaddi(int* p, int v) {
 // in registers across the scan.
 struct RowCtx {
   Lens L;
-  double blx, bly, step, rho2, ldA, ldB, sumScale;
+  double blx, bly, step, rho2, invRho2, ldA, ldB, sumScale;
   double y, px, py;
   int N;
   int binIrs;  // binary ray map (irsBinary)
 };
 
+// sqrt(a) for 0 <= a <= 1 within a few units in the last place: the
+// hardware reciprocal square root and two Goldschmidt refinements (the
+// brightness term only feeds the fixed-point magnification sum, whose
+// rounding is 2^-34 of a pixel).  a = 0 gives 0.
+__device__ __forceinline__ double sqrt_unit(double a) {
+  const double y = __builtin_amdgcn_rsq(a);
+  double s = a * y, h = 0.5 * y;
+  double r = fma(-s, h, 0.5);
+  s = fma(s, r, s);
+  h = fma(h, r, h);
+  r = fma(-s, s, a);
+  s = fma(r, h, s);
+  return a > 0.0 ? s : 0.0;
+}
+
 // Ray test of pixel x of the row (lcirs.cc:400-402) and its brightness
 // A + B sqrt(1 - r^2) (lcirs.cc:406-407, limbdarkeningmodel.cc:45-49) as a
 // multiple of 1/sumScale.  The test is the reference's to the bit (irsSIMD,
-// or irsBinary for m3 = 0); the brightness is computed with IEEE division
-// and square root and rounded to the nearest integer multiple by adding
-// 1.5 * 2^52 (exact for |value| below 2^51, either sign).
+// or irsBinary for m3 = 0).  The brightness is rounded to the nearest
+// integer multiple by adding 1.5 * 2^52 (exact for |value| below 2^51,
+// either sign); its r^2 / rho^2 and square root are within a few units in
+// the last place of the IEEE operations, far below that rounding.
 __device__ __forceinline__ bool eval_px(const RowCtx& R, int x, long long& val) {
   const double xx = R.blx + R.step * (double)x;
   const double rsq = R.binIrs ? irs_binary(R.L, xx, R.y, R.px, R.py)
                               : irs_ray(R.L, xx, R.y, R.px, R.py);
   if (!(rsq < R.rho2)) return false;
-  // sourceBrightnessRSq(rsq / rho^2) (lcirs.cc:406-407,
-  // limbdarkeningmodel.cc:45-49): IEEE division and square root
-  const double v = R.ldA + R.ldB * sqrt(1.0 - rsq / R.rho2);
+  const double v = R.ldA + R.ldB * sqrt_unit(1.0 - rsq * R.invRho2);
   const double t = fma(v, R.sumScale, 6755399441055744.0);  // 1.5 * 2^52
   val = __double_as_longlong(t) - 0x4338000000000000ll;
   return true;
 }
 
-// Result of following a run past the evaluated window.
-struct Ext {
-  int end;
-  long long sum;   // lane-local brightness of the pixels taken in
-  int cnt;
-  int evals;
-};
-
-// Follows a run from column `from` in direction dir (-1 left, +1 right) until
-// its first miss or the grid edge.
-__device__ __forceinline__ Ext extend_run(RowCtx R, int from, int dir) {
+// Follows a run leftwards from column `from` until its first miss or column
+// 0; returns the run's first column and adds the pixels taken in to acc
+// (lane-local) and cnt.
+__device__ __forceinline__ int extend_left(const RowCtx& R, int from, long long& acc,
+                                           unsigned long long& cnt, unsigned long long& evals) {
   const int ln = __lane_id();
-  Ext r;
-  r.sum = 0;
-  r.cnt = 0;
-  r.evals = 0;
   int xcur = from;
   while (true) {
-    if (xcur < 0) { r.end = 0; return r; }
-    if (xcur > R.N - 1) { r.end = R.N - 1; return r; }
-    // lane l looks at xcur + dir*l
-    const int x = xcur + dir * ln;
-    const bool valid = x >= 0 && x <= R.N - 1;
+    if (xcur < 0) return 0;
+    // lane l looks at xcur - l
+    const int x = xcur - ln;
+    const bool valid = x >= 0;
     long long val = 0;
     const bool hit = valid && eval_px(R, x, val);
-    r.evals += __builtin_popcountll(__ballot(valid));
+    const int nb = xcur + 1 < 64 ? xcur + 1 : 64;
+    evals += (unsigned long long)nb;
     const unsigned long long m = __ballot(hit);
     const int z = ~m == 0 ? 64 : __builtin_ctzll(~m);  // hits counted from lane 0
-    if (ln < z) r.sum += val;
-    r.cnt += z;
-    if (z < 64) { r.end = xcur + dir * (z - 1); return r; }
-    xcur += dir * 64;
+    if (ln < z) acc += val;
+    cnt += (unsigned long long)z;
+    if (z < 64) return xcur - (z - 1);
+    xcur -= 64;
   }
 }
 
@@ -1579,9 +1583,11 @@ __device__ __forceinline__ void handle_run(TaskState& S, int xl, int xr, long lo
 // Maximal hit runs of the row that meet [s, e].  The row is evaluated over a
 // window that pads [s, e] to whole 64-pixel steps on the sides where runs
 // may continue (extL, extR); elsewhere the pixel next to [s, e] is a known
-// miss.  Runs inside the padding that do not reach [s, e] are ignored; runs
-// that reach the window's edge are followed past it.  Each run is handed to
-// handle_run once both its ends are known.
+// miss.  Runs inside the padding that do not reach [s, e] are ignored; a run
+// that reaches the window's right edge is followed past it in the same loop,
+// one that reaches its left edge by extend_left.  Each run is handed to
+// handle_run once both its ends are known.  A 64-pixel step that lies wholly
+// inside the open run, or wholly outside any run, costs only its ray tests.
 __device__ __forceinline__ void scan_window(const RowCtx& R, int s, int e, bool extL, bool extR,
                                             TaskState& S) {
   const int ln = __lane_id();
@@ -1591,29 +1597,54 @@ __device__ __forceinline__ void scan_window(const RowCtx& R, int s, int e, bool 
   const int padR = extR ? extra - padL : 0;
   const int ws = s - padL < 0 ? 0 : s - padL;
   const int we = e + padR > R.N - 1 ? R.N - 1 : e + padR;
+  int hi = we;             // last column of the scan (grows past we while a run is followed)
   bool open = false;
-  int runL = 0, cb = ws, nb = 0, pos = 0;
+  int runL = 0, cb = ws - 64, nb = 0, pos = 0;
   unsigned long long m = 0, cnt = 0;
-  long long acc = 0, val = 0;
-  bool hit = false;
-  while (true) {
+  long long acc = 0, val = 0;  // lane-local brightness of the open run, of this lane's pixel
+  for (int guard = 0;; guard++) {
+    if (guard > (1 << 27)) {
+      // more iterations than any row of 2^25 pixels needs: never expected;
+      // the point is flagged and the launch ends
+      flag_point(*S.A, S.gp, kFlagTimeout);
+      if (ln == 0) atomicExch(S.A->abortFlag, 1);
+      break;
+    }
     bool emit = false, last = false;
     int eR = 0;
     if (pos >= nb) {
-      // next 64 pixels
-      if (nb > 0) cb += 64;
-      if (cb > we) {
-        last = true;
-        emit = open && runL <= e;
-        eR = we;
-      } else {
+      // the next 64 pixels, or the end of the scan
+      if (cb + 64 > hi) {
+        if (!open) break;
+        if (hi == we && extR && we < R.N - 1) {
+          hi = R.N - 1;  // follow the run past the window, from we + 1 on
+          cb = we + 1 - 64;
+        } else {
+          last = true;  // the run reaches the scan's last column
+          eR = hi;
+          emit = hi >= s && runL <= e;
+          if (!emit) break;
+        }
+      }
+      if (!last) {
+        if (!open && cb + 64 > e) break;  // no later run can meet [s, e]
+        cb += 64;
         const int x = cb + ln;
         val = 0;
-        hit = x <= we && eval_px(R, x, val);
+        const bool hit = x <= hi && eval_px(R, x, val);
         m = __ballot(hit);
-        nb = we - cb + 1 < 64 ? we - cb + 1 : 64;
+        nb = hi - cb + 1 < 64 ? hi - cb + 1 : 64;
         S.evals += (unsigned long long)nb;
         pos = 0;
+        const unsigned long long full = nb == 64 ? ~0ull : ((1ull << nb) - 1ull);
+        if (open && m == full) {
+          // the step lies inside the open run
+          acc += val;
+          cnt += (unsigned long long)nb;
+          pos = nb;
+        } else if (!open && m == 0) {
+          pos = nb;
+        }
         continue;
       }
     } else if (open) {
@@ -1624,40 +1655,36 @@ __device__ __forceinline__ void scan_window(const RowCtx& R, int s, int e, bool 
       cnt += (unsigned long long)z;
       pos += z;
       if (pos < nb) {
+        // the run ends at cb + pos - 1
         eR = cb + pos - 1;
-        open = false;
         emit = eR >= s && runL <= e;
-        if (!emit) { acc = 0; cnt = 0; }
+        if (!emit) {
+          open = false;
+          acc = 0;
+          cnt = 0;
+          if (hi > we) break;  // the followed run has ended
+        }
       }
     } else {
       const unsigned long long rest = m >> pos;
-      const int tz = rest == 0 ? 64 : __builtin_ctzll(rest);
-      if (pos + tz >= nb) {
+      if (rest == 0) {
         pos = nb;
-      } else {
-        runL = cb + pos + tz;
-        open = true;
-        pos += tz;
+        continue;
       }
+      const int tz = __builtin_ctzll(rest);
+      if (cb + pos + tz > e) break;  // no later run can meet [s, e]
+      runL = cb + pos + tz;
+      open = true;
+      pos += tz;
     }
     if (emit) {
-      for (int side = 0; side < 2; side++) {
-        const bool need = side == 0 ? (runL == ws && extL && ws > 0)
-                                    : (last && extR && we < R.N - 1);
-        if (!need) continue;
-        const Ext x = extend_run(R, side == 0 ? ws - 1 : we + 1, side == 0 ? -1 : 1);
-        if (side == 0) runL = x.end;
-        else eR = x.end;
-        acc += x.sum;
-        cnt += (unsigned long long)x.cnt;
-        S.evals += x.evals;
-      }
+      if (runL == ws && extL && ws > 0) runL = extend_left(R, ws - 1, acc, cnt, S.evals);
       handle_run(S, runL, eR, wave_sum(acc), cnt);
       acc = 0;
       cnt = 0;
       open = false;
+      if (last || hi > we) break;
     }
-    if (last) break;
   }
 }
 
@@ -1696,6 +1723,7 @@ __device__ __forceinline__ void load_task(TaskState& S) {
     S.R.bly = unid(C.bly);
     S.R.step = unid(C.step);
     S.R.rho2 = unid(C.rho2);
+    S.R.invRho2 = 1.0 / S.R.rho2;
     S.R.ldA = unid(C.ldA);
     S.R.ldB = unid(C.ldB);
     S.R.sumScale = unid(C.sumScale);
