@@ -186,8 +186,11 @@ int lcirs_batch_fill_positions(int n, void** lcs, const int* counts, const int* 
  * inserts (ms), out[10] idle polls, out[11] wavefronts of the last launch,
  * out[12] scout tasks, out[13] runs found by scouts, out[14] milliseconds
  * of the solve lists and their polynomials (trial kernels), out[15] of the
- * disk samples, out[16] solves (centre and caustic-trial positions).  out
- * must hold 17 values. */
+ * disk samples, out[16] solves (centre and caustic-trial positions),
+ * out[17] pieces of the verified speculative chain, out[18] of them not in
+ * agreement with their predecessor (repair chains run), out[19] 1 when the
+ * speculative chain could not be resolved and the serial chain ran.  out
+ * must hold 20 values. */
 int tl_last_stats(double* out);
 
 /* Critical curves and caustics of n geometries on the GPU (one lane each):
@@ -231,7 +234,7 @@ int tl_math_selftest(int n, unsigned long long seed);
 /* Engine settings: name/value pairs ("fill_workgroups", "probe_limit",
  * "queue_log2", "scout_rows", "scout_jump", "scout_retry", "scout_width",
  * "scout_share", "share_rays", "use_graph", "leap_every", "leap_jumps", "poll_backoff",
- * "run_pixels", "table_load", "seed_chunks",
+ * "run_pixels", "table_load", "seed_chunks", "spec_piece", "spec_overlap", "spec_depth",
  * "seed_chunk_min", "time_limit_s").  None changes a result.  Returns 0 or
  * -1 for an unknown name.  No counterpart in the reference. */
 int tl_set_option(const char* name, double value);

@@ -161,6 +161,42 @@ def test_c2_full_resolution_matches_reference(E, plain):
     _check_against_fixture(E, g, [lc.lc_irs(*c.traj())], plain)
 
 
+def test_speculative_chain_equals_serial_chain(E):
+    """The verified speculative chain (pieces in parallel, repair chains
+    where a piece disagrees with its predecessor) gives the single chain's
+    seeds and the reference's ray counts on the bench curve; so does the
+    serial chain, and a repair depth too small to resolve the curve falls
+    back to the serial chain with the same results."""
+    g = golden("c2_full.npz")
+    c = _fixture_curves(g)[0]
+    runs = {}
+    try:
+        for name, opts in [("spec", {}), ("serial", {"spec_piece": 0}),
+                           ("short_pieces", {"spec_piece": 16, "spec_overlap": 8,
+                                             "spec_depth": 256}),
+                           ("fallback", {"spec_depth": 1})]:
+            for k, v in opts.items():
+                E.set_option(k, v)
+            lc = _new(E, c)
+            res = lc.lc_irs(*c.traj())
+            st = E.last_stats()
+            runs[name] = (res, lc.seeds(), st)
+            for k in opts:
+                E.set_option(k, {"spec_piece": 32, "spec_overlap": 16, "spec_depth": 64}[k])
+    finally:
+        for k, v in {"spec_piece": 32, "spec_overlap": 16, "spec_depth": 64}.items():
+            E.set_option(k, v)
+    assert runs["spec"][2]["spec_pieces"] == 4096 // 32 and runs["spec"][2]["spec_fallback"] == 0
+    assert runs["serial"][2]["spec_pieces"] == 0
+    assert runs["short_pieces"][2]["spec_pieces"] == 4096 // 16
+    assert runs["fallback"][2]["spec_fallback"] == 1
+    ns0, pix0 = runs["serial"][1]
+    for name, (res, (ns, pix), st) in runs.items():
+        _check_against_fixture(E, g, [res])
+        assert np.array_equal(ns, ns0), name
+        assert np.array_equal(pix, pix0), name
+
+
 def test_c2_window_full_resolution_golden(E):
     """A 65-point curve around the C2 magnification peak at pixel rho/200."""
     g = golden("c2_window.npz")

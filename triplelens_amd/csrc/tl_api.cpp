@@ -220,12 +220,17 @@ struct Options {
   double timeLimitS = 60.0;
   int seedChunks = 3;       // the seed search is cut into this many chunks of positions
   int seedChunkMin = 64;    // ... of at least this many positions per curve
+  int specPiece = 32;       // verified speculative chain: positions per piece (0: the
+                            // serial chain in chunks)
+  int specOverlap = 16;     // ... positions a piece's chain runs into the next piece
+  int specDepth = 64;       // ... piece boundaries a repair chain may cross
 };
 
 struct Stats {
   double tasks = 0, evals = 0, ins = 0, dup = 0, seedMs = 0, fillMs = 0, fillLaunches = 0,
          totalMs = 0, busyMs = 0, insertMs = 0, idlePolls = 0, waves = 0, scouts = 0,
-         scoutRuns = 0, trialMs = 0, ringMs = 0, solves = 0;
+         scoutRuns = 0, trialMs = 0, ringMs = 0, solves = 0, specPieces = 0, specRepairs = 0,
+         specFallback = 0;
 };
 
 // One recorded run of the amoeba print-out: row, leftmost and rightmost pixel.
@@ -295,6 +300,11 @@ struct Engine {
   DBuf<Lens> dLenses;
   DBuf<CccTable> dCccTab;
   DBuf<cx> dCcOut, dCaOut;
+  // verified speculative chain (SpecArgs)
+  DBuf<int> dSpecBase, dSpecMerged, dSpecTarget, dSpecFailed;
+  DBuf<cx> dSpecOwn, dSpecExt, dSpecRep, dSpecStart;
+  std::vector<int> specCalls;  // per curve: the chain's solve count after the run
+  int specPieces = 0;
   Options opt;
   Stats stats;
 
@@ -326,6 +336,7 @@ struct Engine {
   }
 
   void run(Job& job);
+  bool run_spec(std::vector<CurveDev>& hc, const std::vector<int>& pointBase, SeedArgs& sa);
   void point_lc(Curve& c, cx s, cx e);
   void collect_amoebae(const Job& job, const std::vector<int>& ptCurve,
                        const std::vector<int>& batchPts, unsigned long long hcap, int labelCap,
@@ -407,6 +418,86 @@ CurveDev to_dev(const Curve& c, cx s, cx e, int pointBase, int causticBase) {
   return d;
 }
 
+// The verified speculative chain's rounds on the seed stream (after the
+// solve pool is built): cuts every curve into pieces of about
+// opt.specPiece positions, runs rounds 1 and 2 and the resolution, and on
+// success leaves the piece start states in `sa` for seed_kernel (and the
+// pieces in the uploaded curves).  Returns false, with the curves restored
+// to one piece, when some curve could not be resolved within
+// opt.specDepth piece boundaries.
+bool Engine::run_spec(std::vector<CurveDev>& hc, const std::vector<int>& pointBase,
+                      SeedArgs& sa) {
+  const int nc = (int)hc.size();
+  const int nPts = pointBase[nc];
+  std::vector<int> base(nc + 1, 0);
+  int maxP = 1;
+  for (int k = 0; k < nc; k++) {
+    const int n = pointBase[k + 1] - pointBase[k];
+    const int P = std::max(1, n / opt.specPiece);
+    hc[k].split = P;
+    base[k + 1] = base[k] + P;
+    maxP = std::max(maxP, P);
+  }
+  const int G = base[nc];
+  HIPCHK(hipMemcpyAsync(dCurves.p, hc.data(), nc * sizeof(CurveDev), hipMemcpyHostToDevice,
+                        sstream));
+  dSpecBase.ensure(nc + 1);
+  HIPCHK(hipMemcpyAsync(dSpecBase.p, base.data(), (nc + 1) * sizeof(int), hipMemcpyHostToDevice,
+                        sstream));
+  dSpecOwn.ensure((size_t)nPts * 10);
+  dSpecExt.ensure((size_t)nPts * 10);
+  dSpecMerged.ensure(G);
+  dSpecTarget.ensure(G);
+  dSpecRep.ensure((size_t)G * opt.specDepth * 10);
+  dSpecStart.ensure((size_t)G * 10);
+  dSpecFailed.ensure(1);
+  int zero = 0;
+  HIPCHK(hipMemcpyAsync(dSpecFailed.p, &zero, sizeof(int), hipMemcpyHostToDevice, sstream));
+  SpecArgs sp;
+  sp.pieceBase = dSpecBase.p;
+  sp.nPieces = G;
+  sp.overlap = std::max(1, opt.specOverlap);
+  sp.depth = std::max(1, opt.specDepth);
+  sp.own = dSpecOwn.p;
+  sp.ext = dSpecExt.p;
+  sp.merged = dSpecMerged.p;
+  sp.repState = dSpecRep.p;
+  sp.repTarget = dSpecTarget.p;
+  sp.start = dSpecStart.p;
+  sp.failed = dSpecFailed.p;
+  SeedArgs a = sa;
+  a.maxSplit = maxP;
+  a.chunk = 0;
+  a.nChunks = 1;
+  a.chainIn = nullptr;
+  HIPCHK((hipError_t)launch_spec(a, sp, sstream));
+  int failed = 0;
+  std::vector<int> merged(G);
+  HIPCHK(hipMemcpyAsync(&failed, dSpecFailed.p, sizeof(int), hipMemcpyDeviceToHost, sstream));
+  HIPCHK(hipMemcpyAsync(merged.data(), dSpecMerged.p, G * sizeof(int), hipMemcpyDeviceToHost,
+                        sstream));
+  HIPCHK(hipStreamSynchronize(sstream));
+  if (failed) {
+    for (auto& d : hc) d.split = 1;
+    HIPCHK(hipMemcpyAsync(dCurves.p, hc.data(), nc * sizeof(CurveDev), hipMemcpyHostToDevice,
+                          sstream));
+    stats.specFallback = 1;
+    return false;
+  }
+  int repairs = 0;
+  for (int k = 0; k < nc; k++)
+    for (int q = 1; q < hc[k].split; q++) repairs += merged[base[k] + q] ? 0 : 1;
+  stats.specRepairs = repairs;
+  specPieces = G;
+  sa.maxSplit = maxP;
+  sa.chunk = 0;
+  sa.nChunks = 1;
+  sa.chainIn = nullptr;
+  sa.pieceBase = dSpecBase.p;
+  sa.pieceStart = dSpecStart.p;
+  return true;
+}
+
 void Engine::run(Job& job) {
   init();
   auto tStart = std::chrono::steady_clock::now();
@@ -480,12 +571,19 @@ void Engine::run(Job& job) {
   // len positions, so that every chain does the same share of its work
   int nChunks = 1;
   std::vector<hipEvent_t>& evSeed = seedEvents;
+  // the verified speculative chain: the single chain's seeds, computed in
+  // pieces in parallel (not with a split search, and not where the chain
+  // replays recorded slot values)
+  bool spec = job.runSeeds && opt.specPiece > 0 && slots.empty() &&
+              (int)maxLen >= 2 * opt.specPiece;
+  for (auto* c : job.curves) spec = spec && c->split == 1;
   if (job.runSeeds) {
     nChunks = std::max(1, std::min<int>(opt.seedChunks, (int)(maxLen / std::max(1, opt.seedChunkMin))));
     // a split seed search runs in parallel and needs no overlap with the
     // fills; one chunk keeps piece q at positions [n q / P, n (q+1) / P)
     for (auto* c : job.curves)
       if (c->split > 1) nChunks = 1;
+    if (spec) nChunks = 1;
     while ((int)evSeed.size() < nChunks + 1) {
       hipEvent_t ev;
       HIPCHK(hipEventCreate(&ev));
@@ -554,7 +652,26 @@ void Engine::run(Job& job) {
     for (auto* c : job.curves) sa.maxSplit = std::max(sa.maxSplit, c->split);
     sa.flags = dFlags.p;
     sa.chainOut = dChain.p;
-    for (int ch = 0; ch < nChunks; ch++) {
+    sa.pieceBase = nullptr;
+    sa.pieceStart = nullptr;
+    if (spec && run_spec(hc, pointBase, sa)) {
+      HIPCHK((hipError_t)launch_seed(sa, sstream));
+      HIPCHK(hipEventRecord(evSeed[0], sstream));
+      // the pieces' seed launch restarted its solve count at each piece:
+      // the chain's count after the run is the start count plus every
+      // solve of the curve
+      std::vector<int> cnt(nPts);
+      HIPCHK(hipMemcpyAsync(cnt.data(), dPoolCount.p, nPts * sizeof(int), hipMemcpyDeviceToHost,
+                            sstream));
+      HIPCHK(hipStreamSynchronize(sstream));
+      specCalls.assign(nc, 0);
+      for (int k = 0; k < nc; k++) {
+        long long t = job.curves[k]->chain.calls;
+        for (int p = pointBase[k]; p < pointBase[k + 1]; p++) t += cnt[p];
+        specCalls[k] = (int)t;
+      }
+      stats.specPieces = specPieces;
+    } else for (int ch = 0; ch < nChunks; ch++) {
       sa.chunk = ch;
       sa.nChunks = nChunks;
       sa.chainIn = ch == 0 ? nullptr : dChain.p;
@@ -809,7 +926,11 @@ void Engine::run(Job& job) {
     }
     std::vector<RootChain> chains(nc);
     HIPCHK(hipMemcpy(chains.data(), dChain.p, nc * sizeof(RootChain), hipMemcpyDeviceToHost));
-    for (int k = 0; k < nc; k++) job.curves[k]->chain = chains[k];
+    for (int k = 0; k < nc; k++) {
+      if (!specCalls.empty()) chains[k].calls = specCalls[k];
+      job.curves[k]->chain = chains[k];
+    }
+    specCalls.clear();
     HIPCHK(hipMemcpy(nSeeds.data(), dNSeeds.p, nPts * sizeof(int), hipMemcpyDeviceToHost));
   }
   HIPCHK(hipMemcpy(flags.data(), dFlags.p, nPts * sizeof(uint32_t), hipMemcpyDeviceToHost));
@@ -1531,7 +1652,8 @@ int tl_last_stats(double* out) {
   out[4] = s.seedMs; out[5] = s.fillMs; out[6] = s.fillLaunches; out[7] = s.totalMs;
   out[8] = s.busyMs; out[9] = s.insertMs; out[10] = s.idlePolls; out[11] = s.waves;
   out[12] = s.scouts; out[13] = s.scoutRuns; out[14] = s.trialMs; out[15] = s.ringMs;
-  out[16] = s.solves;
+  out[16] = s.solves; out[17] = s.specPieces; out[18] = s.specRepairs;
+  out[19] = s.specFallback;
   return 0;
 }
 
@@ -1558,6 +1680,9 @@ int tl_set_option(const char* name, double value) {
   else if (n == "time_limit_s") o.timeLimitS = std::max(0.1, value);
   else if (n == "seed_chunks") o.seedChunks = std::max(1, (int)value);
   else if (n == "seed_chunk_min") o.seedChunkMin = std::max(1, (int)value);
+  else if (n == "spec_piece") o.specPiece = std::max(0, (int)value);
+  else if (n == "spec_overlap") o.specOverlap = std::min(4096, std::max(1, (int)value));
+  else if (n == "spec_depth") o.specDepth = std::min(4096, std::max(1, (int)value));
   else return -1;
   return 0;
 }

@@ -119,6 +119,41 @@ struct SeedArgs {
   int chunk, nChunks;  // positions [n chunk / nChunks, n (chunk+1) / nChunks) of every
                        // curve of n positions
   int maxSplit;        // workgroups per curve: the largest CurveDev::split of the launch
+  // Pieces with known start states (the verified speculative chain): piece
+  // q > 0 of curve c starts from the roots pieceStart[(pieceBase[c] + q) * 10
+  // + k] (the true chain's state before its first position) instead of a
+  // fresh solve.  Null: fresh starts.
+  const int* pieceBase;
+  const cx* pieceStart;
+};
+
+// The verified speculative chain.  Every curve of n positions is cut into
+// P = CurveDev::split pieces [n q / P, n (q+1) / P).  Round 1 runs every piece
+// from a fresh solve (piece 0 from the curve's own chain state) and on into
+// the next piece for `overlap` positions, recording the state after every
+// position.  Where the next piece's own state equals the predecessor's
+// continuation at one of its first `overlap` positions, both chains agree
+// from there on.  Where none does, round 2 runs a repair chain from the
+// predecessor's end state on through the following pieces until it agrees
+// with one of them.  spec_resolve_kernel then walks each curve's pieces from
+// its first (whose start is known) and picks the true state at every piece
+// start; seed_kernel recomputes all pieces from those states in parallel.
+// A chain that has not agreed after `depth` further pieces sets `failed`,
+// and the host falls back to the serial chain.
+struct SpecArgs {
+  const int* pieceBase;  // [curve] first global piece
+  int nPieces;           // global pieces
+  int overlap;           // positions of the next piece a chain runs into (<= its length)
+  int depth;             // piece boundaries a repair chain may cross
+  cx* own;               // [point * 10 + k] state after the point, by the piece that owns it
+  cx* ext;               // [point * 10 + k] ... by the preceding piece's continuation
+  int* merged;           // [piece] round 1: the piece agrees with its predecessor
+  cx* repState;          // [(piece * depth + j) * 10 + k] repair chain of piece q: state
+                         // before the first position of piece q + 1 + j
+  int* repTarget;        // [piece] the piece its repair chain agreed with; -1: ran to the
+                         // curve's end; -2: crossed `depth` boundaries without agreeing
+  cx* start;             // [piece * 10 + k] resolved start states
+  int* failed;           // set when some curve could not be resolved
 };
 
 struct FillArgs {
@@ -176,6 +211,10 @@ int launch_trials(const TrialArgs& a, hipStream_t s);
 // The polynomials of the first nSolves solves of the pool (after launch_trials).
 int launch_trial_polys(const TrialArgs& a, int nSolves, hipStream_t s);
 int launch_seed(const SeedArgs& a, hipStream_t s);
+// Rounds of the verified speculative chain (SpecArgs), with a.maxSplit and
+// the curves' split set to the pieces: round 1, the repair chains, and the
+// resolution of the piece start states.
+int launch_spec(const SeedArgs& a, const SpecArgs& sp, hipStream_t s);
 // image_poly for n source positions of one lens on the GPU (test access):
 // zeta[n] in, coef[11 n] out (device pointers).
 int launch_image_poly(const Lens& L, const cx* fixedPoly, const cx* zeta, int n, cx* coef,

@@ -498,7 +498,12 @@ __global__ void __launch_bounds__(64) seed_kernel(SeedArgs a) {
   cx chainv = mk(0.0, 0.0);
   int have = 0, calls = 0;
   if (q > 0) {
-    // fresh solve at the piece's first position
+    // a known start state (the verified speculative chain), else a fresh
+    // solve at the piece's first position
+    if (a.pieceStart) {
+      if (ln < 10) chainv = a.pieceStart[((size_t)a.pieceBase[c] + q) * 10 + ln];
+      have = 1;
+    }
   } else if (a.chainIn) {
     if (ln <= 10) chainv = a.chainIn[c].tempRoots[ln <= 10 ? ln : 0];
     have = a.chainIn[c].have;
@@ -582,6 +587,171 @@ __global__ void __launch_bounds__(64) seed_kernel(SeedArgs a) {
       a.chainOut[c].calls = calls;
     }
   }
+}
+
+// ---- verified speculative chain (SpecArgs) ---------------------------------
+// The chain's solves of position p (centre, then caustic trial points), as
+// seed_kernel runs them, without output.
+__device__ __forceinline__ void chain_step(const SeedArgs& a, const CurveDev& C, const Lens& L,
+                                           int p, cx& chainv, int& have) {
+  const int ln = __lane_id();
+  const int base = a.pool.base[p], nSolve = a.pool.count[p];
+  for (int sIdx = 0; sIdx < nSolve; sIdx++) {
+    const size_t sid = (size_t)base + sIdx;
+    const cx cv = ln <= 10 ? a.pool.coef[sid * 11 + (ln <= 10 ? ln : 0)] : mk(0.0, 0.0);
+    bool oob = false;
+    wl_get_images(L, cv, chainv, have, a.pool.zeta[sid], a.trig, C.binImg, oob);
+  }
+}
+
+// The chain state (roots 0..9 on lanes 0..9) equals the stored one, bit for bit.
+__device__ __forceinline__ bool state_equal(const cx& chainv, const cx* st) {
+  const int ln = __lane_id();
+  bool eq = true;
+  if (ln < 10) {
+    const cx v = st[ln];
+    eq = __double_as_longlong(v.re) == __double_as_longlong(chainv.re) &&
+         __double_as_longlong(v.im) == __double_as_longlong(chainv.im);
+  }
+  return __ballot(!eq) == 0;
+}
+
+__device__ __forceinline__ void piece_range(const CurveDev& C, int q, int& b, int& e) {
+  const long long n = (long long)C.lcLength + 1;
+  b = (int)(n * q / C.split);
+  e = (int)(n * (q + 1) / C.split);
+}
+
+// Round 1: piece q from a fresh solve (piece 0 from the curve's chain state),
+// on into the next piece for sp.overlap positions.
+__global__ void __launch_bounds__(64) spec_round1_kernel(SeedArgs a, SpecArgs sp) {
+  const int c = blockIdx.x / a.maxSplit, q = blockIdx.x % a.maxSplit;
+  if (c >= a.nCurves) return;
+  const int ln = __lane_id();
+  const CurveDev& C = a.curves[c];
+  if (q >= C.split) return;
+  const Lens L = C.L;
+  cx chainv = mk(0.0, 0.0);
+  int have = 0;
+  if (q == 0) {
+    if (ln <= 10) chainv = C.chain[ln <= 10 ? ln : 0];
+    have = C.chainHave;
+  }
+  int b, e;
+  piece_range(C, q, b, e);
+  int stop = e;
+  if (q + 1 < C.split) {
+    int b2, e2;
+    piece_range(C, q + 1, b2, e2);
+    stop = e + sp.overlap < e2 ? e + sp.overlap : e2;
+  }
+  for (int i = b; i < stop; i++) {
+    const int p = C.pointBase + i;
+    chain_step(a, C, L, p, chainv, have);
+    cx* dst = (i < e ? sp.own : sp.ext) + (size_t)p * 10;
+    if (ln < 10) dst[ln] = chainv;
+  }
+}
+
+// Round 2: for a piece q > 0 that does not agree with its predecessor's
+// continuation, the repair chain from the predecessor's end state, through
+// the following pieces until its state equals a piece's own at one of that
+// piece's first sp.overlap positions.
+__global__ void __launch_bounds__(64) spec_repair_kernel(SeedArgs a, SpecArgs sp) {
+  const int c = blockIdx.x / a.maxSplit, q = blockIdx.x % a.maxSplit;
+  if (c >= a.nCurves) return;
+  const int ln = __lane_id();
+  const CurveDev& C = a.curves[c];
+  if (q == 0 || q >= C.split) return;
+  const int gq = sp.pieceBase[c] + q;
+  int b, e;
+  piece_range(C, q, b, e);
+  const int lim = e < b + sp.overlap ? e : b + sp.overlap;
+  bool agree = false;
+  for (int i = b; i < lim && !agree; i++) {
+    const size_t p = (size_t)C.pointBase + i;
+    cx v = mk(0.0, 0.0);
+    if (ln < 10) v = sp.ext[p * 10 + ln];
+    agree = state_equal(v, sp.own + p * 10);
+  }
+  if (ln == 0) sp.merged[gq] = agree ? 1 : 0;
+  if (agree) return;
+  const Lens L = C.L;
+  cx chainv = mk(0.0, 0.0);
+  if (ln < 10) chainv = sp.own[((size_t)C.pointBase + b - 1) * 10 + ln];
+  int have = 1;
+  int qq = q, qb = b, qe = e;  // the piece the chain is in
+  const long long n = (long long)C.lcLength + 1;
+  for (int i = b; i < n; i++) {
+    if (i == qe) {
+      // entering piece qq + 1: the chain's state before its first position
+      qq++;
+      qb = qe;
+      piece_range(C, qq, qb, qe);
+      const int j = qq - q - 1;
+      if (j >= sp.depth) {
+        if (ln == 0) sp.repTarget[gq] = -2;
+        return;
+      }
+      if (ln < 10) sp.repState[((size_t)gq * sp.depth + j) * 10 + ln] = chainv;
+    }
+    const int p = C.pointBase + i;
+    chain_step(a, C, L, p, chainv, have);
+    if (qq > q && i < qb + sp.overlap && state_equal(chainv, sp.own + (size_t)p * 10)) {
+      if (ln == 0) sp.repTarget[gq] = qq;
+      return;
+    }
+  }
+  if (ln == 0) sp.repTarget[gq] = -1;
+}
+
+// The true chain's state before each piece, walking every curve's pieces in
+// order (one lane per curve): the chain known to be the true one at each
+// piece start is either the previous piece's own (it agreed with the chain
+// before it, or it is piece 0) or a repair chain that has not yet agreed.
+__global__ void spec_resolve_kernel(SeedArgs a, SpecArgs sp) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.nCurves) return;
+  const CurveDev& C = a.curves[c];
+  const int g0 = sp.pieceBase[c];
+  bool viaPiece = true;  // the true chain at the coming start is piece q - 1's own
+  int rq = 0;            // else the repair chain of piece rq
+  for (int q = 1; q < C.split; q++) {
+    int b, e;
+    piece_range(C, q, b, e);
+    const cx* src;
+    if (viaPiece) {
+      src = sp.own + ((size_t)C.pointBase + b - 1) * 10;
+      if (!sp.merged[g0 + q]) {
+        viaPiece = false;
+        rq = q;
+      }
+    } else {
+      const int j = q - rq - 1;
+      const int tgt = sp.repTarget[g0 + rq];
+      if (j >= sp.depth || (tgt >= 0 && tgt < q)) {  // cannot happen: the chain stopped earlier
+        atomicExch(sp.failed, 1);
+        return;
+      }
+      src = sp.repState + ((size_t)(g0 + rq) * sp.depth + j) * 10;
+      if (tgt == q) viaPiece = true;
+    }
+    for (int k = 0; k < 10; k++) sp.start[((size_t)g0 + q) * 10 + k] = src[k];
+  }
+}
+
+int launch_spec(const SeedArgs& a, const SpecArgs& sp, hipStream_t s) {
+  if (a.nCurves <= 0) return 0;
+  if (a.maxSplit < 1 || (long long)a.nCurves * a.maxSplit > (1ll << 30))
+    return (int)hipErrorInvalidValue;
+  if (!sp.pieceBase || !sp.own || !sp.ext || !sp.merged || !sp.repState || !sp.repTarget ||
+      !sp.start || !sp.failed || sp.overlap < 1 || sp.depth < 1 || !a.curves || !a.trig ||
+      !a.pool.base || !a.pool.count || !a.pool.coef || !a.pool.zeta)
+    return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(spec_round1_kernel, dim3(a.nCurves * a.maxSplit), dim3(64), 0, s, a, sp);
+  hipLaunchKernelGGL(spec_repair_kernel, dim3(a.nCurves * a.maxSplit), dim3(64), 0, s, a, sp);
+  hipLaunchKernelGGL(spec_resolve_kernel, dim3((a.nCurves + 63) / 64), dim3(64), 0, s, a, sp);
+  return (int)hipGetLastError();
 }
 
 // LightCurveBase::getLC / getPointAmp (lcbase.cc:17-52): positions at
@@ -772,6 +942,7 @@ int launch_seed(const SeedArgs& a, hipStream_t s) {
   if (a.nCurves <= 0) return 0;
   if (a.maxSplit < 1 || (long long)a.nCurves * a.maxSplit > (1ll << 30))
     return (int)hipErrorInvalidValue;
+  if (a.pieceStart && !a.pieceBase) return (int)hipErrorInvalidValue;
   hipLaunchKernelGGL(seed_kernel, dim3(a.nCurves * a.maxSplit), dim3(64), 0, s, a);
   return (int)hipGetLastError();
 }
