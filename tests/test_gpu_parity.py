@@ -182,9 +182,9 @@ def test_speculative_chain_equals_serial_chain(E):
             st = E.last_stats()
             runs[name] = (res, lc.seeds(), st)
             for k in opts:
-                E.set_option(k, {"spec_piece": 32, "spec_overlap": 16, "spec_depth": 64}[k])
+                E.set_option(k, {"spec_piece": 32, "spec_overlap": 16, "spec_depth": 4096}[k])
     finally:
-        for k, v in {"spec_piece": 32, "spec_overlap": 16, "spec_depth": 64}.items():
+        for k, v in {"spec_piece": 32, "spec_overlap": 16, "spec_depth": 4096}.items():
             E.set_option(k, v)
     assert runs["spec"][2]["spec_pieces"] == 4096 // 32 and runs["spec"][2]["spec_fallback"] == 0
     assert runs["serial"][2]["spec_pieces"] == 0

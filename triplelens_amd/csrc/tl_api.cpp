@@ -223,7 +223,9 @@ struct Options {
   int specPiece = 32;       // verified speculative chain: positions per piece (0: the
                             // serial chain in chunks)
   int specOverlap = 16;     // ... positions a piece's chain runs into the next piece
-  int specDepth = 64;       // ... piece boundaries a repair chain may cross
+  int specDepth = 4096;     // ... piece boundaries a repair chain may cross (at most the
+                            // batch's largest piece count is used: a repair chain can then
+                            // always run to its curve's end)
 };
 
 struct Stats {
@@ -293,6 +295,7 @@ struct Engine {
   DBuf<uint32_t> dFlags;
   DBuf<int2> dSeedTasks;
   DBuf<unsigned long long> dHtab, dLCount, dQSeq, dSync, dPtCount;
+  DBuf<unsigned long long> dGrp;  // run-table slices: group bases, then masks
   DBuf<unsigned long long> dLSum, dPtSum;  // split sums (hi * 2^32 + lo): 2 words each
   DBuf<Task> dQBuf;
   DBuf<RootChain> dChain;
@@ -339,7 +342,9 @@ struct Engine {
   bool run_spec(std::vector<CurveDev>& hc, const std::vector<int>& pointBase, SeedArgs& sa);
   void point_lc(Curve& c, cx s, cx e);
   void collect_amoebae(const Job& job, const std::vector<int>& ptCurve,
-                       const std::vector<int>& batchPts, unsigned long long hcap, int labelCap,
+                       const std::vector<int>& batchPts,
+                       const std::vector<unsigned long long>& grpBase,
+                       const std::vector<unsigned long long>& grpCap, int labelCap,
                        std::vector<std::vector<Segment>>& out);
 };
 
@@ -347,8 +352,11 @@ struct Engine {
 // its point: a run counts when its label's class holds the point's seed label
 // (runs that only scouts reached, in components without a seed, do not).
 void Engine::collect_amoebae(const Job& job, const std::vector<int>& ptCurve,
-                             const std::vector<int>& batchPts, unsigned long long hcap,
-                             int labelCap, std::vector<std::vector<Segment>>& out) {
+                             const std::vector<int>& batchPts,
+                             const std::vector<unsigned long long>& grpBase,
+                             const std::vector<unsigned long long>& grpCap, int labelCap,
+                             std::vector<std::vector<Segment>>& out) {
+  const unsigned long long hcap = grpBase.back() + grpCap.back();
   std::vector<unsigned long long> tab(2 * hcap);
   std::vector<int> parent(labelCap);
   HIPCHK(hipMemcpy(tab.data(), dHtab.p, tab.size() * sizeof(unsigned long long),
@@ -359,19 +367,21 @@ void Engine::collect_amoebae(const Job& job, const std::vector<int>& ptCurve,
     return x;
   };
   const int nb = (int)batchPts.size();
-  for (unsigned long long e = 0; e < hcap; e++) {
-    const unsigned long long w0 = tab[2 * e], w1 = tab[2 * e + 1];
-    if (w0 == 0) continue;
-    const int b = (int)(w0 & 0x3FFF);
-    const long long xl = (long long)((w0 >> 14) & 0x1FFFFFF);
-    const long long row = (long long)(w0 >> 39) - 1;
-    const int label = (int)(w1 >> 32);
-    const long long xr = (long long)(w1 & 0xFFFFFFFFull);
-    if (b >= nb) continue;
-    const int gp = batchPts[b];
-    if (!job.curves[ptCurve[gp]]->amoebaOut) continue;
-    if (label < 0 || label >= labelCap || root(label) != root(b)) continue;
-    out[gp].push_back(Segment{row, xl, xr});
+  for (size_t g = 0; g < grpBase.size(); g++) {
+    for (unsigned long long e = grpBase[g]; e < grpBase[g] + grpCap[g]; e++) {
+      const unsigned long long w0 = tab[2 * e], w1 = tab[2 * e + 1];
+      if (w0 == 0) continue;
+      const int b = (int)(g * kGroupPoints + (w0 & 0x3FFF));
+      const long long xl = (long long)((w0 >> 14) & 0x1FFFFFF);
+      const long long row = (long long)(w0 >> 39) - 1;
+      const int label = (int)(w1 >> 32);
+      const long long xr = (long long)(w1 & 0xFFFFFFFFull);
+      if (b >= nb) continue;
+      const int gp = batchPts[b];
+      if (!job.curves[ptCurve[gp]]->amoebaOut) continue;
+      if (label < 0 || label >= labelCap || root(label) != root(b)) continue;
+      out[gp].push_back(Segment{row, xl, xr});
+    }
   }
 }
 
@@ -448,7 +458,8 @@ bool Engine::run_spec(std::vector<CurveDev>& hc, const std::vector<int>& pointBa
   dSpecExt.ensure((size_t)nPts * 10);
   dSpecMerged.ensure(G);
   dSpecTarget.ensure(G);
-  dSpecRep.ensure((size_t)G * opt.specDepth * 10);
+  const int depth = std::max(1, std::min(opt.specDepth, maxP));
+  dSpecRep.ensure((size_t)G * depth * 10);
   dSpecStart.ensure((size_t)G * 10);
   dSpecFailed.ensure(1);
   int zero = 0;
@@ -457,7 +468,7 @@ bool Engine::run_spec(std::vector<CurveDev>& hc, const std::vector<int>& pointBa
   sp.pieceBase = dSpecBase.p;
   sp.nPieces = G;
   sp.overlap = std::max(1, opt.specOverlap);
-  sp.depth = std::max(1, opt.specDepth);
+  sp.depth = depth;
   sp.own = dSpecOwn.p;
   sp.ext = dSpecExt.p;
   sp.merged = dSpecMerged.p;
@@ -734,8 +745,9 @@ void Engine::run(Job& job) {
                           stream));
     HIPCHK(hipStreamSynchronize(stream));
   }
-  // the chunk's positions, largest predicted cost first, in batches of at
-  // most kMaxBatch positions
+  // the chunk's positions, largest predicted cost first, in launches of
+  // about kRunBudget predicted runs (groups of kGroupPoints positions with a
+  // run-table slice each)
   order.clear();
   for (int k = 0; k < nc; k++) {
     const long long len = pointBase[k + 1] - pointBase[k];
@@ -744,21 +756,25 @@ void Engine::run(Job& job) {
       if (nSeeds[pointBase[k] + i] > 0) order.push_back(pointBase[k] + i);
   }
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
-  // A batch holds at most kMaxBatch positions and about kRunBudget predicted
-  // runs (the run table is sized from the prediction).
-  constexpr double kRunBudget = 16.0 * 1024 * 1024;
+  // A launch holds about kRunBudget predicted runs (the run table is sized
+  // from the prediction, tableLoad entries per run: up to 16 GiB).
+  constexpr double kRunBudget = 256.0 * 1024 * 1024;
   size_t stepPts = 0;
   for (size_t b0 = 0; b0 < order.size(); b0 += stepPts) {
     double predicted = 0;
     size_t b1 = b0;
-    while (b1 < order.size() && b1 - b0 < (size_t)kMaxBatch) {
+    std::vector<double> grpPred;
+    while (b1 < order.size()) {
       const double pc = std::max(256.0, (double)cost[order[b1]] / opt.runPixels);
       if (b1 > b0 && predicted + pc > kRunBudget) break;
       predicted += pc;
+      if ((b1 - b0) % kGroupPoints == 0) grpPred.push_back(0.0);
+      grpPred.back() += pc;
       b1++;
     }
     stepPts = b1 - b0;
     const int nb = (int)stepPts;
+    const int nGrp = (int)grpPred.size();
     std::vector<int> batchPts(order.begin() + b0, order.begin() + b1);
     std::vector<int2> seedTasks;
     for (int b = 0; b < nb; b++) {
@@ -772,11 +788,28 @@ void Engine::run(Job& job) {
     HIPCHK(hipMemcpyAsync(dBatchPts.p, batchPts.data(), nb * sizeof(int), hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(dSeedTasks.p, seedTasks.data(), seedTasks.size() * sizeof(int2),
                           hipMemcpyHostToDevice, stream));
-    unsigned long long hcap = 1ull << 16;
-    while ((double)hcap < opt.tableLoad * predicted && hcap < (1ull << 28)) hcap <<= 1;
+    // run-table slice of each group: a power of two of at least tableLoad
+    // entries per predicted run
+    std::vector<unsigned long long> grpCap(nGrp), grpBase(nGrp), grpMask(nGrp);
+    for (int g = 0; g < nGrp; g++) {
+      unsigned long long c = 1ull << 12;
+      while ((double)c < opt.tableLoad * grpPred[g] && c < (1ull << 30)) c <<= 1;
+      grpCap[g] = c;
+    }
     int labelCap = nb + std::max(1 << 16, 16 * nb);
     for (int attempt = 0; attempt < 4; attempt++) {
+      unsigned long long hcap = 0;
+      for (int g = 0; g < nGrp; g++) {
+        grpBase[g] = hcap;
+        grpMask[g] = grpCap[g] - 1;
+        hcap += grpCap[g];
+      }
       dHtab.ensure(2 * hcap);
+      dGrp.ensure(2 * (size_t)nGrp);
+      HIPCHK(hipMemcpyAsync(dGrp.p, grpBase.data(), nGrp * sizeof(unsigned long long),
+                            hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(dGrp.p + nGrp, grpMask.data(), nGrp * sizeof(unsigned long long),
+                            hipMemcpyHostToDevice, stream));
       dLParent.ensure(labelCap);
       dLPoint.ensure(labelCap);
       dLCount.ensure(labelCap);
@@ -792,7 +825,9 @@ void Engine::run(Job& job) {
       fa.nSeedTasks = (int)seedTasks.size();
       fa.seeds = dSeeds.p;
       fa.htab = dHtab.p;
-      fa.hmask = hcap - 1;
+      fa.htabEntries = hcap;
+      fa.grpBase = dGrp.p;
+      fa.grpMask = dGrp.p + nGrp;
       fa.probeLimit = opt.probeLimit;
       fa.lparent = dLParent.p;
       fa.lpoint = dLPoint.p;
@@ -868,11 +903,13 @@ void Engine::run(Job& job) {
       std::vector<uint32_t> bflags(nPts);
       HIPCHK(hipMemcpy(bflags.data(), dFlags.p, nPts * sizeof(uint32_t), hipMemcpyDeviceToHost));
       bool hashFull = false, labelFull = false;
+      std::vector<char> grpFull(nGrp, 0);
       for (int b = 0; b < nb; b++) {
-        hashFull |= (bflags[batchPts[b]] & kFlagHashFull) != 0;
+        const bool hf = (bflags[batchPts[b]] & kFlagHashFull) != 0;
+        hashFull |= hf;
+        if (hf) grpFull[b / kGroupPoints] = 1;
         labelFull |= (bflags[batchPts[b]] & kFlagLabelFull) != 0;
       }
-      (void)hashFull;
       stats.tasks += (double)st[2];
       stats.evals += (double)st[3];
       stats.ins += (double)st[4];
@@ -893,7 +930,8 @@ void Engine::run(Job& job) {
         // search)
         HIPCHK((hipError_t)launch_clear_flag_bits(dFlags.p, dBatchPts.p, nb,
                                                   kFlagHashFull | kFlagLabelFull, stream));
-        if (hashFull) hcap = std::min<unsigned long long>(hcap << 2, 1ull << 29);
+        for (int g = 0; g < nGrp; g++)
+          if (grpFull[g]) grpCap[g] = std::min<unsigned long long>(grpCap[g] << 2, 1ull << 31);
         if (labelFull) labelCap *= 4;
         continue;
       }
@@ -908,7 +946,7 @@ void Engine::run(Job& job) {
         ptSumAll[batchPts[b]] = (double)(long long)sum[2 * (size_t)b + 1] * 4294967296.0 +
                                 (double)sum[2 * (size_t)b];
       }
-      if (wantAmoebae) collect_amoebae(job, ptCurve, batchPts, hcap, labelCap, amoebae);
+      if (wantAmoebae) collect_amoebae(job, ptCurve, batchPts, grpBase, grpCap, labelCap, amoebae);
       break;
     }
   }

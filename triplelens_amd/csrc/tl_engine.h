@@ -10,7 +10,8 @@ namespace tl {
 
 constexpr int kCccLen = 500;       // critical-curve samples per branch (lcirs.cc:16)
 constexpr int kMaxSeeds = 64;      // seeds kept per source position
-constexpr int kMaxBatch = 16384;   // source positions per fill launch (14-bit key field)
+constexpr int kGroupPoints = 16384;  // source positions per run-table group (14-bit key
+                                     // field); a fill launch holds any number of groups
 constexpr int kMaxCover = 16;      // known runs read per row lookup
 constexpr int kFillWaves = 16;     // waves per fill workgroup: 1 manager + 15 workers, four
                                    // waves per SIMD
@@ -165,9 +166,12 @@ struct FillArgs {
   const int2* seedTasks;      // (batch point, seed slot) in processing order
   int nSeedTasks;
   const int2* seeds;
-  // run table
+  // run table: one slice per group of kGroupPoints batch points (b >> 14),
+  // slice g holding grpMask[g] + 1 entries from entry grpBase[g] on
   unsigned long long* htab;   // 2 words per entry
-  unsigned long long hmask;   // capacity - 1
+  unsigned long long htabEntries;  // all slices
+  const unsigned long long* grpBase;
+  const unsigned long long* grpMask;
   int probeLimit;
   // labels
   int* lparent;

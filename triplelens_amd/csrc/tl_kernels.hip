@@ -948,20 +948,26 @@ int launch_seed(const SeedArgs& a, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// Run table: open addressing over 16-byte entries.
-//   word0 = (row+1) << 39 | xL << 14 | b     (0 = empty)
-//   word1 = label << 32 | xR                 (~0 = not yet written)
+// Run table: open addressing over 16-byte entries, one slice per group of
+// kGroupPoints batch points (group b >> 14).
+//   word0 = (row+1) << 39 | xL << 14 | (b & 0x3FFF)   (0 = empty)
+//   word1 = label << 32 | xR                        (~0 = not yet written)
 // Rows and columns are those of the grid (0 <= row, xL < 2^25 - 1; scans
-// never leave the columns [0, N-1]) and b < 2^14.  A maximal run is
-// identified by (b, row, xL): two floods that reach the same run compute the
-// same (xL, xR), and the compare-and-swap on word0 lets exactly one of them
-// record it.
+// never leave the columns [0, N-1]).  A maximal run is identified by (b,
+// row, xL): two floods that reach the same run compute the same (xL, xR),
+// and the compare-and-swap on word0 lets exactly one of them record it.
 // ---------------------------------------------------------------------------
 constexpr unsigned long long kEmpty1 = ~0ull;
 
 __device__ __forceinline__ unsigned long long run_key(int b, int row, int xl) {
   return ((unsigned long long)(row + 1) << 39) | ((unsigned long long)(unsigned)xl << 14) |
-         (unsigned long long)b;
+         (unsigned long long)(b & (kGroupPoints - 1));
+}
+// Entry of probe i from hash h in batch point b's slice.
+__device__ __forceinline__ unsigned long long run_slot(const FillArgs& A, int b,
+                                                       unsigned long long h, int i) {
+  const int g = b >> 14;
+  return A.grpBase[g] + ((h + (unsigned long long)i) & A.grpMask[g]);
 }
 __device__ __forceinline__ unsigned long long hash_run(unsigned long long key) {
   unsigned long long h = key;
@@ -988,7 +994,7 @@ __device__ __forceinline__ int insert_run_slow(const FillArgs& A, int b, int row
         ((unsigned long long)(unsigned)label << 32) | (unsigned long long)(unsigned)xr;
     const unsigned long long h = hash_run(key);
     for (int i = 0; i < A.probeLimit; i++) {
-      const unsigned long long idx = (h + (unsigned long long)i) & A.hmask;
+      const unsigned long long idx = run_slot(A, b, h, i);
       unsigned long long expected = 0;
       if (acas(&A.htab[2 * idx], &expected, key)) {
         ast(&A.htab[2 * idx + 1], w1);
@@ -1017,7 +1023,7 @@ __device__ __forceinline__ int insert_run(const FillArgs& A, int b, int row, int
   int res = -2;
   if (__lane_id() == 0) {
     const unsigned long long key = run_key(b, row, xl);
-    const unsigned long long idx = hash_run(key) & A.hmask;
+    const unsigned long long idx = run_slot(A, b, hash_run(key), 0);
     unsigned long long expected = 0;
     if (acas(&A.htab[2 * idx], &expected, key)) {
       ast(&A.htab[2 * idx + 1],
@@ -2280,7 +2286,7 @@ int launch_clear_u64(unsigned long long* p, unsigned long long n, unsigned long 
 __global__ void fill_reset_kernel(FillArgs A) {
   const unsigned long long tid = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x;
   const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
-  for (unsigned long long i = tid; i <= A.hmask; i += stride) {
+  for (unsigned long long i = tid; i < A.htabEntries; i += stride) {
     A.htab[2 * i] = 0;
     A.htab[2 * i + 1] = kEmpty1;
   }
