@@ -231,7 +231,7 @@ def c3_bench(args):
 
     for _ in range(args.warmup):
         E.batch_irs(objects(), trajs)
-    fill_ms = launches = rays = seed_ms = 0.0
+    fill_ms = launches = rays = seed_ms = builds = 0.0
     flags_all = 0
     wall = 0.0
     for _ in range(args.steps):
@@ -243,6 +243,7 @@ def c3_bench(args):
         fill_ms += st["fill_ms"]
         seed_ms += st["seed_ms"]
         launches += st["fill_launches"]
+        builds += st["graph_builds"]
         for _a, r, f in res:
             rays += float(r.sum())
             flags_all |= int(np.bitwise_or.reduce(f))
@@ -274,6 +275,7 @@ def c3_bench(args):
             "seed_kernel_ms_per_step": seed_ms / args.steps,
             "fill_kernel_ms_per_step": fill_ms / args.steps,
             "fill_launches_per_step": launches / args.steps,
+            "fill_graph_captures_per_step": builds / args.steps,
             "status_flags": flags_all,
         },
         "roofline": {

@@ -189,8 +189,9 @@ int lcirs_batch_fill_positions(int n, void** lcs, const int* counts, const int* 
  * disk samples, out[16] solves (centre and caustic-trial positions),
  * out[17] pieces of the verified speculative chain, out[18] of them not in
  * agreement with their predecessor (repair chains run), out[19] 1 when the
- * speculative chain could not be resolved and the serial chain ran.  out
- * must hold 20 values. */
+ * speculative chain could not be resolved and the serial chain ran, out[20]
+ * fill graphs captured (0 when every fill reused the previous launch's
+ * graph).  out must hold 21 values. */
 int tl_last_stats(double* out);
 
 /* Critical curves and caustics of n geometries on the GPU (one lane each):

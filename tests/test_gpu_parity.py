@@ -381,6 +381,14 @@ def test_graph_launches_keep_results(E):
     finally:
         E.set_option("use_graph", 1)
     assert np.array_equal(ref[1][1], g["rays"])
+    # the same launch again replays the graph captured by the run before
+    # (no new capture), with the engine's own work between the replays
+    c, (A0, r0, _) = cases[1], ref[1]
+    for rep in range(4):
+        A, r, _ = _new(E, c).lc_irs(*c.traj())
+        assert np.array_equal(r, r0) and np.array_equal(A, A0), rep
+        if rep > 0:
+            assert E.last_stats()["graph_builds"] == 0, rep
 
 
 # ------------------------------------------------------------ seeded fills

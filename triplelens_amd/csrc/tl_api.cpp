@@ -232,7 +232,7 @@ struct Stats {
   double tasks = 0, evals = 0, ins = 0, dup = 0, seedMs = 0, fillMs = 0, fillLaunches = 0,
          totalMs = 0, busyMs = 0, insertMs = 0, idlePolls = 0, waves = 0, scouts = 0,
          scoutRuns = 0, trialMs = 0, ringMs = 0, solves = 0, specPieces = 0, specRepairs = 0,
-         specFallback = 0;
+         specFallback = 0, graphBuilds = 0;
 };
 
 // One recorded run of the amoeba print-out: row, leftmost and rightmost pixel.
@@ -308,6 +308,19 @@ struct Engine {
   DBuf<cx> dSpecOwn, dSpecExt, dSpecRep, dSpecStart;
   std::vector<int> specCalls;  // per curve: the chain's solve count after the run
   int specPieces = 0;
+  // the last fill launch's graph and the arguments it was captured with
+  struct FillGraphKey {
+    FillArgs fa;
+    int nWg, labelCap, nb;
+    unsigned long long* ptCount;
+    unsigned long long* ptSum;
+  };
+  hipGraphExec_t fillExec = nullptr;
+  FillGraphKey fillKey;
+  void drop_fill_graph() {
+    if (fillExec) (void)hipGraphExecDestroy(fillExec);
+    fillExec = nullptr;
+  }
   Options opt;
   Stats stats;
 
@@ -858,30 +871,44 @@ void Engine::run(Job& job) {
       fa.leapJumps = opt.leapJumps;
       fa.pollBackoff = opt.pollBackoff;
       if (opt.useGraph) {
-        // the batch's reset, clears, fill and resolve as one graph (kernel
-        // nodes only: the resets are kernels, not memsets); the events
-        // around its launch time all four
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-        HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        int rc = launch_fill_reset(fa, stream);
-        if (!rc) rc = launch_clear_u64(dPtCount.p, nb, 0, stream);
-        if (!rc) rc = launch_clear_u64(dPtSum.p, 2 * (unsigned long long)nb, 0, stream);
-        if (!rc) rc = launch_fill(fa, nWg, stream);
-        if (!rc) rc = launch_resolve(fa, labelCap, dPtCount.p, dPtSum.p, stream);
-        const hipError_t ec = hipStreamEndCapture(stream, &graph);
-        if ((rc != 0 || ec != hipSuccess) && graph) hipGraphDestroy(graph);
-        HIPCHK((hipError_t)rc);
-        HIPCHK(ec);
-        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        if (ei != hipSuccess) hipGraphDestroy(graph);
-        HIPCHK(ei);
+        // the launch's reset, clears, fill and resolve as one graph (kernel
+        // nodes only: the resets are kernels, not memsets), captured once
+        // and launched again while its arguments are the same (the next
+        // step of a batch of the same sizes); the events around its launch
+        // time all five
+        FillGraphKey key;
+        std::memset(&key, 0, sizeof(key));
+        key.fa = fa;
+        key.nWg = nWg;
+        key.labelCap = labelCap;
+        key.nb = nb;
+        key.ptCount = dPtCount.p;
+        key.ptSum = dPtSum.p;
+        if (!fillExec || std::memcmp(&key, &fillKey, sizeof(key)) != 0) {
+          drop_fill_graph();
+          hipGraph_t graph = nullptr;
+          HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+          int rc = launch_fill_reset(fa, stream);
+          if (!rc) rc = launch_clear_u64(dPtCount.p, nb, 0, stream);
+          if (!rc) rc = launch_clear_u64(dPtSum.p, 2 * (unsigned long long)nb, 0, stream);
+          if (!rc) rc = launch_fill(fa, nWg, stream);
+          if (!rc) rc = launch_resolve(fa, labelCap, dPtCount.p, dPtSum.p, stream);
+          const hipError_t ec = hipStreamEndCapture(stream, &graph);
+          if ((rc != 0 || ec != hipSuccess) && graph) (void)hipGraphDestroy(graph);
+          HIPCHK((hipError_t)rc);
+          HIPCHK(ec);
+          const hipError_t ei = hipGraphInstantiate(&fillExec, graph, nullptr, nullptr, 0);
+          (void)hipGraphDestroy(graph);
+          if (ei != hipSuccess) fillExec = nullptr;
+          HIPCHK(ei);
+          fillKey = key;
+          stats.graphBuilds += 1;
+        }
         HIPCHK(hipEventRecord(ev0, stream));
-        const hipError_t el = hipGraphLaunch(exec, stream);
+        const hipError_t el = hipGraphLaunch(fillExec, stream);
         if (el == hipSuccess) HIPCHK(hipEventRecord(ev1, stream));
         const hipError_t es = el == hipSuccess ? hipStreamSynchronize(stream) : el;
-        hipGraphExecDestroy(exec);
-        hipGraphDestroy(graph);
+        if (es != hipSuccess) drop_fill_graph();
         HIPCHK(es);
       } else {
         HIPCHK((hipError_t)launch_fill_reset(fa, stream));
@@ -1691,7 +1718,7 @@ int tl_last_stats(double* out) {
   out[8] = s.busyMs; out[9] = s.insertMs; out[10] = s.idlePolls; out[11] = s.waves;
   out[12] = s.scouts; out[13] = s.scoutRuns; out[14] = s.trialMs; out[15] = s.ringMs;
   out[16] = s.solves; out[17] = s.specPieces; out[18] = s.specRepairs;
-  out[19] = s.specFallback;
+  out[19] = s.specFallback; out[20] = s.graphBuilds;
   return 0;
 }
 

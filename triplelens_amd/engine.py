@@ -88,11 +88,11 @@ def set_option(name: str, value: float) -> None:
 
 
 def last_stats() -> dict:
-    out = np.zeros(20)
+    out = np.zeros(21)
     load().tl_last_stats(out.ctypes.data)
     keys = ["tasks", "evals", "runs", "dups", "seed_ms", "fill_ms", "fill_launches", "total_ms",
             "busy_ms", "insert_ms", "idle_polls", "waves", "scouts", "scout_runs", "trial_ms",
-            "ring_ms", "solves", "spec_pieces", "spec_repairs", "spec_fallback"]
+            "ring_ms", "solves", "spec_pieces", "spec_repairs", "spec_fallback", "graph_builds"]
     return dict(zip(keys, out.tolist()))
 
 
