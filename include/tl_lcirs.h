@@ -74,6 +74,16 @@ void lcirs_delete(void* lc);
  * root chain, without the last results (no set-up is recomputed).  NULL on
  * error.  No counterpart in the reference. */
 void* lcirs_clone(void* lc);
+/* lcirs_new with the lens's critical curves and caustics given (cc, ca: 6 *
+ * 500 complex values each, branch-major as lcirs_copy_cc_ca and tl_ccc_batch
+ * give them) instead of computed on the host: for callers that build many
+ * geometries at once and compute their critical curves with tl_ccc_batch
+ * (a Metropolis step's proposals).  The object equals lcirs_new's when the
+ * curves are that geometry's.  NULL on error.  No counterpart in the
+ * reference. */
+void* lcirs_new_ccc(double a, double b, double th, double m2, double m3, double sourceSize,
+                    unsigned lcLength, long pointsPerRadius, const double* cc,
+                    const double* ca);
 
 /* Per-point results of the last get_lc_irs, lcLength+1 values each:
  * magnifications (all of them, copy_lc stops at lcLength), ray counts and

@@ -176,3 +176,19 @@ def test_slot_trace_arguments_are_checked(lib):
     lib.lcirs_set_slot_trace.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long]
     assert lib.lcirs_set_slot_trace(lc.h, None, 5) == -1
     assert lib.lcirs_set_slot_trace(None, None, 0) == -1
+
+
+def test_object_with_given_critical_curves(lib):
+    """lcirs_new_ccc with a geometry's own critical curves builds the object
+    lcirs_new builds (grid, caustic boxes, curves); null arrays are refused."""
+    from triplelens_amd import engine as E, workloads as W
+    for c in W.c3()[:6]:
+        a = E.LightCurve(*c.lens(), c.rho, c.lc_len, c.ppr)
+        cc, ca = a.cc_ca()
+        b = E.LightCurve.with_critical_curves(*c.lens(), c.rho, c.lc_len, c.ppr, cc, ca)
+        assert np.array_equal(a.grid(), b.grid())
+        cc2, ca2 = b.cc_ca()
+        assert np.array_equal(cc, cc2) and np.array_equal(ca, ca2)
+    L = E.load()
+    assert L.lcirs_new_ccc(1.0, 0.9, 1.0, 0.1, 0.1, 1e-3, 10, 50, None, None) is None
+    assert b"null critical-curve" in L.tl_last_error()

@@ -126,8 +126,10 @@ void ld_default(Curve& c) {
   c.intensity = 1.0 / (c.ldA + 2.0 / 3.0 * c.ldB);
 }
 
+// cc, ca: the lens's critical curves and caustics (6 * kCccLen each, as
+// critical_curves makes them), or null to compute them here.
 Curve* make_curve(double a, double b, double th, double m2, double m3, double rho,
-                  unsigned lcLength, long ppr) {
+                  unsigned lcLength, long ppr, const cx* cc = nullptr, const cx* ca = nullptr) {
   if (!(rho > 0.0) || !std::isfinite(rho)) throw std::invalid_argument("sourceSize must be > 0");
   if (ppr <= 0) throw std::invalid_argument("pointsPerRadius must be > 0");
   if (lcLength == 0) throw std::invalid_argument("lcLength must be > 0");
@@ -142,7 +144,12 @@ Curve* make_curve(double a, double b, double th, double m2, double m3, double rh
   c->rho2 = rho * rho;
   c->rho2pow = std::pow(rho, 2.0);
   ld_default(*c);
-  critical_curves(c->L, c->cc, c->ca);
+  if (cc && ca) {
+    c->cc.assign(cc, cc + 6 * kCccLen);
+    c->ca.assign(ca, ca + 6 * kCccLen);
+  } else {
+    critical_curves(c->L, c->cc, c->ca);
+  }
   // the binary lens (m3 = 0): irsBinary (lcirs.cc:29-36) and the binary
   // image-count rule, which also holds for a or b zero (imgpoint.cc:64-67);
   // the critical curves stay the triple-lens form with m3 = 0
@@ -1157,6 +1164,18 @@ void* lcirs_new(double a, double b, double th, double m2, double m3, double sour
   Curve* c = nullptr;
   guarded([&] {
     c = make_curve(a, b, th, m2, m3, sourceSize, lcLength, pointsPerRadius);
+    return 0;
+  });
+  return c;
+}
+
+void* lcirs_new_ccc(double a, double b, double th, double m2, double m3, double sourceSize,
+                    unsigned lcLength, long pointsPerRadius, const double* cc, const double* ca) {
+  Curve* c = nullptr;
+  guarded([&] {
+    if (!cc || !ca) throw std::invalid_argument("null critical-curve arrays");
+    c = make_curve(a, b, th, m2, m3, sourceSize, lcLength, pointsPerRadius,
+                   reinterpret_cast<const cx*>(cc), reinterpret_cast<const cx*>(ca));
     return 0;
   });
   return c;

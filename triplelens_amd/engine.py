@@ -46,6 +46,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib = ctypes.CDLL(path)
     lib.lcirs_new.argtypes = [_D] * 6 + [ctypes.c_uint, ctypes.c_long]
     lib.lcirs_new.restype = _P
+    lib.lcirs_new_ccc.argtypes = [_D] * 6 + [ctypes.c_uint, ctypes.c_long, _P, _P]
+    lib.lcirs_new_ccc.restype = _P
     lib.lcirs_delete.argtypes = [_P]
     lib.lcirs_clone.argtypes = [_P]
     lib.lcirs_clone.restype = _P
@@ -113,6 +115,19 @@ class LightCurve:
         if not self.h:
             raise EngineError(last_error() or "lcirs_new failed")
         self.lc_len = int(lc_len)
+
+    @classmethod
+    def with_critical_curves(cls, a, b, th, m2, m3, rho, lc_len, ppr, cc, ca):
+        """lcirs_new_ccc: the object with the critical curves and caustics
+        given (one geometry's (6, 500) complex arrays, as ccc_batch gives
+        them) instead of computed on the host."""
+        lib = load()
+        cc = np.ascontiguousarray(cc, np.complex128).reshape(6, 500)
+        ca = np.ascontiguousarray(ca, np.complex128).reshape(6, 500)
+        h = lib.lcirs_new_ccc(a, b, th, m2, m3, rho, lc_len, ppr, cc.ctypes.data, ca.ctypes.data)
+        if not h:
+            raise EngineError(last_error() or "lcirs_new_ccc failed")
+        return cls(0, 0, 0, 0, 0, 0, lc_len, 0, _handle=h)
 
     def clone(self):
         """A new object with this one's set-up, options and root chain

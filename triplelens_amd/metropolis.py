@@ -66,9 +66,23 @@ def _new(c: Curve):
     return lc
 
 
+def _new_many(cs: list[Curve]):
+    """Objects of many geometries, their critical curves computed together
+    on the GPU (tl_ccc_batch) instead of one by one on the host."""
+    if not cs:
+        return []
+    cc, ca = E.ccc_batch(np.array([c.lens() for c in cs]))
+    out = []
+    for c, k1, k2 in zip(cs, cc, ca):
+        lc = E.LightCurve.with_critical_curves(*c.lens(), c.rho, c.lc_len, c.ppr, k1, k2)
+        lc.set_limb_darkening("linear", c.u)
+        out.append(lc)
+    return out
+
+
 def exact_curves(curves: list[Curve]) -> list[np.ndarray]:
     """Plain ray-shot light curves of many geometries in one engine run."""
-    lcs = [_new(c) for c in curves]
+    lcs = _new_many(curves)
     res = E.batch_irs(lcs, np.array([c.traj() for c in curves]))
     out = []
     for amp, _rays, flags in res:
@@ -113,7 +127,7 @@ def run(starts: list[Curve], obs, err, n_props: int, mode: str = "surrogate", su
     def curves_of(cs):
         if mode == "irs":
             return exact_curves(cs)
-        return [a for a, _ in sur.light_curves([_new(c) for c in cs], cs, audit_rng)]
+        return [a for a, _ in sur.light_curves(_new_many(cs), cs, audit_rng)]
 
     init = curves_of(starts)
     chains = [Chain(c, a, chi2(a, obs, err), np.random.default_rng([seed, k, 1]),
@@ -131,7 +145,7 @@ def run(starts: list[Curve], obs, err, n_props: int, mode: str = "surrogate", su
             # the step's proposals share each engine run (seed searches,
             # network, fills of the shot positions)
             amps = {}
-            res = sur.light_curves([_new(props[k]) for k in live], [props[k] for k in live],
+            res = sur.light_curves(_new_many([props[k] for k in live]), [props[k] for k in live],
                                    audit_rng) if live else []
             for k, (amp, info) in zip(live, res):
                 amps[k] = amp
