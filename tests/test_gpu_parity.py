@@ -350,12 +350,12 @@ def test_scout_options_keep_results(E):
                 E.set_option(n, v)
             amp, rays, flags = _new(E, c).lc_irs(*c.traj())
             for n in opts:
-                E.set_option(n, {"scout_rows": 8, "scout_retry": 1, "scout_width": 2048}[n])
+                E.set_option(n, {"scout_rows": 8, "scout_retry": 1, "scout_width": 768}[n])
             assert np.array_equal(rays, g["rays"]), opts
             _assert_no_status(E, flags)
             runs.append(amp)
     finally:
-        for n, v in {"scout_rows": 8, "scout_retry": 1, "scout_width": 2048}.items():
+        for n, v in {"scout_rows": 8, "scout_retry": 1, "scout_width": 768}.items():
             E.set_option(n, v)
     for amp in runs[1:]:
         assert np.array_equal(amp, runs[0])

@@ -30,10 +30,10 @@ from . import workloads as W
 DEFAULT_SPACE = {
     "fill_workgroups": [0, 512],
     "scout_jump": [256, 512, 1024],
-    "scout_width": [0, 1024, 2048],
+    "scout_width": [0, 512, 768, 2048],
     "run_pixels": [64, 128],
 }
-DEFAULTS = {"fill_workgroups": 0, "scout_jump": 512, "scout_width": 2048, "run_pixels": 128,
+DEFAULTS = {"fill_workgroups": 0, "scout_jump": 512, "scout_width": 768, "run_pixels": 128,
             "queue_log2": 21}
 
 

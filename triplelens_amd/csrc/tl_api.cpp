@@ -214,7 +214,7 @@ struct Options {
   int queueLog2 = 21;
   int scoutRows = 8;
   int scoutJump = 512;
-  int scoutWidth = 2048;    // runs wider than this shorten the scout jump (0: never)
+  int scoutWidth = 768;     // runs wider than this shorten the scout jump (0: never)
   int scoutRetry = 1;       // a scout that finds nothing is sent again half as far
   int useGraph = 1;         // each batch's reset, fill and resolve launched as one hipGraph
   int scoutShare = 0;       // sweeps queue their scouts in the shared queue at once
@@ -816,7 +816,10 @@ void Engine::run(Job& job) {
       while ((double)c < opt.tableLoad * grpPred[g] && c < (1ull << 30)) c <<= 1;
       grpCap[g] = c;
     }
-    int labelCap = nb + std::max(1 << 16, 16 * nb);
+    // labels: one per seed class and one per scout run; heavy positions
+    // send many scouts, so the capacity follows the predicted runs as well
+    int labelCap = nb + std::max(std::max(1 << 16, 16 * nb),
+                                 (int)std::min(predicted / 8.0, (double)(1 << 26)));
     for (int attempt = 0; attempt < 4; attempt++) {
       unsigned long long hcap = 0;
       for (int g = 0; g < nGrp; g++) {

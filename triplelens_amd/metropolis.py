@@ -112,12 +112,13 @@ def starts_near(truth: Curve, n: int, spread: float = 0.02, seed: int = 0) -> li
 
 
 def run(starts: list[Curve], obs, err, n_props: int, mode: str = "surrogate", sur=None,
-        step: float = 0.01, seed: int = 0) -> dict:
+        step: float = 0.01, seed: int = 0, progress=None) -> dict:
     """Runs len(starts) chains for n_props proposals each.  Returns the
     chains, the wall time of the proposals, the light-curve points
     evaluated, (surrogate mode) the fallback and audit counts, and the light
     curves of the first proposals by chain (both modes draw the same first
-    proposals from the same starts)."""
+    proposals from the same starts).  progress(step, points, seconds), if
+    given, is called every 16 steps."""
     if mode not in ("surrogate", "irs"):
         raise ValueError(f"unknown mode {mode}")
     if mode == "surrogate" and sur is None:
@@ -165,6 +166,8 @@ def run(starts: list[Curve], obs, err, n_props: int, mode: str = "surrogate", su
                 ch.curve, ch.amp, ch.chi2 = props[k], amps[k], x
                 ch.accepted += 1
             ch.chi2_seq.append(ch.chi2)
+        if progress is not None and (step_i + 1) % 16 == 0:
+            progress(step_i + 1, points, time.perf_counter() - t0)
     wall = time.perf_counter() - t0
     return {"chains": chains, "wall_s": wall, "points": points, "fallback": n_fb,
             "audited": n_aud, "first": first}
