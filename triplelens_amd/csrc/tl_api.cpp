@@ -315,6 +315,26 @@ struct Engine {
   DBuf<cx> dSpecOwn, dSpecExt, dSpecRep, dSpecStart;
   std::vector<int> specCalls;  // per curve: the chain's solve count after the run
   int specPieces = 0;
+  // pinned host staging for the large device-to-host copies (pageable
+  // copies run at a fraction of the link's rate)
+  void* hstage = nullptr;
+  size_t hstageCap = 0;
+  void copy_out(void* dst, const void* dev, size_t bytes) {
+    if (bytes < (64u << 10)) {
+      HIPCHK(hipMemcpy(dst, dev, bytes, hipMemcpyDeviceToHost));
+      return;
+    }
+    constexpr size_t kStage = 16u << 20;
+    if (!hstage) {
+      HIPCHK(hipHostMalloc(&hstage, kStage, hipHostMallocDefault));
+      hstageCap = kStage;
+    }
+    for (size_t off = 0; off < bytes; off += hstageCap) {
+      const size_t n = std::min(hstageCap, bytes - off);
+      HIPCHK(hipMemcpy(hstage, static_cast<const char*>(dev) + off, n, hipMemcpyDeviceToHost));
+      std::memcpy(static_cast<char*>(dst) + off, hstage, n);
+    }
+  }
   // the last fill launch's graph and the arguments it was captured with
   struct FillGraphKey {
     FillArgs fa;
@@ -1020,24 +1040,28 @@ void Engine::run(Job& job) {
   std::vector<double2> cenHost;
   if (job.runSeeds) {
     seedsHost.resize((size_t)nPts * kMaxSeeds);
-    HIPCHK(hipMemcpy(seedsHost.data(), dSeeds.p, seedsHost.size() * sizeof(int2), hipMemcpyDeviceToHost));
+    copy_out(seedsHost.data(), dSeeds.p, seedsHost.size() * sizeof(int2));
     ampHost.resize(nPts);
     cenHost.resize(nPts);
-    HIPCHK(hipMemcpy(ampHost.data(), dPtAmp.p, nPts * sizeof(double), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(cenHost.data(), dPtCen.p, nPts * sizeof(double2), hipMemcpyDeviceToHost));
+    copy_out(ampHost.data(), dPtAmp.p, nPts * sizeof(double));
+    copy_out(cenHost.data(), dPtCen.p, nPts * sizeof(double2));
     causHost.resize(nPts);
-    HIPCHK(hipMemcpy(causHost.data(), dPtCaus.p, nPts * sizeof(double), hipMemcpyDeviceToHost));
+    copy_out(causHost.data(), dPtCaus.p, nPts * sizeof(double));
     pairHost.resize(nPts);
-    HIPCHK(hipMemcpy(pairHost.data(), dPtPair.p, nPts * sizeof(double), hipMemcpyDeviceToHost));
+    copy_out(pairHost.data(), dPtPair.p, nPts * sizeof(double));
     if (anyRing) {
+      // the disk samples and centre roots of the curves that asked for them
       ringHost.resize((size_t)nPts * kRingSamples);
-      HIPCHK(hipMemcpy(ringHost.data(), dRing.p, ringHost.size() * sizeof(double),
-                       hipMemcpyDeviceToHost));
       crossHost.resize(nPts);
-      HIPCHK(hipMemcpy(crossHost.data(), dRingCross.p, nPts * sizeof(int), hipMemcpyDeviceToHost));
       rootsHost.resize((size_t)nPts * 10);
-      HIPCHK(hipMemcpy(rootsHost.data(), dPtRoots.p, rootsHost.size() * sizeof(cx),
-                       hipMemcpyDeviceToHost));
+      for (int k = 0; k < nc; k++) {
+        if (!job.curves[k]->ring) continue;
+        const size_t p0 = pointBase[k], n = pointBase[k + 1] - pointBase[k];
+        copy_out(ringHost.data() + p0 * kRingSamples, dRing.p + p0 * kRingSamples,
+                 n * kRingSamples * sizeof(double));
+        copy_out(crossHost.data() + p0, dRingCross.p + p0, n * sizeof(int));
+        copy_out(rootsHost.data() + p0 * 10, dPtRoots.p + p0 * 10, n * 10 * sizeof(cx));
+      }
     }
   }
   for (int k = 0; k < nc; k++) {

@@ -51,7 +51,7 @@ struct CurveDev {
   int pointBase;               // first global point index
   int causticBase;             // first caustic sample (6*kCccLen per curve)
   int chainHave;               // root-reuse state at the start of the run
-  int ring;                    // compute the disk-sample magnifications (ring_amp_kernel)
+  int ring;                    // compute the disk-sample magnifications (ring_lane_kernel)
   int split;                   // independent pieces of the seed search (1: one chain)
   int binImg;                  // binary image-count rule (a, b or m3 zero)
   int binIrs;                  // binary ray map (m3 = 0)

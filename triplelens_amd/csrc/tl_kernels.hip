@@ -841,9 +841,10 @@ __constant__ double kRingOff[kRingSamples][2] = {
 // residual sits at the acceptance threshold and that come and go between
 // nearby points, while an image pair born on a fold is always bright.
 // The image polynomial at every disk sample (one lane per sample) for
-// ring_amp_kernel: coef[((p * kRingSamples) + s) * 11 + k].
+// ring_lane_kernel: coef[((p * kRingSamples) + s) * 11 + k].
 __global__ void __launch_bounds__(64) ring_poly_kernel(const CurveDev* curves, const int* ptCurve,
-                                                       const double2* ptPos, int nPts, cx* coef) {
+                                                       const double2* ptPos, int nPts, cx* coef,
+                                                       int* cross) {
   const long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (g >= (long long)nPts * kRingSamples) return;
   const int p = (int)(g / kRingSamples), smp = (int)(g % kRingSamples);
@@ -854,45 +855,84 @@ __global__ void __launch_bounds__(64) ring_poly_kernel(const CurveDev* curves, c
   cx cf[11];
   image_poly(C.L, C.fixedPoly, zeta, cf);
   for (int k = 0; k < 11; k++) coef[(size_t)g * 11 + k] = cf[k];
+  if (smp == 0) cross[p] = 0;
 }
 
-__global__ void __launch_bounds__(64) ring_amp_kernel(const CurveDev* curves, const int* ptCurve,
-                                                      const double2* ptPos, const cx* ptRoots,
-                                                      int nPts, const TrigTable* trig,
-                                                      double* ring, int* cross, const cx* coef) {
-  const int p = blockIdx.x;
-  if (p >= nPts) return;
+// Point magnification and significant-image count of one image list, in
+// list order (tlo_ring_amp's significant_images and point_amp): sig counts
+// the images carrying at least kSignificant of the sum.
+__device__ __forceinline__ double lane_amp(const Lens& L, const cx* imgs, int n, int& sig) {
+  double t[10], amp = 0.0;
+  for (int k = 0; k < n; k++) {
+    t[k] = point_amp_term(L, imgs[k]);
+    amp += t[k];
+  }
+  sig = 0;
+  for (int k = 0; k < n; k++) sig += t[k] >= kSignificant * amp ? 1 : 0;
+  return amp;
+}
+
+// getRootsPrecalculated + getImages (imgpoint.cc:160-274) on one lane, from
+// the root-reuse state chain[0..10] (chain[10] is the slot read past the end:
+// 0): the sequence wl_get_images runs on a wavefront, with the serial
+// Laguerre routines of tl_math.h (the same values).  Returns the number of
+// accepted images, in root order, in imgs.
+__device__ __forceinline__ int lane_images(const Lens& L, const cx* cf, cx* chain, cx zeta,
+                                           const TrigTable* trig, bool binImg, cx* imgs) {
+  bool withInit = true, second = false;
+  while (true) {
+    cx roots[10];
+    lag_solve<10>(cf, withInit ? chain : nullptr, roots, trig);
+    if (withInit && !lag_check<10>(cf, roots)) {
+      withInit = false;
+      continue;
+    }
+    for (int k = 0; k < 10; k++) chain[k] = roots[k];
+    chain[10] = mk(0.0, 0.0);
+    int n = 0;
+    for (int k = 0; k < 10; k++)
+      if (img_check(L, roots[k], zeta, 1.0e-6)) imgs[n++] = roots[k];
+    // a valid count: even and at least 4, or for the binary lens odd and at
+    // least 3 (imgpoint.cc:236-237)
+    const bool bad = binImg ? (n % 2 != 1 || n < 3) : (n % 2 != 0 || n < 4);
+    if (!bad || second || !withInit) return n;
+    second = true;
+    withInit = false;
+  }
+}
+
+// One lane per disk sample (position g / kRingSamples, sample g %
+// kRingSamples): the sample's images by a root-reuse solve from the
+// position's own centre roots, its point magnification (root order), and
+// whether its count of significant images differs from the centre's
+// (cross[p], cleared by ring_poly_kernel, counts those samples).
+__global__ void __launch_bounds__(64) ring_lane_kernel(const CurveDev* curves, const int* ptCurve,
+                                                       const double2* ptPos, const cx* ptRoots,
+                                                       int nPts, const TrigTable* trig,
+                                                       double* ring, int* cross, const cx* coef) {
+  const long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (g >= (long long)nPts * kRingSamples) return;
+  const int p = (int)(g / kRingSamples), smp = (int)(g % kRingSamples);
   const CurveDev& C = curves[ptCurve[p]];
   if (!C.ring) return;
-  const int ln = __lane_id();
   const Lens L = C.L;
-  const cx centre = ln < 10 ? ptRoots[(size_t)p * 10 + (ln < 10 ? ln : 0)] : mk(0.0, 0.0);
   const double2 pp = ptPos[p];
-  const bool img0 = ln < 10 && img_check(L, centre, mk(pp.x, pp.y), 1.0e-6);
-  const double term0 = img0 ? point_amp_term(L, centre) : 0.0;
-  const double amp0 = wave_sum_d(term0);
-  const int n0 = __builtin_popcountll(__ballot(img0 && term0 >= kSignificant * amp0));
-  int nCross = 0;
-  for (int s = 0; s < kRingSamples; s++) {
-    cx chainv = centre;
-    int have = 1;
-    const cx zeta = mk(pp.x + C.rho * kRingOff[s][0], pp.y + C.rho * kRingOff[s][1]);
-    const size_t g = (size_t)p * kRingSamples + s;
-    const cx cv = ln <= 10 ? coef[g * 11 + (ln <= 10 ? ln : 0)] : mk(0.0, 0.0);
-    bool oob = false;
-    unsigned long long m = wl_get_images(L, cv, chainv, have, zeta, trig, C.binImg, oob);
-    const bool img = (m >> ln) & 1ull;
-    const double term = img ? point_amp_term(L, chainv) : 0.0;
-    double amp = 0.0;
-    while (m) {
-      const int k = __builtin_ctzll(m);
-      m &= m - 1;
-      amp += rl(term, k);
-    }
-    nCross += __builtin_popcountll(__ballot(img && term >= kSignificant * amp)) != n0 ? 1 : 0;
-    if (ln == 0) ring[(size_t)p * kRingSamples + s] = amp;
+  cx chain[11], imgs[10];
+  int n0 = 0;
+  for (int k = 0; k < 10; k++) {
+    chain[k] = ptRoots[(size_t)p * 10 + k];
+    if (img_check(L, chain[k], mk(pp.x, pp.y), 1.0e-6)) imgs[n0++] = chain[k];
   }
-  if (ln == 0) cross[p] = nCross;
+  chain[10] = mk(0.0, 0.0);
+  int sig0 = 0;
+  lane_amp(L, imgs, n0, sig0);
+  cx cf[11];
+  for (int k = 0; k < 11; k++) cf[k] = coef[(size_t)g * 11 + k];
+  const cx zeta = mk(pp.x + C.rho * kRingOff[smp][0], pp.y + C.rho * kRingOff[smp][1]);
+  const int n = lane_images(L, cf, chain, zeta, trig, C.binImg, imgs);
+  int sig = 0;
+  ring[g] = lane_amp(L, imgs, n, sig);
+  if (sig != sig0) atomicAdd(&cross[p], 1);
 }
 
 int launch_ring_amp(const CurveDev* curves, const int* ptCurve, const double2* ptPos,
@@ -901,9 +941,9 @@ int launch_ring_amp(const CurveDev* curves, const int* ptCurve, const double2* p
   if (nPts <= 0) return 0;
   const long long lanes = (long long)nPts * kRingSamples;
   hipLaunchKernelGGL(ring_poly_kernel, dim3((unsigned)((lanes + 63) / 64)), dim3(64), 0, s, curves,
-                     ptCurve, ptPos, nPts, coef);
-  hipLaunchKernelGGL(ring_amp_kernel, dim3(nPts), dim3(64), 0, s, curves, ptCurve, ptPos, ptRoots,
-                     nPts, trig, ring, cross, coef);
+                     ptCurve, ptPos, nPts, coef, cross);
+  hipLaunchKernelGGL(ring_lane_kernel, dim3((unsigned)((lanes + 63) / 64)), dim3(64), 0, s, curves,
+                     ptCurve, ptPos, ptRoots, nPts, trig, ring, cross, coef);
   return (int)hipGetLastError();
 }
 
