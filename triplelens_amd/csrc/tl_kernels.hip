@@ -1845,23 +1845,27 @@ __device__ __forceinline__ void scan_window(const RowCtx& R, int s, int e, bool 
       }
       if (!last) {
         if (!open && cb + 64 > e) break;  // no later run can meet [s, e]
-        cb += 64;
-        const int x = cb + ln;
-        val = 0;
-        const bool hit = x <= hi && eval_px(R, x, val);
-        m = __ballot(hit);
-        nb = hi - cb + 1 < 64 ? hi - cb + 1 : 64;
-        S.evals += (unsigned long long)nb;
-        pos = 0;
-        const unsigned long long full = nb == 64 ? ~0ull : ((1ull << nb) - 1ull);
-        if (open && m == full) {
-          // the step lies inside the open run
+        // the next steps; while each lies wholly inside the open run it is
+        // taken in without the boundary bookkeeping
+        unsigned evals = 0;
+        bool inside;
+        while (true) {
+          cb += 64;
+          const int x = cb + ln;
+          val = 0;
+          const bool hit = x <= hi && eval_px(R, x, val);
+          m = __ballot(hit);
+          nb = hi - cb + 1 < 64 ? hi - cb + 1 : 64;
+          evals += (unsigned)nb;
+          const unsigned long long full = nb == 64 ? ~0ull : ((1ull << nb) - 1ull);
+          inside = open && m == full;
+          if (!inside) break;
           acc += val;
           cnt += (unsigned long long)nb;
-          pos = nb;
-        } else if (!open && m == 0) {
-          pos = nb;
+          if (cb + 64 > hi) break;
         }
+        S.evals += evals;
+        pos = (inside || (!open && m == 0)) ? nb : 0;
         continue;
       }
     } else if (open) {
