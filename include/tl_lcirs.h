@@ -242,6 +242,13 @@ int tl_image_poly(double a, double b, double th, double m2, double m3, int n, co
  * whose results differ in any bit.  No counterpart in the reference. */
 int tl_math_selftest(int n, unsigned long long seed);
 
+/* GPU self-test of the fill's in-range division (the IEEE double division
+ * sequence without operand pre-scaling, used for the ray test's quotients
+ * m_k / |z - z_k|^2 when both lie in its range) against IEEE division on n
+ * generated operand pairs; returns the number whose results differ in any
+ * bit, or -1 on error.  No counterpart in the reference. */
+int tl_div_selftest(long long n, unsigned long long seed);
+
 /* Engine settings: name/value pairs ("fill_workgroups", "probe_limit",
  * "queue_log2", "scout_rows", "scout_jump", "scout_retry", "scout_width",
  * "scout_share", "share_rays", "use_graph", "leap_every", "leap_jumps", "poll_backoff",

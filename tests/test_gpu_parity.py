@@ -12,6 +12,8 @@ brightness in fixed point, the reference in double in fill order) and within
 """
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -305,6 +307,18 @@ def test_image_polynomial_on_device_equals_host_and_reference(E):
         host = E.image_poly(*lens, z)
         dev = E.image_poly(*lens, z, on_device=True)
         assert np.array_equal(host.view(np.float64), dev.view(np.float64))
+
+
+def test_in_range_division_equals_ieee_division(E):
+    """tl_div_selftest: the fill's in-range quotient (the ray test's
+    m_k / |z - z_k|^2) equals IEEE division to the bit on 2^26 operand pairs
+    from its range, among them quotients half an ulp from a rounding
+    boundary and the ends of the range."""
+    lib = E.load()
+    lib.tl_div_selftest.argtypes = [ctypes.c_longlong, ctypes.c_ulonglong]
+    lib.tl_div_selftest.restype = ctypes.c_int
+    for seed in (1, 0x5DEECE66D):
+        assert lib.tl_div_selftest(1 << 26, seed) == 0
 
 
 def test_full_c2_curve_properties(E, oracle):

@@ -1716,6 +1716,29 @@ int tl_image_poly(double a, double b, double th, double m2, double m3, int n, co
   });
 }
 
+int tl_div_selftest(long long n, unsigned long long seed) {
+  return guarded([&] {
+    if (n < 0) throw std::invalid_argument("bad arguments");
+    Engine& E = engine();
+    std::lock_guard<std::mutex> lock(E.mu);
+    E.init();
+    unsigned long long* d = nullptr;
+    unsigned long long bad = 0;
+    HIPCHK(hipMalloc(&d, sizeof bad));
+    try {
+      HIPCHK(hipMemsetAsync(d, 0, sizeof bad, E.stream));
+      HIPCHK((hipError_t)launch_div_check(n, seed, d, E.stream));
+      HIPCHK(hipMemcpyAsync(&bad, d, sizeof bad, hipMemcpyDeviceToHost, E.stream));
+      HIPCHK(hipStreamSynchronize(E.stream));
+    } catch (...) {
+      hipFree(d);
+      throw;
+    }
+    hipFree(d);
+    return bad > 0x7fffffffull ? 0x7fffffff : (int)bad;
+  });
+}
+
 int tl_math_selftest(int n, unsigned long long seed) {
   // the branch-free complex helpers against the general ones, on the host
   unsigned long long st = seed ? seed : 1;

@@ -219,6 +219,10 @@ int launch_seed(const SeedArgs& a, hipStream_t s);
 // the curves' split set to the pieces: round 1, the repair chains, and the
 // resolution of the piece start states.
 int launch_spec(const SeedArgs& a, const SpecArgs& sp, hipStream_t s);
+// The fill's in-range division against IEEE division on n generated operand
+// pairs (test access): adds the number of differing results to *bad (device).
+int launch_div_check(long long n, unsigned long long seed, unsigned long long* bad,
+                     hipStream_t s);
 // image_poly for n source positions of one lens on the GPU (test access):
 // zeta[n] in, coef[11 n] out (device pointers).
 int launch_image_poly(const Lens& L, const cx* fixedPoly, const cx* zeta, int n, cx* coef,

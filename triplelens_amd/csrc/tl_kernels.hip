@@ -1178,8 +1178,132 @@ struct RowCtx {
   double blx, bly, step, rho2, invRho2, ldA, ldB, sumScale;
   double y, px, py;
   int N;
-  int binIrs;  // binary ray map (irsBinary)
+  int map;      // ray map of the row: kMapInRange, kMapTriple or kMapBinary
+  int inRange;  // the position's lens and grid allow kMapInRange rows
 };
+constexpr int kMapInRange = 0;  // irsSIMD with the in-range division
+constexpr int kMapTriple = 1;   // irsSIMD with IEEE division
+constexpr int kMapBinary = 2;   // irsBinary (m3 = 0)
+
+// a / b rounded to nearest, for 2^-500 <= |a| <= 2^10 and
+// 2^-500 <= |b| <= 2^500: the hardware reciprocal estimate, two
+// Newton-Raphson refinements, the quotient estimate and one correction by
+// the residual a - b q computed exactly with a fused multiply-add
+// (Markstein's correctly rounded division).  This is the IEEE double
+// division sequence without its operand pre-scaling and special-value
+// fix-up, which leave operands and results unchanged in that range (every
+// intermediate value stays normal), so the quotient is the IEEE quotient
+// to the bit.
+__device__ __forceinline__ double div_in_range(double a, double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  double t = fma(-b, r, 1.0);
+  r = fma(r, t, r);
+  t = fma(-b, r, 1.0);
+  r = fma(r, t, r);
+  const double q = a * r;
+  const double e = fma(-b, q, a);
+  return fma(e, r, q);
+}
+
+// irs_ray (the irsSIMD lane order, lcirs.cc:290-341) with its three
+// quotients m_k / |z - z_k|^2 taken by div_in_range: the same value to the
+// bit on rows where every |z - z_k|^2 lies in [2^-500, 2^500] and every
+// mass in [2^-500, 2^10] (row_in_range checks this per row).
+__device__ __forceinline__ double irs_ray_in_range(const Lens& L, double x, double y, double sx,
+                                                   double sy) {
+  double dr3 = L.z3.re - x, dr2 = L.z2.re - x, dr1 = L.z1.re - x, drs = sx - x;
+  double di3 = L.z3.im - y, di2 = L.z2.im - y, di1 = L.z1.im - y, dis = sy - y;
+  double q3 = dr3 * dr3 + di3 * di3;
+  double q2 = dr2 * dr2 + di2 * di2;
+  double q1 = dr1 * dr1 + di1 * di1;
+  double c3 = div_in_range(L.m3, q3), c2 = div_in_range(L.m2, q2), c1 = div_in_range(L.m1, q1);
+  double tr3 = dr3 * c3, tr2 = dr2 * c2, tr1 = dr1 * c1, trs = -drs;
+  double ti3 = di3 * c3, ti2 = di2 * c2, ti1 = di1 * c1, tis = -dis;
+  double r0 = tr3 + tr1, r1 = tr2 + trs;
+  double i0 = ti3 + ti1, i1 = ti2 + tis;
+  return (r0 + r1) * (r0 + r1) + (i0 + i1) * (i0 + i1);
+}
+
+// Self-test of div_in_range against IEEE division on n operand pairs drawn
+// from its range: random sign, exponent and mantissa (one pair in two), the
+// ends of the range, and numerators built as the rounded product of the
+// denominator and a quotient half an ulp above a random double (the
+// quotients nearest a rounding boundary).  bad counts the pairs whose
+// results differ in any bit.
+__global__ void div_check_kernel(long long n, unsigned long long seed, unsigned long long* bad) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  unsigned long long cnt = 0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    unsigned long long z = seed + 0x9E3779B97F4A7C15ull * (unsigned long long)(i + 1);
+    auto next = [&z]() {
+      z += 0x9E3779B97F4A7C15ull;
+      unsigned long long x = z;
+      x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+      x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+      return x ^ (x >> 31);
+    };
+    auto make = [](unsigned long long r, int elo, int ehi) {
+      const unsigned long long mant = r & ((1ull << 52) - 1);
+      const int e = elo + (int)((r >> 52) % (unsigned long long)(ehi - elo + 1));
+      const unsigned long long sign = (r >> 63) << 63;
+      return __longlong_as_double((long long)(sign | ((unsigned long long)(e + 1023) << 52) | mant));
+    };
+    const unsigned long long r0 = next(), r1 = next(), kind = next() & 7;
+    double a, b;
+    if (kind < 4) {
+      a = make(r0, -500, 9);
+      b = make(r1, -500, 499);
+    } else if (kind < 6) {
+      b = make(r1, -480, 480);
+      const double q = make(r0, -15, 15);
+      const double h = ldexp(1.0, ilogb(q) - 53);  // half an ulp of q
+      a = fma(b, h, b * q);
+      if (!(fabs(a) >= 0x1p-500 && fabs(a) < 0x1p10)) a = q;
+    } else {
+      const double ends[6] = {0x1p-500, 0x1.fffffffffffffp9, 1.0, 0x1.0000000000001p-500,
+                              0x1.fffffffffffffp499, 0x1p500};
+      a = ends[r0 % 3] * ((r0 >> 8) & 1 ? -1.0 : 1.0);
+      b = ends[(r1 % 3) * 2 + ((r1 >> 4) & 1)] * ((r1 >> 8) & 1 ? -1.0 : 1.0);
+      if (kind == 7) b = make(r1, -500, 499);
+    }
+    const double x = div_in_range(a, b), y = a / b;
+    cnt += __double_as_longlong(x) != __double_as_longlong(y) ? 1 : 0;
+  }
+  if (cnt) atomicAdd(bad, cnt);
+}
+
+int launch_div_check(long long n, unsigned long long seed, unsigned long long* bad,
+                     hipStream_t s) {
+  if (n <= 0) return 0;
+  if (!bad) return (int)hipErrorInvalidValue;
+  const long long blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(div_check_kernel, dim3((unsigned)(blocks < 4096 ? blocks : 4096)), dim3(256),
+                     0, s, n, seed, bad);
+  return (int)hipGetLastError();
+}
+
+// Whether a position's lens and pixel grid keep every row's squared lens
+// distances below 2^500 and its masses in [2^-500, 2^10]: lens and pixel
+// coordinates at most 2^200 in magnitude.
+__device__ __forceinline__ bool lens_in_range(const Lens& L, double blx, double bly, double step,
+                                              int N) {
+  const double lim = 0x1p200, mlo = 0x1p-500, mhi = 0x1p10;
+  const double ext = step * (double)N;
+  const bool grid = fabs(blx) + ext <= lim && fabs(bly) + ext <= lim;
+  const bool pos = fabs(L.z1.re) <= lim && fabs(L.z1.im) <= lim && fabs(L.z2.re) <= lim &&
+                   fabs(L.z2.im) <= lim && fabs(L.z3.re) <= lim && fabs(L.z3.im) <= lim;
+  const bool mass = L.m1 >= mlo && L.m1 <= mhi && L.m2 >= mlo && L.m2 <= mhi && L.m3 >= mlo &&
+                    L.m3 <= mhi;
+  return grid && pos && mass;
+}
+
+// Whether row y keeps every |z - z_k|^2 at least 2^-500: |y - y_k| >= 2^-250
+// for the three lenses (|z - z_k|^2 >= (y - y_k)^2 for every pixel z of the
+// row, and the rounded squares and sums keep that order).
+__device__ __forceinline__ bool row_in_range(const Lens& L, double y) {
+  const double lo = 0x1p-250;
+  return fabs(L.z1.im - y) >= lo && fabs(L.z2.im - y) >= lo && fabs(L.z3.im - y) >= lo;
+}
 
 // sqrt(a) for 0 <= a <= 1 within a few units in the last place: the
 // hardware reciprocal square root and two Goldschmidt refinements (the
@@ -1203,15 +1327,25 @@ __device__ __forceinline__ double sqrt_unit(double a) {
 // integer multiple by adding 1.5 * 2^52 (exact for |value| below 2^51,
 // either sign); its r^2 / rho^2 and square root are within a few units in
 // the last place of the IEEE operations, far below that rounding.
-__device__ __forceinline__ bool eval_px(const RowCtx& R, int x, long long& val) {
+// MAP is the row's ray map (R.map), fixed at compile time for the scan's
+// inner loop; eval_px dispatches on R.map.
+template <int MAP>
+__device__ __forceinline__ bool eval_px_map(const RowCtx& R, int x, long long& val) {
   const double xx = R.blx + R.step * (double)x;
-  const double rsq = R.binIrs ? irs_binary(R.L, xx, R.y, R.px, R.py)
-                              : irs_ray(R.L, xx, R.y, R.px, R.py);
+  const double rsq = MAP == kMapInRange ? irs_ray_in_range(R.L, xx, R.y, R.px, R.py)
+                     : MAP == kMapTriple ? irs_ray(R.L, xx, R.y, R.px, R.py)
+                                         : irs_binary(R.L, xx, R.y, R.px, R.py);
   if (!(rsq < R.rho2)) return false;
   const double v = R.ldA + R.ldB * sqrt_unit(1.0 - rsq * R.invRho2);
   const double t = fma(v, R.sumScale, 6755399441055744.0);  // 1.5 * 2^52
   val = __double_as_longlong(t) - 0x4338000000000000ll;
   return true;
+}
+
+__device__ __forceinline__ bool eval_px(const RowCtx& R, int x, long long& val) {
+  return R.map == kMapInRange ? eval_px_map<kMapInRange>(R, x, val)
+         : R.map == kMapTriple ? eval_px_map<kMapTriple>(R, x, val)
+                               : eval_px_map<kMapBinary>(R, x, val);
 }
 
 // Follows a run leftwards from column `from` until its first miss or column
@@ -1797,6 +1931,36 @@ __device__ __forceinline__ void handle_run(TaskState& S, int xl, int xr, long lo
   }
 }
 
+// The scan's next 64-pixel steps from column cb + 64 on: each step's hit
+// mask m and pixel count nb; while a step lies wholly inside the open run its
+// pixels are added to acc and cnt and the next step follows (up to column
+// hi).  Returns whether the last step taken lies inside the open run; cb is
+// left at that step's first column and val holds its lane's brightness.
+template <int MAP>
+__device__ __forceinline__ bool scan_steps(const RowCtx& R, bool open, int hi, int& cb,
+                                           unsigned long long& m, int& nb, long long& val,
+                                           long long& acc, unsigned long long& cnt,
+                                           unsigned& evals) {
+  const int ln = __lane_id();
+  bool inside;
+  while (true) {
+    cb += 64;
+    const int x = cb + ln;
+    val = 0;
+    const bool hit = x <= hi && eval_px_map<MAP>(R, x, val);
+    m = __ballot(hit);
+    nb = hi - cb + 1 < 64 ? hi - cb + 1 : 64;
+    evals += (unsigned)nb;
+    const unsigned long long full = nb == 64 ? ~0ull : ((1ull << nb) - 1ull);
+    inside = open && m == full;
+    if (!inside) break;
+    acc += val;
+    cnt += (unsigned long long)nb;
+    if (cb + 64 > hi) break;
+  }
+  return inside;
+}
+
 // Maximal hit runs of the row that meet [s, e].  The row is evaluated over a
 // window that pads [s, e] to whole 64-pixel steps on the sides where runs
 // may continue (extL, extR); elsewhere the pixel next to [s, e] is a known
@@ -1848,22 +2012,12 @@ __device__ __forceinline__ void scan_window(const RowCtx& R, int s, int e, bool 
         // the next steps; while each lies wholly inside the open run it is
         // taken in without the boundary bookkeeping
         unsigned evals = 0;
-        bool inside;
-        while (true) {
-          cb += 64;
-          const int x = cb + ln;
-          val = 0;
-          const bool hit = x <= hi && eval_px(R, x, val);
-          m = __ballot(hit);
-          nb = hi - cb + 1 < 64 ? hi - cb + 1 : 64;
-          evals += (unsigned)nb;
-          const unsigned long long full = nb == 64 ? ~0ull : ((1ull << nb) - 1ull);
-          inside = open && m == full;
-          if (!inside) break;
-          acc += val;
-          cnt += (unsigned long long)nb;
-          if (cb + 64 > hi) break;
-        }
+        const bool inside =
+            R.map == kMapInRange
+                ? scan_steps<kMapInRange>(R, open, hi, cb, m, nb, val, acc, cnt, evals)
+            : R.map == kMapTriple
+                ? scan_steps<kMapTriple>(R, open, hi, cb, m, nb, val, acc, cnt, evals)
+                : scan_steps<kMapBinary>(R, open, hi, cb, m, nb, val, acc, cnt, evals);
         S.evals += evals;
         pos = (inside || (!open && m == 0)) ? nb : 0;
         continue;
@@ -1948,7 +2102,9 @@ __device__ __forceinline__ void load_task(TaskState& S) {
     S.R.ldA = unid(C.ldA);
     S.R.ldB = unid(C.ldB);
     S.R.sumScale = unid(C.sumScale);
-    S.R.binIrs = uni(C.binIrs);
+    const int bin = uni(C.binIrs);
+    S.R.inRange = bin ? 0 : uni((int)lens_in_range(S.R.L, S.R.blx, S.R.bly, S.R.step, S.N));
+    S.R.map = bin ? kMapBinary : kMapTriple;
     S.heavy = A.shareRays > 0.0f && A.ptCost[S.gp] >= A.shareRays;
     S.R.N = S.N;
     const double2 pos = A.ptPos[S.gp];
@@ -1992,7 +2148,11 @@ __device__ __forceinline__ void process_task(TaskState& S) {
     } else if (job == 1 && k1 > k2) {
       continue;  // a task without a known run inside its interval scans once
     }
-    if ((job & 1) == 0) S.R.y = S.R.bly + S.R.step * (double)S.t.row;
+    if ((job & 1) == 0) {
+      S.R.y = S.R.bly + S.R.step * (double)S.t.row;
+      if (S.R.map != kMapBinary)
+        S.R.map = S.R.inRange && uni((int)row_in_range(S.R.L, S.R.y)) ? kMapInRange : kMapTriple;
+    }
     const int lo = S.xa < 0 ? 0 : S.xa, hi = S.xb > S.N - 1 ? S.N - 1 : S.xb;
     int s = lo, e = hi;
     bool extL = true, extR = true;
