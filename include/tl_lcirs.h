@@ -159,6 +159,13 @@ int lcirs_batch_irs(int n, void** lcs, const double* traj);
  * Returns 0 or -1. */
 int lcirs_copy_seeds(void* lc, int* nSeeds, int* pix);
 
+/* lcirs_copy_seeds for the n positions idx[0..n-1] only: nSeeds[i] and the
+ * 64 pixel pairs of row i at pix[2*(i*64 + k) + {0,1}].  Returns 0 or a
+ * negative error (bad index, no seed search).  No counterpart in the
+ * reference: it saves callers that need a few positions' seeds the copy of
+ * the whole table. */
+int lcirs_copy_seeds_at(void* lc, int n, const int* idx, int* nSeeds, int* pix);
+
 /* Fills given seeds on the GPU (no seed search): n source positions pos[2i..],
  * seeds as pixel pairs pix[2*(i*64+k)+{0,1}] for k < nSeeds[i].  Writes ray
  * counts, magnifications and flags per position.  For tests of the fill. */

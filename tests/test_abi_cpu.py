@@ -132,6 +132,8 @@ def test_new_entry_points_reject_bad_arguments(lib):
     idx = np.array([0], np.int32)
     with pytest.raises(E.EngineError, match="no seed search"):
         lc.fill_positions(idx, (np.array([1], np.int32), np.zeros((1, 64, 2), np.int32)))
+    with pytest.raises(E.EngineError, match="no seed search"):
+        lc.seeds(idx)
     # a clone is a separate object with the original's set-up
     cl = lc.clone()
     assert cl.h != lc.h and np.array_equal(cl.grid(), lc.grid())

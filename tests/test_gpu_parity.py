@@ -446,6 +446,23 @@ def test_fill_from_given_seeds(E, oracle):
         assert abs(amp[k] - b * g[7]) <= AMP_RTOL * max(abs(b * g[7]), 1e-300), k
 
 
+def test_seed_rows_of_chosen_positions(E):
+    """lcirs_copy_seeds_at: the seed rows of chosen positions equal those
+    rows of the whole table (lcirs_copy_seeds); an index outside the
+    positions is rejected."""
+    from triplelens_amd import workloads as W
+    c = W.c1()
+    lc = _new(E, c)
+    lc.find_seeds(*c.traj())
+    ns, pix = lc.seeds()
+    idx = np.array([0, 7, 7, 150, c.lc_len], np.int32)
+    ns2, pix2 = lc.seeds(idx)
+    assert np.array_equal(ns2, ns[idx]) and np.array_equal(pix2, pix[idx])
+    assert lc.seeds(np.zeros(0, np.int32))[0].shape == (0,)
+    with pytest.raises(E.EngineError, match="out of range"):
+        lc.seeds(np.array([c.lc_len + 1], np.int32))
+
+
 def test_fill_seeds_rejects_bad_arguments(E):
     from triplelens_amd import workloads as W
     c = W.c1()

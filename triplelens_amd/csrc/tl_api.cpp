@@ -291,6 +291,7 @@ struct Engine {
   DBuf<cx> dPoolZeta, dPoolCoef, dSlots;
   DBuf<cx> dPolyScratch;  // polynomials of the disk samples / point-source positions
   DBuf<double> dPtCausD2;
+  std::vector<int2> hSeeds;  // host copy of the last seed search's seed table
   DBuf<unsigned long long> dPoolCursor;
   DBuf<int> dNSeeds, dPtCurve, dBatchPts, dLParent, dLPoint;
   DBuf<double2> dPtPos;
@@ -1033,7 +1034,7 @@ void Engine::run(Job& job) {
   for (uint32_t f : flags) allFlags |= f;
 
   // results back into the objects
-  std::vector<int2> seedsHost;
+  std::vector<int2>& seedsHost = hSeeds;  // kept: a fresh 8 MB buffer per run costs page faults
   std::vector<double> ampHost, causHost, pairHost, ringHost;
   std::vector<cx> rootsHost;
   std::vector<int> crossHost;
@@ -1078,9 +1079,8 @@ void Engine::run(Job& job) {
     }
     if (job.runSeeds) {
       c.nSeeds.assign(nSeeds.begin() + pointBase[k], nSeeds.begin() + pointBase[k + 1]);
-      c.seedPix.assign((size_t)n * kMaxSeeds * 2, 0);
-      std::memcpy(c.seedPix.data(), seedsHost.data() + (size_t)pointBase[k] * kMaxSeeds,
-                  (size_t)n * kMaxSeeds * sizeof(int2));
+      const int* sp = reinterpret_cast<const int*>(seedsHost.data() + (size_t)pointBase[k] * kMaxSeeds);
+      c.seedPix.assign(sp, sp + (size_t)n * kMaxSeeds * 2);
       if (job.runFill) {
         for (unsigned i = 0; i < c.lcLength && i < (unsigned)n; i++) c.lcVec[i] = c.full[i];
         c.hasIRS = true;
@@ -1422,6 +1422,24 @@ int lcirs_copy_seeds(void* lc, int* nSeeds, int* pix) {
   std::copy(c->nSeeds.begin(), c->nSeeds.end(), nSeeds);
   std::copy(c->seedPix.begin(), c->seedPix.end(), pix);
   return 0;
+}
+
+int lcirs_copy_seeds_at(void* lc, int n, const int* idx, int* nSeeds, int* pix) {
+  return guarded([&] {
+    const Curve* c = static_cast<Curve*>(lc);
+    if (!c) throw std::invalid_argument("null light-curve object");
+    if (!c->hasSeeds) throw std::runtime_error("no seed search has run on this object");
+    if (n < 0 || (n > 0 && (!idx || !nSeeds || !pix))) throw std::invalid_argument("bad arguments");
+    const int np = (int)c->nSeeds.size();
+    for (int i = 0; i < n; i++)
+      if (idx[i] < 0 || idx[i] >= np) throw std::invalid_argument("position index out of range");
+    const size_t row = (size_t)kMaxSeeds * 2;
+    for (int i = 0; i < n; i++) {
+      nSeeds[i] = c->nSeeds[idx[i]];
+      std::memcpy(pix + i * row, c->seedPix.data() + (size_t)idx[i] * row, row * sizeof(int));
+    }
+    return 0;
+  });
 }
 
 int lcirs_get_grid(void* lc, double* out) {

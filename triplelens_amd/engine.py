@@ -255,7 +255,17 @@ class LightCurve:
         _check(self._lib.lcirs_copy_cc_ca(self.h, cc.ctypes.data, ca.ctypes.data))
         return cc.reshape(6, 500), ca.reshape(6, 500)
 
-    def seeds(self):
+    def seeds(self, idx=None):
+        """Seed counts (n,) and pixels (n, 64, 2) of the last seed search, at
+        every position, or at positions idx only (lcirs_copy_seeds_at)."""
+        if idx is not None:
+            idx = np.ascontiguousarray(idx, np.int32)
+            ns = np.empty(len(idx), np.int32)
+            pix = np.empty((len(idx), MAX_SEEDS, 2), np.int32)
+            self._lib.lcirs_copy_seeds_at.argtypes = [_P, ctypes.c_int, _P, _P, _P]
+            _check(self._lib.lcirs_copy_seeds_at(self.h, len(idx), idx.ctypes.data,
+                                                 ns.ctypes.data, pix.ctypes.data))
+            return ns, pix
         n = self.lc_len + 1
         ns = np.zeros(n, np.int32)
         pix = np.zeros((n, MAX_SEEDS, 2), np.int32)

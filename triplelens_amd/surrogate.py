@@ -525,11 +525,10 @@ class Surrogate:
             parts.append([A, fallback, audited])
             if shoot.any():
                 idx = np.nonzero(shoot)[0]
-                ns, pix = lc.seeds()
-                sd = (ns[idx], pix[idx])
+                sd = lc.seeds(idx)
                 for cl in clones_of[k]:
-                    ns2, pix2 = cl.seeds()
-                    sd = union_seeds(sd[0], sd[1], ns2[idx], pix2[idx])
+                    ns2, pix2 = cl.seeds(idx)
+                    sd = union_seeds(sd[0], sd[1], ns2, pix2)
                 shots.append(idx)
                 seeds_list.append(sd)
                 shot_k.append(k)
@@ -563,11 +562,10 @@ class Surrogate:
             for k in failed:
                 A, info = out[k]
                 idx = np.nonzero(~(info["fallback"] | info["audited"]))[0]
-                ns, pix = lcs[k].seeds()
-                sd = (ns[idx], pix[idx])
+                sd = lcs[k].seeds(idx)
                 for cl in clones_of[k]:
-                    ns2, pix2 = cl.seeds()
-                    sd = union_seeds(sd[0], sd[1], ns2[idx], pix2[idx])
+                    ns2, pix2 = cl.seeds(idx)
+                    sd = union_seeds(sd[0], sd[1], ns2, pix2)
                 rest.append(idx)
                 rest_seeds.append(sd)
             res = _E.batch_fill_positions([lcs[k] for k in failed], rest,
