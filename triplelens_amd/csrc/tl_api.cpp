@@ -73,7 +73,14 @@ struct Curve {
   std::vector<double> full;    // lcLength + 1 values of the last IRS curve
   std::vector<uint64_t> rays;
   std::vector<uint32_t> flags;
-  std::vector<int> nSeeds, seedPix;
+  std::vector<int> nSeeds, seedPix;  // seedPix: seedStride pixel pairs per position
+  int seedStride = kMaxSeeds;        // the seed search's largest seed count (at least 1)
+  // position p's seed row as 64 pixel pairs (pairs past the stride are 0)
+  void seed_row(int p, int* out) const {
+    const size_t w = (size_t)seedStride * 2;
+    std::memcpy(out, seedPix.data() + (size_t)p * w, w * sizeof(int));
+    std::memset(out + w, 0, ((size_t)kMaxSeeds * 2 - w) * sizeof(int));
+  }
   std::vector<double> pointAmp;   // point-source magnification of the last IRS curve
   cx seedStart{0.0, 0.0}, seedEnd{0.0, 0.0};  // trajectory of the last seed search
   std::vector<double> centroid;   // x, y of the weighted image centroid per position
@@ -334,6 +341,25 @@ struct Engine {
       const size_t n = std::min(hstageCap, bytes - off);
       HIPCHK(hipMemcpy(hstage, static_cast<const char*>(dev) + off, n, hipMemcpyDeviceToHost));
       std::memcpy(static_cast<char*>(dst) + off, hstage, n);
+    }
+  }
+  // rows of `width` bytes, `pitch` bytes apart on the device, packed into dst
+  void copy_out_rows(void* dst, const void* dev, size_t rows, size_t pitch, size_t width) {
+    if (width == pitch) {
+      copy_out(dst, dev, rows * width);
+      return;
+    }
+    constexpr size_t kStage = 16u << 20;
+    if (!hstage) {
+      HIPCHK(hipHostMalloc(&hstage, kStage, hipHostMallocDefault));
+      hstageCap = kStage;
+    }
+    const size_t per = std::max<size_t>(1, hstageCap / width);
+    for (size_t r = 0; r < rows; r += per) {
+      const size_t n = std::min(per, rows - r);
+      HIPCHK(hipMemcpy2D(hstage, width, static_cast<const char*>(dev) + r * pitch, pitch, width, n,
+                         hipMemcpyDeviceToHost));
+      std::memcpy(static_cast<char*>(dst) + r * width, hstage, n * width);
     }
   }
   // the last fill launch's graph and the arguments it was captured with
@@ -1039,9 +1065,12 @@ void Engine::run(Job& job) {
   std::vector<cx> rootsHost;
   std::vector<int> crossHost;
   std::vector<double2> cenHost;
+  int stride = 1;  // seed slots kept per position: the largest seed count
   if (job.runSeeds) {
-    seedsHost.resize((size_t)nPts * kMaxSeeds);
-    copy_out(seedsHost.data(), dSeeds.p, seedsHost.size() * sizeof(int2));
+    for (int p = 0; p < nPts; p++) stride = std::max(stride, std::min(nSeeds[p], kMaxSeeds));
+    seedsHost.resize((size_t)nPts * stride);
+    copy_out_rows(seedsHost.data(), dSeeds.p, nPts, kMaxSeeds * sizeof(int2),
+                  stride * sizeof(int2));
     ampHost.resize(nPts);
     cenHost.resize(nPts);
     copy_out(ampHost.data(), dPtAmp.p, nPts * sizeof(double));
@@ -1079,8 +1108,9 @@ void Engine::run(Job& job) {
     }
     if (job.runSeeds) {
       c.nSeeds.assign(nSeeds.begin() + pointBase[k], nSeeds.begin() + pointBase[k + 1]);
-      const int* sp = reinterpret_cast<const int*>(seedsHost.data() + (size_t)pointBase[k] * kMaxSeeds);
-      c.seedPix.assign(sp, sp + (size_t)n * kMaxSeeds * 2);
+      const int* sp = reinterpret_cast<const int*>(seedsHost.data() + (size_t)pointBase[k] * stride);
+      c.seedPix.assign(sp, sp + (size_t)n * stride * 2);
+      c.seedStride = stride;
       if (job.runFill) {
         for (unsigned i = 0; i < c.lcLength && i < (unsigned)n; i++) c.lcVec[i] = c.full[i];
         c.hasIRS = true;
@@ -1420,7 +1450,7 @@ int lcirs_copy_seeds(void* lc, int* nSeeds, int* pix) {
   const Curve* c = static_cast<Curve*>(lc);
   if (!c || !c->hasSeeds) return -1;
   std::copy(c->nSeeds.begin(), c->nSeeds.end(), nSeeds);
-  std::copy(c->seedPix.begin(), c->seedPix.end(), pix);
+  for (int p = 0; p < (int)c->nSeeds.size(); p++) c->seed_row(p, pix + (size_t)p * kMaxSeeds * 2);
   return 0;
 }
 
@@ -1436,7 +1466,7 @@ int lcirs_copy_seeds_at(void* lc, int n, const int* idx, int* nSeeds, int* pix) 
     const size_t row = (size_t)kMaxSeeds * 2;
     for (int i = 0; i < n; i++) {
       nSeeds[i] = c->nSeeds[idx[i]];
-      std::memcpy(pix + i * row, c->seedPix.data() + (size_t)idx[i] * row, row * sizeof(int));
+      c->seed_row(idx[i], pix + i * row);
     }
     return 0;
   });
@@ -1568,9 +1598,12 @@ int lcirs_batch_fill_positions(int n, void** lcs, const int* counts, const int* 
         const int p = idx[i];
         const cx x = traj_pos(c->seedStart, c->seedEnd, (unsigned)p, c->lcLength);
         job.pos[i] = make_double2(x.re, x.im);
-        const int* src = nSeeds ? &pix[(size_t)i * kMaxSeeds * 2] : &c->seedPix[(size_t)p * kMaxSeeds * 2];
         job.nSeeds[i] = nSeeds ? nSeeds[i] : c->nSeeds[p];
-        std::memcpy(&job.seeds[(size_t)i * kMaxSeeds], src, kMaxSeeds * sizeof(int2));
+        if (nSeeds)
+          std::memcpy(&job.seeds[(size_t)i * kMaxSeeds], &pix[(size_t)i * kMaxSeeds * 2],
+                      kMaxSeeds * sizeof(int2));
+        else
+          c->seed_row(p, reinterpret_cast<int*>(&job.seeds[(size_t)i * kMaxSeeds]));
         job.cost[i] = (float)(c->pointAmp[p] * area);
       }
       off += counts[k];
@@ -1627,8 +1660,7 @@ int lcirs_fill_positions_seeds(void* lc, int n, const int* idx, const int* nSeed
                     kMaxSeeds * sizeof(int2));
       } else {
         job.nSeeds[i] = c->nSeeds[p];
-        std::memcpy(&job.seeds[(size_t)i * kMaxSeeds], &c->seedPix[(size_t)p * kMaxSeeds * 2],
-                    kMaxSeeds * sizeof(int2));
+        c->seed_row(p, reinterpret_cast<int*>(&job.seeds[(size_t)i * kMaxSeeds]));
       }
       // the seed search's prediction, as get_lc_irs orders and sizes by it
       job.cost[i] = (float)(c->pointAmp[p] * area);
