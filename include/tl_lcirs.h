@@ -166,6 +166,16 @@ int lcirs_copy_seeds(void* lc, int* nSeeds, int* pix);
  * the whole table. */
 int lcirs_copy_seeds_at(void* lc, int n, const int* idx, int* nSeeds, int* pix);
 
+/* The union of the seeds of lc and of the nOther objects others[] (same
+ * trajectory length, each after a seed search) at the n positions idx: per
+ * position, lc's seeds in order, then each other object's seeds not seen
+ * before, in order, at most 64; rows as in lcirs_copy_seeds_at.  Returns 0
+ * or a negative error.  No counterpart in the reference: the surrogate's
+ * fallback positions are filled from the union of several split seed
+ * searches. */
+int lcirs_copy_seeds_union(void* lc, int nOther, void** others, int n, const int* idx,
+                           int* nSeeds, int* pix);
+
 /* Fills given seeds on the GPU (no seed search): n source positions pos[2i..],
  * seeds as pixel pairs pix[2*(i*64+k)+{0,1}] for k < nSeeds[i].  Writes ray
  * counts, magnifications and flags per position.  For tests of the fill. */

@@ -463,6 +463,34 @@ def test_seed_rows_of_chosen_positions(E):
         lc.seeds(np.array([c.lc_len + 1], np.int32))
 
 
+def test_seed_union_equals_the_python_union(E):
+    """lcirs_copy_seeds_union over an object and two clones after split seed
+    searches of different piece counts equals surrogate.union_seeds applied
+    in turn to their whole seed tables, at every position."""
+    from triplelens_amd import workloads as W
+    from triplelens_amd.surrogate import union_seeds
+    c = W.c1()
+    lc = _new(E, c)
+    clones = [lc.clone(), lc.clone()]
+    for o, parts in zip([lc] + clones, (5, 4, 7)):
+        o.set_seed_split(parts)
+    E.batch_find_seeds([lc] + clones, np.array([c.traj()] * 3))
+    idx = np.arange(c.lc_len + 1, dtype=np.int32)
+    ns, pix = lc.seeds()
+    for o in clones:
+        ns2, pix2 = o.seeds()
+        ns, pix = union_seeds(ns, pix, ns2, pix2)
+    ns_u, pix_u = lc.seeds_union(clones, idx)
+    assert np.array_equal(ns_u, ns)
+    for i in idx:
+        assert np.array_equal(pix_u[i, :ns[i]], pix[i, :ns[i]])
+    sel = np.array([3, 3, 150], np.int32)
+    ns_s, pix_s = lc.seeds_union(clones, sel)
+    assert np.array_equal(ns_s, ns[sel]) and np.array_equal(pix_s, pix_u[sel])
+    with pytest.raises(E.EngineError, match="out of range"):
+        lc.seeds_union(clones, np.array([-1], np.int32))
+
+
 def test_fill_seeds_rejects_bad_arguments(E):
     from triplelens_amd import workloads as W
     c = W.c1()

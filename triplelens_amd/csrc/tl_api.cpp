@@ -1472,6 +1472,50 @@ int lcirs_copy_seeds_at(void* lc, int n, const int* idx, int* nSeeds, int* pix) 
   });
 }
 
+int lcirs_copy_seeds_union(void* lc, int nOther, void** others, int n, const int* idx,
+                           int* nSeeds, int* pix) {
+  return guarded([&] {
+    const Curve* c = static_cast<Curve*>(lc);
+    if (!c) throw std::invalid_argument("null light-curve object");
+    if (nOther < 0 || (nOther > 0 && !others) || n < 0 || (n > 0 && (!idx || !nSeeds || !pix)))
+      throw std::invalid_argument("bad arguments");
+    std::vector<const Curve*> src{c};
+    for (int k = 0; k < nOther; k++) {
+      if (!others[k]) throw std::invalid_argument("null light-curve object");
+      src.push_back(static_cast<const Curve*>(others[k]));
+    }
+    const size_t np = c->nSeeds.size();
+    for (const Curve* s : src) {
+      if (!s->hasSeeds) throw std::runtime_error("no seed search has run on this object");
+      if (s->nSeeds.size() != np) throw std::invalid_argument("objects of different lengths");
+    }
+    for (int i = 0; i < n; i++)
+      if (idx[i] < 0 || (size_t)idx[i] >= np) throw std::invalid_argument("position index out of range");
+    const size_t row = (size_t)kMaxSeeds * 2;
+    for (int i = 0; i < n; i++) {
+      const int p = idx[i];
+      int* out = pix + i * row;
+      int cnt = 0;
+      for (const Curve* s : src) {
+        const int* sp = s->seedPix.data() + (size_t)p * s->seedStride * 2;
+        const int m = std::min(s->nSeeds[p], s->seedStride);
+        for (int k = 0; k < m && cnt < kMaxSeeds; k++) {
+          const int x = sp[2 * k], y = sp[2 * k + 1];
+          bool seen = false;
+          for (int j = 0; j < cnt && !seen; j++) seen = out[2 * j] == x && out[2 * j + 1] == y;
+          if (seen) continue;
+          out[2 * cnt] = x;
+          out[2 * cnt + 1] = y;
+          cnt++;
+        }
+      }
+      std::memset(out + 2 * cnt, 0, (row - 2 * (size_t)cnt) * sizeof(int));
+      nSeeds[i] = cnt;
+    }
+    return 0;
+  });
+}
+
 int lcirs_get_grid(void* lc, double* out) {
   g_err.clear();
   const Curve* c = static_cast<Curve*>(lc);

@@ -272,6 +272,19 @@ class LightCurve:
         _check(self._lib.lcirs_copy_seeds(self.h, ns.ctypes.data, pix.ctypes.data))
         return ns, pix
 
+    def seeds_union(self, others, idx):
+        """Seed counts and pixels at positions idx of the union of this
+        object's seeds and those of `others` (lcirs_copy_seeds_union): this
+        object's seeds first, then each other's new ones, at most 64."""
+        idx = np.ascontiguousarray(idx, np.int32)
+        ns = np.empty(len(idx), np.int32)
+        pix = np.empty((len(idx), MAX_SEEDS, 2), np.int32)
+        arr = (ctypes.c_void_p * max(1, len(others)))(*[o.h for o in others])
+        self._lib.lcirs_copy_seeds_union.argtypes = [_P, ctypes.c_int, _P, ctypes.c_int, _P, _P, _P]
+        _check(self._lib.lcirs_copy_seeds_union(self.h, len(others), arr, len(idx),
+                                                idx.ctypes.data, ns.ctypes.data, pix.ctypes.data))
+        return ns, pix
+
     def fill_seeds(self, pos, nseeds, pix):
         """GPU fill of given pixel seeds at given source positions."""
         pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 2)

@@ -525,10 +525,7 @@ class Surrogate:
             parts.append([A, fallback, audited])
             if shoot.any():
                 idx = np.nonzero(shoot)[0]
-                sd = lc.seeds(idx)
-                for cl in clones_of[k]:
-                    ns2, pix2 = cl.seeds(idx)
-                    sd = union_seeds(sd[0], sd[1], ns2, pix2)
+                sd = lc.seeds_union(clones_of[k], idx)
                 shots.append(idx)
                 seeds_list.append(sd)
                 shot_k.append(k)
@@ -562,10 +559,7 @@ class Surrogate:
             for k in failed:
                 A, info = out[k]
                 idx = np.nonzero(~(info["fallback"] | info["audited"]))[0]
-                sd = lcs[k].seeds(idx)
-                for cl in clones_of[k]:
-                    ns2, pix2 = cl.seeds(idx)
-                    sd = union_seeds(sd[0], sd[1], ns2, pix2)
+                sd = lcs[k].seeds_union(clones_of[k], idx)
                 rest.append(idx)
                 rest_seeds.append(sd)
             res = _E.batch_fill_positions([lcs[k] for k in failed], rest,
