@@ -270,7 +270,7 @@ int tl_div_selftest(long long n, unsigned long long seed);
  * "queue_log2", "scout_rows", "scout_jump", "scout_retry", "scout_width",
  * "scout_share", "share_rays", "use_graph", "leap_every", "leap_jumps", "poll_backoff",
  * "run_pixels", "table_load", "seed_chunks", "spec_piece", "spec_overlap", "spec_depth",
- * "seed_chunk_min", "time_limit_s").  None changes a result.  Returns 0 or
+ * "seed_chunk_min", "pool_newest", "time_limit_s").  None changes a result.  Returns 0 or
  * -1 for an unknown name.  No counterpart in the reference. */
 int tl_set_option(const char* name, double value);
 

@@ -352,24 +352,26 @@ def test_full_c2_curve_properties(E, oracle):
 
 def test_scout_options_keep_results(E):
     """Scouts only add labels that merge: without them, with the retry of
-    missed scouts off, and with the jump shortened on wide runs, the window
-    around the C2 peak gives the reference's ray counts and the same
-    magnifications."""
+    missed scouts off, with the jump shortened on wide runs and with the
+    pool handing out its oldest task first, the window around the C2 peak
+    gives the reference's ray counts and the same magnifications."""
     g = golden("c2_window.npz")
     c = curve_from_params(g["params"])
     runs = []
+    defaults = {"scout_rows": 8, "scout_retry": 1, "scout_width": 768, "pool_newest": 1}
     try:
-        for opts in [{"scout_rows": 0}, {"scout_retry": 0}, {"scout_width": 0}, {}]:
+        for opts in [{"scout_rows": 0}, {"scout_retry": 0}, {"scout_width": 0},
+                     {"pool_newest": 0}, {}]:
             for n, v in opts.items():
                 E.set_option(n, v)
             amp, rays, flags = _new(E, c).lc_irs(*c.traj())
             for n in opts:
-                E.set_option(n, {"scout_rows": 8, "scout_retry": 1, "scout_width": 768}[n])
+                E.set_option(n, defaults[n])
             assert np.array_equal(rays, g["rays"]), opts
             _assert_no_status(E, flags)
             runs.append(amp)
     finally:
-        for n, v in {"scout_rows": 8, "scout_retry": 1, "scout_width": 768}.items():
+        for n, v in defaults.items():
             E.set_option(n, v)
     for amp in runs[1:]:
         assert np.array_equal(amp, runs[0])

@@ -229,6 +229,7 @@ struct Options {
   int leapEvery = 4;        // every this many scouts a chain sends a leap scout (0: never)
   int leapJumps = 8;        // ... this many jumps ahead
   int pollBackoff = 6;
+  int poolNewest = 1;       // workers take the pool's newest task first
   double runPixels = 128.0;  // predicted rays per run: sizes the run table
   double tableLoad = 4.0;   // run-table slots per predicted run
   double timeLimitS = 60.0;
@@ -927,6 +928,7 @@ void Engine::run(Job& job) {
       fa.leapEvery = opt.leapEvery;
       fa.leapJumps = opt.leapJumps;
       fa.pollBackoff = opt.pollBackoff;
+      fa.poolNewest = opt.poolNewest;
       if (opt.useGraph) {
         // the launch's reset, clears, fill and resolve as one graph (kernel
         // nodes only: the resets are kernels, not memsets), captured once
@@ -1903,6 +1905,7 @@ int tl_set_option(const char* name, double value) {
   else if (n == "share_rays") o.shareRays = std::max(0.0, value);
   else if (n == "leap_jumps") o.leapJumps = std::min(64, std::max(2, (int)value));
   else if (n == "poll_backoff") o.pollBackoff = std::min(6, std::max(0, (int)value));
+  else if (n == "pool_newest") o.poolNewest = value != 0.0 ? 1 : 0;
   else if (n == "run_pixels") o.runPixels = std::min(65536.0, std::max(1.0, value));
   else if (n == "table_load") o.tableLoad = std::min(64.0, std::max(1.5, value));
   else if (n == "time_limit_s") o.timeLimitS = std::max(0.1, value);

@@ -207,6 +207,7 @@ struct FillArgs {
   int leapEvery;              // every this many scouts a chain sends a leap scout (0: never)
   int leapJumps;              // ... this many jumps ahead
   int pollBackoff;            // idle polls of the global queue sleep up to 2^pollBackoff x 1024
+  int poolNewest;             // a worker takes the pool's newest task (else its oldest)
                               // cycles
 };
 
