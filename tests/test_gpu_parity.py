@@ -451,12 +451,14 @@ def test_fill_from_given_seeds(E, oracle):
 def test_seed_rows_of_chosen_positions(E):
     """lcirs_copy_seeds_at: the seed rows of chosen positions equal those
     rows of the whole table (lcirs_copy_seeds); an index outside the
-    positions is rejected."""
+    positions is rejected.  Slots past a position's seed count are zero."""
     from triplelens_amd import workloads as W
     c = W.c1()
     lc = _new(E, c)
     lc.find_seeds(*c.traj())
     ns, pix = lc.seeds()
+    past = np.arange(pix.shape[1])[None, :] >= ns[:, None]
+    assert ns.min() < pix.shape[1] and not pix[past].any()
     idx = np.array([0, 7, 7, 150, c.lc_len], np.int32)
     ns2, pix2 = lc.seeds(idx)
     assert np.array_equal(ns2, ns[idx]) and np.array_equal(pix2, pix[idx])

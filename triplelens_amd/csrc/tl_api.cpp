@@ -76,9 +76,10 @@ struct Curve {
   std::vector<int> nSeeds, seedPix;  // seedPix: seedStride pixel pairs per position
   int seedStride = kMaxSeeds;        // the seed search's largest seed count (at least 1)
   // position p's seed row as 64 pixel pairs (pairs past the stride are 0)
+  // the position's seeds in kMaxSeeds slots, the slots past its count zero
   void seed_row(int p, int* out) const {
-    const size_t w = (size_t)seedStride * 2;
-    std::memcpy(out, seedPix.data() + (size_t)p * w, w * sizeof(int));
+    const size_t w = (size_t)std::min(nSeeds[p], seedStride) * 2;
+    std::memcpy(out, seedPix.data() + (size_t)p * seedStride * 2, w * sizeof(int));
     std::memset(out + w, 0, ((size_t)kMaxSeeds * 2 - w) * sizeof(int));
   }
   std::vector<double> pointAmp;   // point-source magnification of the last IRS curve
