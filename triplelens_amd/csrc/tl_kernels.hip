@@ -1076,6 +1076,7 @@ __device__ __forceinline__ int insert_run(const FillArgs& A, int b, int row, int
   return res;
 }
 
+
 // ---------------------------------------------------------------------------
 // Labels: union-find over flood labels; label b (< nBatch) is the class of
 // the reference seeds of batch point b.  Roots link larger index to smaller.
@@ -1952,6 +1953,35 @@ __device__ __forceinline__ bool scan_steps(const RowCtx& R, bool open, int hi, i
   const int ln = __lane_id();
   bool inside;
   while (true) {
+    if (open && cb + 192 <= hi + 1) {
+      // two whole steps inside the scan: both evaluated before either
+      // ballot (twice the independent work in flight); when the first does
+      // not lie wholly inside the run the second is dropped, and evaluated
+      // again by the caller's next step
+      const int x1 = cb + 64 + ln;
+      long long v1 = 0, v2 = 0;
+      const bool h1 = eval_px_map<MAP>(R, x1, v1);
+      const bool h2 = eval_px_map<MAP>(R, x1 + 64, v2);
+      const unsigned long long m1 = __ballot(h1), m2 = __ballot(h2);
+      evals += 128u;
+      cb += 64;
+      nb = 64;
+      if (m1 != ~0ull) {
+        m = m1;
+        val = v1;
+        return false;
+      }
+      acc += v1;
+      cnt += 64ull;
+      cb += 64;
+      m = m2;
+      val = v2;
+      if (m2 != ~0ull) return false;
+      acc += v2;
+      cnt += 64ull;
+      if (cb + 64 > hi) return true;
+      continue;
+    }
     cb += 64;
     const int x = cb + ln;
     val = 0;
