@@ -1039,16 +1039,15 @@ void Engine::run(Job& job) {
   }
   }  // chunks
   if (job.runSeeds) {
-    HIPCHK(hipStreamSynchronize(sstream));
+    // the seed search's outputs are final once its last launch is done; the
+    // disk samples behind it on the seed stream are waited for after the
+    // other results are copied back
+    HIPCHK(hipEventSynchronize(evSeed[nChunks - 1]));
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, evTrial1, evSeed[nChunks - 1]));
     stats.seedMs = ms;
     HIPCHK(hipEventElapsedTime(&ms, evTrial0, evTrial1));
     stats.trialMs = ms;
-    if (anyRing) {
-      HIPCHK(hipEventElapsedTime(&ms, evRing0, evRing1));
-      stats.ringMs = ms;
-    }
     std::vector<RootChain> chains(nc);
     HIPCHK(hipMemcpy(chains.data(), dChain.p, nc * sizeof(RootChain), hipMemcpyDeviceToHost));
     for (int k = 0; k < nc; k++) {
@@ -1082,20 +1081,6 @@ void Engine::run(Job& job) {
     copy_out(causHost.data(), dPtCaus.p, nPts * sizeof(double));
     pairHost.resize(nPts);
     copy_out(pairHost.data(), dPtPair.p, nPts * sizeof(double));
-    if (anyRing) {
-      // the disk samples and centre roots of the curves that asked for them
-      ringHost.resize((size_t)nPts * kRingSamples);
-      crossHost.resize(nPts);
-      rootsHost.resize((size_t)nPts * 10);
-      for (int k = 0; k < nc; k++) {
-        if (!job.curves[k]->ring) continue;
-        const size_t p0 = pointBase[k], n = pointBase[k + 1] - pointBase[k];
-        copy_out(ringHost.data() + p0 * kRingSamples, dRing.p + p0 * kRingSamples,
-                 n * kRingSamples * sizeof(double));
-        copy_out(crossHost.data() + p0, dRingCross.p + p0, n * sizeof(int));
-        copy_out(rootsHost.data() + p0 * 10, dPtRoots.p + p0 * 10, n * 10 * sizeof(cx));
-      }
-    }
   }
   for (int k = 0; k < nc; k++) {
     Curve& c = *job.curves[k];
@@ -1123,13 +1108,7 @@ void Engine::run(Job& job) {
       c.seedEnd = job.ends[k];
       c.causDist.assign(causHost.begin() + pointBase[k], causHost.begin() + pointBase[k + 1]);
       c.pairShare.assign(pairHost.begin() + pointBase[k], pairHost.begin() + pointBase[k + 1]);
-      if (c.ring) {
-        c.ringAmp.assign(ringHost.begin() + (size_t)pointBase[k] * kRingSamples,
-                         ringHost.begin() + (size_t)pointBase[k + 1] * kRingSamples);
-        c.centreRoots.assign(rootsHost.begin() + (size_t)pointBase[k] * 10,
-                             rootsHost.begin() + (size_t)pointBase[k + 1] * 10);
-        c.ringCross.assign(crossHost.begin() + pointBase[k], crossHost.begin() + pointBase[k + 1]);
-      } else {
+      if (!c.ring) {
         c.ringCross.clear();
         c.ringAmp.clear();
         c.centreRoots.clear();
@@ -1142,6 +1121,30 @@ void Engine::run(Job& job) {
       }
       if (wantAmoebae && c.amoebaOut)
         for (int i = 0; i < n; i++) write_amoeba_file(c, i, amoebae[pointBase[k] + i]);
+    }
+  }
+  // the seed stream's last work: the disk samples and centre roots of the
+  // curves that asked for them
+  HIPCHK(hipStreamSynchronize(sstream));
+  if (anyRing) {
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, evRing0, evRing1));
+    stats.ringMs = ms;
+    ringHost.resize((size_t)nPts * kRingSamples);
+    crossHost.resize(nPts);
+    rootsHost.resize((size_t)nPts * 10);
+    for (int k = 0; k < nc; k++) {
+      Curve& c = *job.curves[k];
+      if (!c.ring) continue;
+      const size_t p0 = pointBase[k], n = pointBase[k + 1] - pointBase[k];
+      copy_out(ringHost.data() + p0 * kRingSamples, dRing.p + p0 * kRingSamples,
+               n * kRingSamples * sizeof(double));
+      copy_out(crossHost.data() + p0, dRingCross.p + p0, n * sizeof(int));
+      copy_out(rootsHost.data() + p0 * 10, dPtRoots.p + p0 * 10, n * 10 * sizeof(cx));
+      c.ringAmp.assign(ringHost.begin() + p0 * kRingSamples,
+                       ringHost.begin() + (p0 + n) * kRingSamples);
+      c.centreRoots.assign(rootsHost.begin() + p0 * 10, rootsHost.begin() + (p0 + n) * 10);
+      c.ringCross.assign(crossHost.begin() + p0, crossHost.begin() + p0 + n);
     }
   }
   stats.totalMs =
