@@ -858,44 +858,165 @@ __global__ void __launch_bounds__(64) ring_poly_kernel(const CurveDev* curves, c
   if (smp == 0) cross[p] = 0;
 }
 
-// Point magnification and significant-image count of one image list, in
-// list order (tlo_ring_amp's significant_images and point_amp): sig counts
-// the images carrying at least kSignificant of the sum.
-__device__ __forceinline__ double lane_amp(const Lens& L, const cx* imgs, int n, int& sig) {
+// The serial Laguerre routines of tl_math.h for one lane with every array in
+// registers: the degree is a template argument, so no array is indexed at
+// run time (tl_math.h's run-time degree puts them in scratch memory, which
+// made the disk samples several times slower).  The same operations in the
+// same order, so the same values.
+// lag_iter (laguerre.cc:18-99) for a polynomial of degree M.
+template <int M>
+__device__ __forceinline__ bool lag_iter_reg(const cx* poly, cx& xp, const TrigTable* trig) {
+  if constexpr (M == 1) {
+    if (cabs_(poly[1]) != 0.0) {
+      xp = cdiv(cneg(poly[0]), poly[1]);
+      return true;
+    }
+    xp = mk(999.0, 999.0);
+    return false;
+  } else {
+    const double frac[8] = {0.5, 0.23, 0.73, 0.13, 0.37, 0.61, 0.89, 1.0};
+    cx x = xp;
+    const double md = (double)M;
+    for (int i = 1; i <= kLagMaxIt; i++) {
+      cx b = poly[M];
+      double err = cabs_(b);
+      const double ax = cabs_(x);
+      cx s = mk(0.0, 0.0), f = mk(0.0, 0.0);
+#pragma unroll
+      for (int j = M - 1; j >= 0; j--) {
+        s = cadd(cmul(x, s), f);
+        f = cadd(cmul(x, f), b);
+        b = cadd(cmul(x, b), poly[j]);
+        err = cabs_(b) + ax * err;
+      }
+      err *= kLagEps;
+      if (cabs_(b) <= err) {
+        xp = x;
+        return true;
+      }
+      cx g = cdiv(f, b);
+      cx h = csub(cmul(g, g), cdiv(cscale(s, 2.0), b));
+      cx rad = cscale(csub(cscale(h, md), cmul(g, g)), md - 1.0);
+      cx sq = csqrt_(rad);
+      cx gp = cadd(g, sq);
+      cx gm = csub(g, sq);
+      if (cabs_(gp) < cabs_(gm)) gp = gm;
+      cx dx;
+      if (cabs_(gp) > 0.0) dx = cdiv_rc(md, gp);
+      else dx = cscale(mk(trig->c[i], trig->s[i]), 1.0 + cabs_(x));
+      cx x1 = csub(x, dx);
+      if (x.re == x1.re && x.im == x1.im) {
+        xp = x;
+        return true;
+      }
+      if (i % kLagMT) x = x1;
+      else x = csub(x, cmul(mk(frac[(i / kLagMT) & 7], 0.0), dx));
+    }
+    xp = x;
+    return false;
+  }
+}
+
+// lag_solve's deflation steps J..1 (laguerre.cc:103-167): the root found at
+// step J goes to roots[DEG - J].
+template <int DEG, int J>
+__device__ __forceinline__ void lag_deflate_reg(cx* tmp, const cx* init, bool withInit, cx* roots,
+                                                const TrigTable* trig) {
+  if constexpr (J >= 1) {
+    cx x = withInit ? init[J] : mk(kLagEps, kLagEps);
+    if (!lag_iter_reg<J>(tmp, x, trig) && withInit) x = mk(kLagEps, kLagEps);
+    roots[DEG - J] = x;
+    cx b = tmp[J];
+#pragma unroll
+    for (int jj = J - 1; jj >= 0; jj--) {
+      cx c = tmp[jj];
+      tmp[jj] = b;
+      b = cadd(cmul(x, b), c);
+    }
+    lag_deflate_reg<DEG, J - 1>(tmp, init, withInit, roots, trig);
+  }
+}
+
+// lag_solve: the deflation, then each root polished on the full polynomial
+// in root order (the roots rotate through roots[0], so one copy of the
+// polishing code serves all of them).
+template <int DEG>
+__device__ __forceinline__ void lag_solve_reg(const cx* coeffs, const cx* init, bool withInit,
+                                              cx* roots, const TrigTable* trig) {
+  cx tmp[DEG + 1];
+#pragma unroll
+  for (int i = 0; i <= DEG; i++) tmp[i] = coeffs[i];
+  lag_deflate_reg<DEG, DEG>(tmp, init, withInit, roots, trig);
+#pragma unroll 1
+  for (int j = 0; j < DEG; j++) {
+    cx x = roots[0];
+    lag_iter_reg<DEG>(coeffs, x, trig);
+#pragma unroll
+    for (int k = 0; k < DEG - 1; k++) roots[k] = roots[k + 1];
+    roots[DEG - 1] = x;
+  }
+}
+
+// lag_check (laguerre.cc:188-198)
+template <int DEG>
+__device__ __forceinline__ bool lag_check_reg(const cx* coeffs, const cx* roots) {
+  cx c0 = coeffs[DEG];
+#pragma unroll
+  for (int j = 0; j < DEG; j++) c0 = cmul(c0, cneg(roots[j]));
+  if (cabs_(coeffs[0]) > 1.0e-8) return (1.0 - cabs_(cdiv(c0, coeffs[0]))) < 1.0e-5;
+  return cabs_(csub(c0, coeffs[0])) < 1.0e-10;
+}
+
+// Point magnification and significant-image count of the roots marked in
+// `mask`, in root order (tlo_ring_amp's significant_images and point_amp):
+// sig counts the images carrying at least kSignificant of the sum.
+__device__ __forceinline__ double lane_amp(const Lens& L, const cx* roots, unsigned mask,
+                                           int& sig) {
   double t[10], amp = 0.0;
-  for (int k = 0; k < n; k++) {
-    t[k] = point_amp_term(L, imgs[k]);
-    amp += t[k];
+#pragma unroll
+  for (int k = 0; k < 10; k++) {
+    t[k] = 0.0;
+    if ((mask >> k) & 1u) {
+      t[k] = point_amp_term(L, roots[k]);
+      amp += t[k];
+    }
   }
   sig = 0;
-  for (int k = 0; k < n; k++) sig += t[k] >= kSignificant * amp ? 1 : 0;
+#pragma unroll
+  for (int k = 0; k < 10; k++) sig += ((mask >> k) & 1u) && t[k] >= kSignificant * amp ? 1 : 0;
   return amp;
 }
 
 // getRootsPrecalculated + getImages (imgpoint.cc:160-274) on one lane, from
 // the root-reuse state chain[0..10] (chain[10] is the slot read past the end:
-// 0): the sequence wl_get_images runs on a wavefront, with the serial
-// Laguerre routines of tl_math.h (the same values).  Returns the number of
-// accepted images, in root order, in imgs.
-__device__ __forceinline__ int lane_images(const Lens& L, const cx* cf, cx* chain, cx zeta,
-                                           const TrigTable* trig, bool binImg, cx* imgs) {
+// 0): the sequence wl_get_images runs on a wavefront, with the Laguerre
+// routines above (the same values).  Leaves the roots in chain[0..9] and
+// returns the mask of the accepted images among them.
+__device__ __forceinline__ unsigned lane_images(const Lens& L, const cx* cf, cx* chain, cx zeta,
+                                                const TrigTable* trig, bool binImg) {
   bool withInit = true, second = false;
   while (true) {
     cx roots[10];
-    lag_solve<10>(cf, withInit ? chain : nullptr, roots, trig);
-    if (withInit && !lag_check<10>(cf, roots)) {
+    lag_solve_reg<10>(cf, chain, withInit, roots, trig);
+    if (withInit && !lag_check_reg<10>(cf, roots)) {
       withInit = false;
       continue;
     }
+#pragma unroll
     for (int k = 0; k < 10; k++) chain[k] = roots[k];
     chain[10] = mk(0.0, 0.0);
+    unsigned mask = 0;
     int n = 0;
+#pragma unroll
     for (int k = 0; k < 10; k++)
-      if (img_check(L, roots[k], zeta, 1.0e-6)) imgs[n++] = roots[k];
+      if (img_check(L, roots[k], zeta, 1.0e-6)) {
+        mask |= 1u << k;
+        n++;
+      }
     // a valid count: even and at least 4, or for the binary lens odd and at
     // least 3 (imgpoint.cc:236-237)
     const bool bad = binImg ? (n % 2 != 1 || n < 3) : (n % 2 != 0 || n < 4);
-    if (!bad || second || !withInit) return n;
+    if (!bad || second || !withInit) return mask;
     second = true;
     withInit = false;
   }
@@ -917,21 +1038,23 @@ __global__ void __launch_bounds__(64) ring_lane_kernel(const CurveDev* curves, c
   if (!C.ring) return;
   const Lens L = C.L;
   const double2 pp = ptPos[p];
-  cx chain[11], imgs[10];
-  int n0 = 0;
+  cx chain[11];
+  unsigned mask0 = 0;
+#pragma unroll
   for (int k = 0; k < 10; k++) {
     chain[k] = ptRoots[(size_t)p * 10 + k];
-    if (img_check(L, chain[k], mk(pp.x, pp.y), 1.0e-6)) imgs[n0++] = chain[k];
+    if (img_check(L, chain[k], mk(pp.x, pp.y), 1.0e-6)) mask0 |= 1u << k;
   }
   chain[10] = mk(0.0, 0.0);
   int sig0 = 0;
-  lane_amp(L, imgs, n0, sig0);
+  lane_amp(L, chain, mask0, sig0);
   cx cf[11];
+#pragma unroll
   for (int k = 0; k < 11; k++) cf[k] = coef[(size_t)g * 11 + k];
   const cx zeta = mk(pp.x + C.rho * kRingOff[smp][0], pp.y + C.rho * kRingOff[smp][1]);
-  const int n = lane_images(L, cf, chain, zeta, trig, C.binImg, imgs);
+  const unsigned mask = lane_images(L, cf, chain, zeta, trig, C.binImg);
   int sig = 0;
-  ring[g] = lane_amp(L, imgs, n, sig);
+  ring[g] = lane_amp(L, chain, mask, sig);
   if (sig != sig0) atomicAdd(&cross[p], 1);
 }
 
