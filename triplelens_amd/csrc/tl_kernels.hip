@@ -140,18 +140,20 @@ __device__ __forceinline__ double smith_part(cx n, cx d, int part) {
 }
 
 // Laguerre::laguerre (laguerre.cc:18-99) for an x held by every lane; the
-// polynomial's coefficient j is lane j of pv.  The Horner recurrence runs on
-// all lanes and lane j keeps b_j; independent scalar steps of the reference
-// run on separate lanes in one instruction sequence: the moduli |b_j|,
-// |p_m| and |x|, the four parts of f/b and 2s/b, the moduli of the two
-// candidate denominators and the two parts of m/g.  Every value is the one
-// the serial code computes.
-__device__ __forceinline__ bool wl_lag(const cx& pv, int m, cx* xp, const TrigTable* trig,
-                                       bool& oob) {
+// polynomial's coefficient j is lane j of pv, and M is its degree.  The
+// coefficients are read once per call (the degree is a template argument,
+// so every lane index is a constant and the loops unroll), not once per
+// iteration.  The Horner recurrence runs on all lanes and lane j keeps b_j;
+// independent scalar steps of the reference run on separate lanes in one
+// instruction sequence: the moduli |b_j|, |p_m| and |x|, the four parts of
+// f/b and 2s/b, the moduli of the two candidate denominators and the two
+// parts of m/g.  Every value is the one the serial code computes.
+template <int M>
+__device__ __forceinline__ bool wl_lag(const cx& pv, cx* xp, const TrigTable* trig, bool& oob) {
   const int ln = __lane_id();
   cx x = *xp;
-  const double md = (double)m;
-  if (m == 1) {
+  const double md = (double)M;
+  if constexpr (M == 1) {
     const cx p0 = rlc(pv, 0), p1 = rlc(pv, 1);
     if (cabs_sel(p1) != 0.0) {
       *xp = cdiv_sel(cneg(p0), p1);
@@ -159,80 +161,101 @@ __device__ __forceinline__ bool wl_lag(const cx& pv, int m, cx* xp, const TrigTa
     }
     *xp = mk(999.0, 999.0);
     return false;
+  } else {
+    cx p[M + 1];
+#pragma unroll
+    for (int j = 0; j <= M; j++) p[j] = rlc(pv, j);
+    for (int i = 1; i <= kLagMaxIt; i++) {
+      const cx pm = p[M];
+      cx b = pm;
+      cx s = mk(0.0, 0.0), f = mk(0.0, 0.0);
+      cx mine = ln == M ? pm : (ln == M + 1 ? x : mk(0.0, 0.0));
+#pragma unroll
+      for (int j = M - 1; j >= 0; j--) {
+        s = cadd(cmul(x, s), f);
+        f = cadd(cmul(x, f), b);
+        b = cadd(cmul(x, b), p[j]);
+        if (ln == j) mine = b;
+      }
+      // lane j < M: |b_j|; lane M: |p_M| (the error bound's start); lane M+1: |x|
+      const double mabs = cabs_sel(mine);
+      const double ax = rl(mabs, M + 1);
+      double err = rl(mabs, M);
+#pragma unroll
+      for (int j = M - 1; j >= 0; j--) err = rl(mabs, j) + ax * err;
+      err *= kLagEps;
+      if (rl(mabs, 0) <= err) {
+        *xp = x;
+        return true;
+      }
+      // g = f / b (lanes 0, 1) and 2s / b (lanes 2, 3)
+      const double qv = smith_part(ln < 2 ? f : cscale(s, 2.0), b, ln & 1);
+      const cx g = mk(rl(qv, 0), rl(qv, 1));
+      const cx h = csub(cmul(g, g), mk(rl(qv, 2), rl(qv, 3)));
+      const cx rad = cscale(csub(cscale(h, md), cmul(g, g)), md - 1.0);
+      const cx sq = csqrt_sel(rad);
+      cx gp = cadd(g, sq);
+      const cx gm = csub(g, sq);
+      // |gp| (lane 0) and |gm| (lane 1)
+      const double av = cabs_sel(ln == 0 ? gp : gm);
+      double agp = rl(av, 0);
+      const double agm = rl(av, 1);
+      if (agp < agm) {
+        gp = gm;
+        agp = agm;
+      }
+      cx dx;
+      if (agp > 0.0) {
+        const double dv = smith_part(mk(md, 0.0), gp, ln & 1);
+        dx = mk(rl(dv, 0), rl(dv, 1));
+      } else {
+        dx = cscale(mk(trig->c[i], trig->s[i]), 1.0 + ax);
+      }
+      cx x1 = csub(x, dx);
+      if (x.re == x1.re && x.im == x1.im) {
+        *xp = x;
+        return true;
+      }
+      if (i % kLagMT) {
+        x = x1;
+      } else {
+        oob |= i / kLagMT >= 8;
+        x = csub(x, cmul(mk(kLagFrac[(i / kLagMT) & 7], 0.0), dx));
+      }
+    }
+    *xp = x;
+    return false;
   }
-  for (int i = 1; i <= kLagMaxIt; i++) {
-    const cx pm = rlc(pv, m);
-    cx b = pm;
-    cx s = mk(0.0, 0.0), f = mk(0.0, 0.0);
-    cx mine = ln == m ? pm : (ln == m + 1 ? x : mk(0.0, 0.0));
-    for (int j = m - 1; j >= 0; j--) {
-      s = cadd(cmul(x, s), f);
-      f = cadd(cmul(x, f), b);
-      b = cadd(cmul(x, b), rlc(pv, j));
-      if (ln == j) mine = b;
-    }
-    // lane j < m: |b_j|; lane m: |p_m| (the error bound's start); lane m+1: |x|
-    const double mabs = cabs_sel(mine);
-    const double ax = rl(mabs, m + 1);
-    double err = rl(mabs, m);
-    for (int j = m - 1; j >= 0; j--) err = rl(mabs, j) + ax * err;
-    err *= kLagEps;
-    if (rl(mabs, 0) <= err) { *xp = x; return true; }
-    // g = f / b (lanes 0, 1) and 2s / b (lanes 2, 3)
-    const double qv = smith_part(ln < 2 ? f : cscale(s, 2.0), b, ln & 1);
-    const cx g = mk(rl(qv, 0), rl(qv, 1));
-    const cx h = csub(cmul(g, g), mk(rl(qv, 2), rl(qv, 3)));
-    const cx rad = cscale(csub(cscale(h, md), cmul(g, g)), md - 1.0);
-    const cx sq = csqrt_sel(rad);
-    cx gp = cadd(g, sq);
-    const cx gm = csub(g, sq);
-    // |gp| (lane 0) and |gm| (lane 1)
-    const double av = cabs_sel(ln == 0 ? gp : gm);
-    double agp = rl(av, 0);
-    const double agm = rl(av, 1);
-    if (agp < agm) {
-      gp = gm;
-      agp = agm;
-    }
-    cx dx;
-    if (agp > 0.0) {
-      const double dv = smith_part(mk(md, 0.0), gp, ln & 1);
-      dx = mk(rl(dv, 0), rl(dv, 1));
-    } else {
-      dx = cscale(mk(trig->c[i], trig->s[i]), 1.0 + ax);
-    }
-    cx x1 = csub(x, dx);
-    if (x.re == x1.re && x.im == x1.im) { *xp = x; return true; }
-    if (i % kLagMT) {
-      x = x1;
-    } else {
-      oob |= i / kLagMT >= 8;
-      x = csub(x, cmul(mk(kLagFrac[(i / kLagMT) & 7], 0.0), dx));
-    }
-  }
-  *xp = x;
-  return false;
 }
 
 // Laguerre::laguerre with a different x on every lane (the polishing step):
-// the coefficients are the lanes of pv, read in the same order by all lanes.
-__device__ __forceinline__ bool wl_lag_lanes(const cx& pv, int m, cx* xp,
-                                             const TrigTable* trig, bool& oob) {
+// the coefficients are the lanes of pv (degree M), read once per call and
+// used in the same order by all lanes.
+template <int M>
+__device__ __forceinline__ bool wl_lag_lanes(const cx& pv, cx* xp, const TrigTable* trig,
+                                             bool& oob) {
   cx x = *xp;
-  const double md = (double)m;
+  const double md = (double)M;
+  cx p[M + 1];
+#pragma unroll
+  for (int j = 0; j <= M; j++) p[j] = rlc(pv, j);
   for (int i = 1; i <= kLagMaxIt; i++) {
-    cx b = rlc(pv, m);
+    cx b = p[M];
     double err = cabs_sel(b);
     const double ax = cabs_sel(x);
     cx s = mk(0.0, 0.0), f = mk(0.0, 0.0);
-    for (int j = m - 1; j >= 0; j--) {
+#pragma unroll
+    for (int j = M - 1; j >= 0; j--) {
       s = cadd(cmul(x, s), f);
       f = cadd(cmul(x, f), b);
-      b = cadd(cmul(x, b), rlc(pv, j));
+      b = cadd(cmul(x, b), p[j]);
       err = cabs_sel(b) + ax * err;
     }
     err *= kLagEps;
-    if (cabs_sel(b) <= err) { *xp = x; return true; }
+    if (cabs_sel(b) <= err) {
+      *xp = x;
+      return true;
+    }
     cx g = cdiv_sel(f, b);
     cx h = csub(cmul(g, g), cdiv_sel(cscale(s, 2.0), b));
     cx rad = cscale(csub(cscale(h, md), cmul(g, g)), md - 1.0);
@@ -244,7 +267,10 @@ __device__ __forceinline__ bool wl_lag_lanes(const cx& pv, int m, cx* xp,
     if (cabs_sel(gp) > 0.0) dx = cdiv_rc_sel(md, gp);
     else dx = cscale(mk(trig->c[i], trig->s[i]), 1.0 + cabs_sel(x));
     cx x1 = csub(x, dx);
-    if (x.re == x1.re && x.im == x1.im) { *xp = x; return true; }
+    if (x.re == x1.re && x.im == x1.im) {
+      *xp = x;
+      return true;
+    }
     if (i % kLagMT) {
       x = x1;
     } else {
@@ -256,29 +282,40 @@ __device__ __forceinline__ bool wl_lag_lanes(const cx& pv, int m, cx* xp,
   return false;
 }
 
+// The deflation steps J..1 of Laguerre::solveRoots (laguerre.cc:103-167):
+// every lane holds the step's root, the deflated coefficients stay on the
+// lanes of tmp, and the root of step J goes to lane 10 - J of rootv.
+template <int J>
+__device__ __forceinline__ void wl_deflate(cx& tmp, cx& rootv, bool init, const cx& initv,
+                                           const TrigTable* trig, bool& oob) {
+  if constexpr (J >= 1) {
+    const int ln = __lane_id();
+    cx x = init ? rlc(initv, J) : mk(kLagEps, kLagEps);
+    if (!wl_lag<J>(tmp, &x, trig, oob) && init) x = mk(kLagEps, kLagEps);
+    if (ln == 10 - J) rootv = x;
+    cx b = rlc(tmp, J);
+#pragma unroll
+    for (int jj = J - 1; jj >= 0; jj--) {
+      const cx c = rlc(tmp, jj);
+      if (ln == jj) tmp = b;
+      b = cadd(cmul(x, b), c);
+    }
+    wl_deflate<J - 1>(tmp, rootv, init, initv, trig, oob);
+  }
+}
+
 // Laguerre::solveRoots for the degree-10 image polynomial (coefficients on
-// lanes 0..10 of cv): deflation with every lane holding the step's root, the
-// deflated coefficients on the lanes of tmp; then lane r polishes root r on
-// the full polynomial.  Returns root r on lane r (r < 10; 0 elsewhere).
-// With init, initv's lane j seeds the search of step j (j = 10..1).
+// lanes 0..10 of cv): the deflation, then lane r polishes root r on the
+// full polynomial.  Returns root r on lane r (r < 10; 0 elsewhere).  With
+// init, initv's lane j seeds the search of step j (j = 10..1).
 __device__ __forceinline__ cx wl_solve10(const cx& cv, bool init, const cx& initv,
                                          const TrigTable* trig, bool& oob) {
   const int ln = __lane_id();
   cx tmp = cv;
   cx rootv = mk(0.0, 0.0);
-  for (int j = 10; j >= 1; j--) {
-    cx x = init ? rlc(initv, j) : mk(kLagEps, kLagEps);
-    if (!wl_lag(tmp, j, &x, trig, oob) && init) x = mk(kLagEps, kLagEps);
-    if (ln == 10 - j) rootv = x;
-    cx b = rlc(tmp, j);
-    for (int jj = j - 1; jj >= 0; jj--) {
-      const cx c = rlc(tmp, jj);
-      if (ln == jj) tmp = b;
-      b = cadd(cmul(x, b), c);
-    }
-  }
+  wl_deflate<10>(tmp, rootv, init, initv, trig, oob);
   bool laneOob = false;
-  if (ln < 10) wl_lag_lanes(cv, 10, &rootv, trig, laneOob);
+  if (ln < 10) wl_lag_lanes<10>(cv, &rootv, trig, laneOob);
   oob |= __ballot(laneOob) != 0;
   return rootv;
 }
