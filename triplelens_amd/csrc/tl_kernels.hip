@@ -262,10 +262,17 @@ __device__ __forceinline__ bool wl_lag_lanes(const cx& pv, cx* xp, const TrigTab
     cx sq = csqrt_sel(rad);
     cx gp = cadd(g, sq);
     cx gm = csub(g, sq);
-    if (cabs_sel(gp) < cabs_sel(gm)) gp = gm;
+    // the moduli are taken once: |gp| after the choice is the chosen one's,
+    // and |x| is ax
+    double agp = cabs_sel(gp);
+    const double agm = cabs_sel(gm);
+    if (agp < agm) {
+      gp = gm;
+      agp = agm;
+    }
     cx dx;
-    if (cabs_sel(gp) > 0.0) dx = cdiv_rc_sel(md, gp);
-    else dx = cscale(mk(trig->c[i], trig->s[i]), 1.0 + cabs_sel(x));
+    if (agp > 0.0) dx = cdiv_rc_sel(md, gp);
+    else dx = cscale(mk(trig->c[i], trig->s[i]), 1.0 + ax);
     cx x1 = csub(x, dx);
     if (x.re == x1.re && x.im == x1.im) {
       *xp = x;
@@ -937,10 +944,16 @@ __device__ __forceinline__ bool lag_iter_reg(const cx* poly, cx& xp, const TrigT
       cx sq = csqrt_(rad);
       cx gp = cadd(g, sq);
       cx gm = csub(g, sq);
-      if (cabs_(gp) < cabs_(gm)) gp = gm;
+      // each modulus taken once (|x| is ax)
+      double agp = cabs_(gp);
+      const double agm = cabs_(gm);
+      if (agp < agm) {
+        gp = gm;
+        agp = agm;
+      }
       cx dx;
-      if (cabs_(gp) > 0.0) dx = cdiv_rc(md, gp);
-      else dx = cscale(mk(trig->c[i], trig->s[i]), 1.0 + cabs_(x));
+      if (agp > 0.0) dx = cdiv_rc(md, gp);
+      else dx = cscale(mk(trig->c[i], trig->s[i]), 1.0 + ax);
       cx x1 = csub(x, dx);
       if (x.re == x1.re && x.im == x1.im) {
         xp = x;
