@@ -214,16 +214,18 @@ class LightCurve:
         self._lib.lcirs_set_seed_split.argtypes = [_P, ctypes.c_int]
         _check(self._lib.lcirs_set_seed_split(self.h, int(parts)))
 
-    def ring_amp(self):
+    def ring_amp(self, with_roots: bool = True):
         """Disk-sample magnifications (n, 12), each position's own roots
-        (n, 10) and the number of samples across a caustic from the centre
-        (n,) of the last seed search, n = lcLength+1."""
+        (n, 10; None without with_roots) and the number of samples across a
+        caustic from the centre (n,) of the last seed search, n =
+        lcLength+1."""
         n = self.lc_len + 1
         ring = np.zeros((n, 12))
-        roots = np.zeros((n, 10), np.complex128)
+        roots = np.zeros((n, 10), np.complex128) if with_roots else None
         cross = np.zeros(n, np.int32)
         self._lib.lcirs_copy_ring_amp.argtypes = [_P, _P, _P, _P]
-        _check(self._lib.lcirs_copy_ring_amp(self.h, ring.ctypes.data, roots.ctypes.data,
+        _check(self._lib.lcirs_copy_ring_amp(self.h, ring.ctypes.data,
+                                             roots.ctypes.data if with_roots else None,
                                              cross.ctypes.data))
         return ring, roots, cross
 

@@ -244,7 +244,7 @@ def _point_features(lc, c):
     """A_point, the feature rows and the disk samples' caustic crossings of
     the last seed search of lc over curve c."""
     ap, cen, caus = lc.point_amp()
-    ring, _, cross = lc.ring_amp()
+    ring, _, cross = lc.ring_amp(with_roots=False)
     X = features(c.lens(), c.rho, c.u, positions(c), cen, caus, ap, lc.pair_share(), ring)
     return ap, X, cross
 
