@@ -169,7 +169,10 @@ __device__ __forceinline__ bool wl_lag(const cx& pv, cx* xp, const TrigTable* tr
       const cx pm = p[M];
       cx b = pm;
       cx s = mk(0.0, 0.0), f = mk(0.0, 0.0);
-      cx mine = ln == M ? pm : (ln == M + 1 ? x : mk(0.0, 0.0));
+      // lanes past M + 1 carry 1 + i, whose modulus (unused) takes the
+      // branch-free path of cabs_sel: with 0 there they would send the wave
+      // through the general routine as well
+      cx mine = ln == M ? pm : (ln == M + 1 ? x : mk(1.0, 1.0));
 #pragma unroll
       for (int j = M - 1; j >= 0; j--) {
         s = cadd(cmul(x, s), f);
