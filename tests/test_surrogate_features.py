@@ -56,6 +56,27 @@ def test_hexadecapole_exact_for_fields_up_to_fourth_order(u):
     assert t4[0] == pytest.approx(c4 * rho ** 4 * x4, rel=1e-9)
 
 
+def test_hexadecapole_means_equal_numpy_means():
+    """The four-sample means are summed in the order numpy's mean sums four
+    columns, so the features (and the gate calibrated on them) keep their
+    values to the bit: checked against the expansion written with
+    ndarray.mean over magnitudes from 1e-5 to 1e5."""
+    rng = np.random.default_rng(7)
+    x2, x4 = S.disk_moments(0.5)
+    for trial in range(20):
+        scale = 10.0 ** rng.integers(-5, 6)
+        amp = scale * (1.0 + rng.random(512))
+        ring = amp[:, None] * (1.0 + rng.normal(0.0, 1e-3, (512, 12)))
+        half = ring[:, 0:4].mean(axis=1) - amp
+        full = ring[:, 4:8].mean(axis=1) - amp
+        diag = ring[:, 8:12].mean(axis=1) - amp
+        a2 = (16.0 * half - full) / 3.0
+        a4 = 0.5 * (full + diag) - a2
+        A_hex, t4 = S.hexadecapole(amp, ring, 0.5)
+        assert np.array_equal(A_hex.view(np.uint64), (amp + a2 * x2 + a4 * x4).view(np.uint64))
+        assert np.array_equal(t4.view(np.uint64), (a4 * x4).view(np.uint64))
+
+
 def _numpy_point_amp(lens, zeta):
     """Images from numpy's roots of the oracle's polynomial (numpy's
     companion-matrix eigenvalues, not Laguerre), kept when they satisfy the
